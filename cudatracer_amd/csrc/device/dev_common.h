@@ -1,0 +1,128 @@
+// Device-side types of the wavefront loop (gfx950).
+//
+// All kernels are compiled with -ffp-contract=off and correctly rounded
+// fp32 division/sqrt, and every vector expression below keeps glm's
+// evaluation order (dot = (x*x' + y*y') + z*z', normalize = v * (1/sqrt(dot)),
+// min(x, y) = (y < x) ? y : x), so the FP32 results equal the CPU oracle's.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../../include/ct_fp.h"
+
+#define CT_DEV __device__ __forceinline__
+
+namespace ctd {
+
+struct f3 {
+    float x, y, z;
+};
+CT_DEV f3 mk3(float a, float b, float c) { return f3{a, b, c}; }
+CT_DEV f3 xyz(float4 v) { return f3{v.x, v.y, v.z}; }
+CT_DEV f3 operator+(f3 a, f3 b) { return f3{a.x + b.x, a.y + b.y, a.z + b.z}; }
+CT_DEV f3 operator-(f3 a, f3 b) { return f3{a.x - b.x, a.y - b.y, a.z - b.z}; }
+CT_DEV f3 operator*(f3 a, f3 b) { return f3{a.x * b.x, a.y * b.y, a.z * b.z}; }
+CT_DEV f3 operator*(f3 a, float s) { return f3{a.x * s, a.y * s, a.z * s}; }
+CT_DEV f3 operator*(float s, f3 a) { return f3{s * a.x, s * a.y, s * a.z}; }
+CT_DEV f3 operator/(f3 a, float s) { return f3{a.x / s, a.y / s, a.z / s}; }
+CT_DEV f3 operator-(f3 a) { return f3{-a.x, -a.y, -a.z}; }
+CT_DEV float dot(f3 a, f3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+CT_DEV f3 cross(f3 x, f3 y) { return f3{x.y * y.z - y.y * x.z, x.z * y.x - y.z * x.x, x.x * y.y - y.x * x.y}; }
+CT_DEV f3 normalize(f3 v) { return v * (1.0f / ct_sqrtf(dot(v, v))); }
+CT_DEV float length(f3 v) { return ct_sqrtf(dot(v, v)); }
+CT_DEV float gmin(float x, float y) { return (y < x) ? y : x; }
+CT_DEV float gmax(float x, float y) { return (x < y) ? y : x; }
+CT_DEV f3 vmax0(f3 v) { return f3{gmax(0.0f, v.x), gmax(0.0f, v.y), gmax(0.0f, v.z)}; }
+CT_DEV bool isnanf_(float x) { return x != x; }
+
+// Path flags (two high bits of the path word).
+enum : uint32_t { LT_SPECULAR = 0u, LT_DIFFUSE = 1u, LT_GLOSSY = 2u };
+constexpr uint32_t kSlotMask = 0x3fffffffu;
+
+// Leaf-ordered triangle record: three float4 per triangle, in prim_indices
+// order so a leaf is a contiguous run.  .w of the first holds the primitive
+// id with bit 31 set on the last triangle of a leaf.
+constexpr uint32_t kLastInLeaf = 0x80000000u;
+
+// Stack entries: interior node index, or kLeafBit | first triangle record.
+constexpr uint32_t kLeafBit = 0x80000000u;
+constexpr int kMaxStack = 128;  // >= 3*32+1 (quad tree of a depth-64 binary tree)
+
+struct DevScene {
+    const float4* tris;     // 3 per triangle (leaf order): {p0, id}, {p1-p0}, {p2-p0}
+    const float4* nodes4;   // 8 per quad node: bminx,bmaxx,bminy,bmaxy,bminz,bmaxz,childs,counts
+    const float4* nodes2;   // 2 per binary node: {min, leftFirst}, {max, count}
+    const float4* fnodes;   // 4 per compressed quad node (64 B), see fast traversal
+    const float4* sh_pos;   // 3 per primitive: v0 (w = material index bits), v1, v2
+    const float4* sh_nrm;   // 4 per primitive: n0, n1, n2, centre normal
+    const float4* mats;     // 2 per material: {albedo, roughness}, {type bits, ior, alpha, 0}
+    const float4* lights;   // 5 per light: v0, v1, v2, centre normal (w = area), emission
+    uint32_t n_tris, n_nodes4, n_nodes2, n_fnodes, n_mats, n_lights;
+    int sky_enabled;
+    float sky_r, sky_g, sky_b;
+};
+
+// One camera view (plain frame, batch member or served request).
+struct ViewDesc {
+    float pos[3], w[3], hor[3], ver[3];
+    float invw, invh;
+    int32_t width, height, spp;
+    uint32_t frame_base;
+    uint64_t accum_offset;  // float4 index of the view's pixel 0 in the accumulator arena
+};
+
+// A run of consecutive path slots of one view inside a chunk.
+struct Segment {
+    uint32_t chunk_offset;  // first slot of the segment in the chunk
+    uint32_t view;
+    uint32_t first_path;    // path index within the view (pixel * spp + sample)
+    uint32_t n;             // paths
+    uint32_t pix_offset;    // first resolve task of the segment
+    uint32_t pad[3];
+};
+
+// Per-chunk sizes, written by the host before the chunk's kernels run (so a
+// captured graph of the bounce loop replays unchanged for every chunk).
+struct ChunkHdr {
+    uint32_t n;        // path slots in the chunk
+    uint32_t n_tasks;  // resolve tasks (pixels touched)
+    uint32_t nseg;     // segments
+    uint32_t pad;
+};
+
+struct TraceOpts {
+    int accel, traversal;
+    float teps;
+};
+
+struct ShadeOpts {
+    int max_depth, rr_start, bsdf_mode, shadow, indirect;
+    float neps, deps, teps;
+};
+
+// Queue buffers of one context (structure of arrays).
+struct Queues {
+    float4* ray_o[2];   // origin.xyz
+    float4* ray_d[2];   // direction.xyz
+    float4* thr[2];     // throughput.xyz, lastBsdfPdf
+    uint32_t* path[2];  // slot | lastBounceType << 30
+    float2* hit;        // t, primitive id bits
+    float4* sh_o;       // shadow origin.xyz, tmax
+    float4* sh_d;       // shadow direction.xyz, slot bits
+    float4* sh_c;       // shadow colour.xyz
+    float4* lpath;      // per-slot radiance (xyz)
+    uint32_t* counters; // see CounterIndex
+    uint32_t* spill;    // traversal stack overflow area
+    uint32_t* err;      // error flags
+};
+
+// counters[] layout: for bounce b in [0, kMaxBounce]
+constexpr int kMaxBounce = 64;
+CT_DEV uint32_t* cnt_active(uint32_t* c, int b) { return c + b; }
+CT_DEV uint32_t* cnt_shadow(uint32_t* c, int b) { return c + (kMaxBounce + 1) + b; }
+CT_DEV uint32_t* cnt_fetch_ext(uint32_t* c, int b) { return c + 2 * (kMaxBounce + 1) + b; }
+CT_DEV uint32_t* cnt_fetch_sh(uint32_t* c, int b) { return c + 3 * (kMaxBounce + 1) + b; }
+constexpr int kCounterWords = 4 * (kMaxBounce + 1) + 64;  // + misc
+
+}  // namespace ctd

@@ -1,0 +1,560 @@
+// Wavefront path-tracing kernels for gfx950 (MI355X).
+//
+// Reference stages (src/CUDA/Kernel.cu):        here:
+//   setGlobals          :75-86                  k_setup (all bounce counters zeroed once per chunk)
+//   generatePrimaryRays :208-234                k_generate
+//   extend              :236-245                k_extend   (persistent, wave-fetched)
+//   shade               :88-206                 k_shade    (wave64 ballot + prefix-sum queue append)
+//   connect             :247-256                k_connect  (persistent, wave-fetched)
+//   accumulation        :100,134,255            k_resolve  (per-path radiance summed in sample order)
+//   draw_framebuffer    :258-270                k_tonemap
+//
+// Path state is structure-of-arrays (float4 rows, 16-byte coalesced loads).
+// Queue lengths live in device counters that the next kernel reads, so the
+// bounce loop needs no host read-back (the reference copies two counters to
+// the host after every bounce, Kernel.cu:309-310,335-336).
+#include "launch.h"
+
+#include "bsdf.h"
+#include "traverse.h"
+
+namespace ctd {
+
+// ------------------------------------------------------------ wave tools --
+CT_DEV uint32_t lane_id() { return __lane_id(); }
+
+// Append for all lanes with pred set: one atomic per wave, offsets by a
+// prefix popcount of the ballot.  Must be reached by every lane of the wave.
+CT_DEV uint32_t wave_append(bool pred, uint32_t* counter) {
+    const uint64_t mask = __ballot(pred ? 1 : 0);
+    if (mask == 0) return 0;
+    const uint32_t lane = lane_id();
+    const int leader = __ffsll((unsigned long long)mask) - 1;
+    uint32_t base = 0;
+    if ((int)lane == leader) base = atomicAdd(counter, (uint32_t)__popcll(mask));
+    base = __shfl(base, leader);
+    return base + (uint32_t)__popcll(mask & ((1ull << lane) - 1ull));
+}
+
+CT_DEV int find_seg(const Segment* __restrict__ segs, int nseg, uint32_t slot) {
+    int lo = 0, hi = nseg - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (segs[mid].chunk_offset <= slot) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+struct SlotInfo {
+    uint32_t pixel, frame, view;
+};
+CT_DEV SlotInfo decode_slot(const ViewDesc* __restrict__ views, const Segment* __restrict__ segs, int nseg,
+                            uint32_t slot) {
+    const Segment sg = segs[find_seg(segs, nseg, slot)];
+    const ViewDesc& v = views[sg.view];
+    const uint32_t local = sg.first_path + (slot - sg.chunk_offset);
+    const uint32_t spp = (uint32_t)v.spp;
+    const uint32_t pixel = local / spp;
+    SlotInfo si;
+    si.pixel = pixel;
+    si.frame = v.frame_base + (local - pixel * spp);
+    si.view = sg.view;
+    return si;
+}
+
+// ---------------------------------------------------------------- setup --
+__global__ void k_setup(uint32_t* counters, const ChunkHdr* hdr) {
+    for (int i = threadIdx.x; i < kCounterWords; i += blockDim.x) counters[i] = 0u;
+    __syncthreads();
+    if (threadIdx.x == 0) *cnt_active(counters, 0) = hdr->n;
+}
+
+// ------------------------------------------------------------- generate --
+// generatePrimaryRays + Ray::generate (Kernel.cu:208-234, Ray.cuh:53-61,85-95)
+__global__ void k_generate(const ViewDesc* __restrict__ views, const Segment* __restrict__ segs,
+                           const ChunkHdr* hdr, Queues q) {
+    const uint32_t n = hdr->n;
+    const int nseg = (int)hdr->nseg;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const Segment sg = segs[find_seg(segs, nseg, i)];
+        const ViewDesc& v = views[sg.view];
+        const uint32_t local = sg.first_path + (i - sg.chunk_offset);
+        const uint32_t spp = (uint32_t)v.spp;
+        const uint32_t pixel = local / spp;
+        const uint32_t frame = v.frame_base + (local - pixel * spp);
+        const uint32_t x = pixel % (uint32_t)v.width, y = pixel / (uint32_t)v.width;
+        uint32_t seed = ct_primary_seed(pixel, frame);
+        const float jx = ct_rand_float(&seed);
+        const float jy = ct_rand_float(&seed);
+        const float px = (float)x + jx - 0.5f;
+        const float py = (float)y + jy - 0.5f;
+        const float sx = 2.0f * (px * v.invw) - 1.0f;
+        const float sy = 1.0f - 2.0f * (py * v.invh);
+        const f3 w = mk3(v.w[0], v.w[1], v.w[2]);
+        const f3 hor = mk3(v.hor[0], v.hor[1], v.hor[2]);
+        const f3 ver = mk3(v.ver[0], v.ver[1], v.ver[2]);
+        const f3 d = normalize(w + hor * sx + ver * sy);
+        q.ray_o[0][i] = make_float4(v.pos[0], v.pos[1], v.pos[2], 0.0f);
+        q.ray_d[0][i] = make_float4(d.x, d.y, d.z, 0.0f);
+        q.thr[0][i] = make_float4(1.0f, 1.0f, 1.0f, 1.0f);
+        q.path[0][i] = i | (LT_SPECULAR << 30);
+        q.lpath[i] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    }
+}
+
+// --------------------------------------------------------------- extend --
+// Persistent closest-hit: each wave takes 64 rays per atomic from the fetch
+// counter until the bounce's queue is drained.
+__global__ void k_extend(DevScene sc, TraceOpts to, Queues q, int sel, int bounce, int lds_depth,
+                         unsigned long long* wstats) {
+    extern __shared__ uint32_t lds_stack[];
+    const uint32_t n = *cnt_active(q.counters, bounce);
+    uint32_t* fetch = cnt_fetch_ext(q.counters, bounce);
+    const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
+    LaneStack st;
+    st.lds = lds_stack + threadIdx.x;
+    st.stride = blockDim.x;
+    st.depth = lds_depth;
+    st.spill = q.spill + (size_t)gtid * (size_t)(kMaxStack - lds_depth);
+    st.overflow = false;
+    const float4* __restrict__ ro = q.ray_o[sel];
+    const float4* __restrict__ rd = q.ray_d[sel];
+    TravCount tc{0u, 0u};
+    while (true) {
+        uint32_t base = 0;
+        if (lane_id() == 0) base = atomicAdd(fetch, 64u);
+        base = __shfl(base, 0);
+        if (base >= n) break;
+        const uint32_t i = base + lane_id();
+        if (i < n) {
+            const float4 o4 = ro[i], d4 = rd[i];
+            float t = 1e34f;
+            int prim = -1;
+            trace_ray<false>(sc, to, xyz(o4), xyz(d4), t, prim, st, wstats ? &tc : nullptr);
+            if (st.overflow) { atomicOr(q.err, 1u); st.overflow = false; t = 1e34f; prim = -1; }
+            q.hit[i] = make_float2(t, __int_as_float(prim));
+        }
+    }
+    if (wstats) {
+        atomicAdd(&wstats[0], (unsigned long long)tc.nodes);
+        atomicAdd(&wstats[1], (unsigned long long)tc.tris);
+    }
+}
+
+// -------------------------------------------------------------- connect --
+__global__ void k_connect(DevScene sc, TraceOpts to, Queues q, int bounce, int lds_depth,
+                          unsigned long long* wstats) {
+    extern __shared__ uint32_t lds_stack[];
+    const uint32_t n = *cnt_shadow(q.counters, bounce);
+    uint32_t* fetch = cnt_fetch_sh(q.counters, bounce);
+    const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
+    LaneStack st;
+    st.lds = lds_stack + threadIdx.x;
+    st.stride = blockDim.x;
+    st.depth = lds_depth;
+    st.spill = q.spill + (size_t)gtid * (size_t)(kMaxStack - lds_depth);
+    st.overflow = false;
+    TravCount tc{0u, 0u};
+    while (true) {
+        uint32_t base = 0;
+        if (lane_id() == 0) base = atomicAdd(fetch, 64u);
+        base = __shfl(base, 0);
+        if (base >= n) break;
+        const uint32_t i = base + lane_id();
+        if (i < n) {
+            const float4 o4 = q.sh_o[i], d4 = q.sh_d[i];
+            float tmax = o4.w;
+            int dummy = -1;
+            const bool occ = trace_ray<true>(sc, to, xyz(o4), xyz(d4), tmax, dummy, st, wstats ? &tc : nullptr);
+            if (st.overflow) { atomicOr(q.err, 1u); st.overflow = false; }
+            else if (!occ) {
+                const uint32_t slot = __float_as_uint(d4.w);
+                const float4 c = q.sh_c[i];
+                float4 L = q.lpath[slot];
+                L.x = L.x + c.x;
+                L.y = L.y + c.y;
+                L.z = L.z + c.z;
+                q.lpath[slot] = L;
+            }
+        }
+    }
+    if (wstats) {
+        atomicAdd(&wstats[2], (unsigned long long)tc.nodes);
+        atomicAdd(&wstats[3], (unsigned long long)tc.tris);
+    }
+}
+
+// ---------------------------------------------------------------- shade --
+// shade (Kernel.cu:88-206) for queue entries [0, active[bounce]); see the
+// oracle's path_sample for the same steps on the CPU.
+__global__ void k_shade(DevScene sc, ShadeOpts so, const ViewDesc* __restrict__ views,
+                        const Segment* __restrict__ segs, const ChunkHdr* hdr, Queues q, int sel, int bounce) {
+    const uint32_t n = *cnt_active(q.counters, bounce);
+    const int nseg = (int)hdr->nseg;
+    const int nsel = sel ^ 1;
+    const uint32_t stride = gridDim.x * blockDim.x;
+    // wave-uniform loop: all lanes of a wave iterate together
+    for (uint32_t base = blockIdx.x * blockDim.x + (threadIdx.x & ~63u); base < n; base += stride) {
+        const uint32_t i = base + lane_id();
+        const bool active = i < n;
+        bool push_ext = false, push_sh = false;
+        f3 e_o = mk3(0, 0, 0), e_d = mk3(0, 0, 0), e_thr = mk3(0, 0, 0);
+        float e_pdf = 0.0f;
+        uint32_t e_type = 0, slot = 0;
+        f3 s_o = mk3(0, 0, 0), s_d = mk3(0, 0, 0), s_c = mk3(0, 0, 0);
+        float s_tmax = 0.0f;
+        if (active) {
+            const float4 o4 = q.ray_o[sel][i], d4 = q.ray_d[sel][i], th4 = q.thr[sel][i];
+            const uint32_t pw = q.path[sel][i];
+            const float2 h = q.hit[i];
+            slot = pw & kSlotMask;
+            const uint32_t lastType = pw >> 30;
+            const float t = h.x;
+            const int prim = __float_as_int(h.y);
+            f3 dir = xyz(d4);
+            f3 thr = xyz(th4);
+            const float lastPdf = th4.w;
+            f3 add = mk3(0, 0, 0);
+            bool contrib = false;
+            if (!(t < 1e34f)) {
+                if (sc.sky_enabled) {
+                    add = (thr / lastPdf) * mk3(sc.sky_r, sc.sky_g, sc.sky_b);
+                    contrib = true;
+                }
+            } else {
+                const float4 P0 = sc.sh_pos[3 * (size_t)prim], P1 = sc.sh_pos[3 * (size_t)prim + 1],
+                             P2 = sc.sh_pos[3 * (size_t)prim + 2];
+                const uint32_t mi = __float_as_uint(P0.w);
+                const float4 M0 = sc.mats[2 * (size_t)mi], M1 = sc.mats[2 * (size_t)mi + 1];
+                const int mtype = __float_as_int(M1.x);
+                const f3 p0 = xyz(P0), p1 = xyz(P1), p2 = xyz(P2);
+                const f3 org0 = xyz(o4);
+                const f3 hitp = org0 + t * dir;
+                const float4 N0 = sc.sh_nrm[4 * (size_t)prim], N1 = sc.sh_nrm[4 * (size_t)prim + 1],
+                             N2 = sc.sh_nrm[4 * (size_t)prim + 2], NC = sc.sh_nrm[4 * (size_t)prim + 3];
+                f3 N = xyz(NC);
+                const f3 bary = bary_coords(hitp, N, p0, p1, p2);
+                f3 iN = bary.x * xyz(N0) + bary.y * xyz(N1) + bary.z * xyz(N2);
+                if (mtype == MAT_LIGHT) {
+                    if (so.indirect) {
+                        const float DdotNL = -dot(dir, iN);
+                        const f3 Le = xyz(M0);
+                        f3 c;
+                        if (lastType == LT_SPECULAR) {
+                            c = thr * Le;
+                        } else {
+                            const float lightPdf = t * t / (DdotNL * tri_area(p0, p1, p2)) * (float)sc.n_lights;
+                            c = thr * Le * (1.0f / (lastPdf + lightPdf));
+                        }
+                        add = vmax0(c);
+                        contrib = true;
+                    }
+                } else {
+                    const SlotInfo si = decode_slot(views, segs, nseg, slot);
+                    f3 org = hitp;
+                    thr = vmax0(thr / lastPdf);
+                    const bool flipped = dot(dir, N) > 0.0f;
+                    if (flipped) { N = -N; iN = -iN; }
+                    uint32_t seed = ct_shade_seed(si.pixel, si.frame, ct_path_length_key(bounce));
+                    const float r3 = ct_rand_float(&seed);
+                    const float r4 = ct_rand_float(&seed);
+                    const f3 color = xyz(M0);
+                    const f3 wiw = -dir;
+                    f3 wo = mk3(0, 0, 0), f = mk3(0, 0, 0);
+                    float pdf = 0.0f;
+                    uint32_t type = LT_DIFFUSE;
+                    bool valid = false, transmit = false;
+                    const bool typed = so.bsdf_mode == CT_BSDF_TYPED;
+                    const float roughness = M0.w;
+                    const float alpha = gmax(roughness, 1e-3f);
+                    if ((!typed && roughness < 0.1f) || (typed && mtype == MAT_SPECULAR)) {
+                        wo = reflect(dir, iN);
+                        if (dot(N, wo) > 0.0f) {
+                            pdf = 1.0f;
+                            type = LT_SPECULAR;
+                            f = color * (1.0f / ct_fabsf(dot(iN, wo)));
+                            valid = true;
+                        }
+                    } else if (typed && is_dielectric(mtype)) {
+                        const float ior = M1.y;
+                        const float etaI = flipped ? ior : 1.0f;
+                        const float etaT = flipped ? 1.0f : ior;
+                        const float cosI = dot(wiw, iN);
+                        const float F = fresnel_dielectric(cosI, etaT / etaI);
+                        type = LT_SPECULAR;
+                        pdf = 1.0f;
+                        if (r3 < F || F >= 1.0f) {
+                            wo = reflect(dir, iN);
+                            valid = dot(N, wo) > 0.0f;
+                        } else {
+                            const float eta = etaI / etaT;
+                            const float dv = dot(iN, dir);
+                            const float k = 1.0f - eta * eta * (1.0f - dv * dv);
+                            wo = eta * dir - (eta * dv + ct_sqrtf(k)) * iN;
+                            valid = k >= 0.0f && dot(N, wo) < 0.0f;
+                            transmit = true;
+                        }
+                        if (valid) f = color * (1.0f / ct_fabsf(dot(iN, wo)));
+                    } else if (typed && is_microfacet(mtype)) {
+                        f3 Tv, Bv;
+                        tangent_frame(iN, Tv, Bv);
+                        const f3 wol = mk3(dot(wiw, Tv), dot(wiw, Bv), dot(wiw, iN));
+                        if (wol.z > 0.0f) {
+                            const f3 wm = sample_vndf(mtype, wol, alpha, r3, r4);
+                            const f3 wil = reflect(-wol, wm);
+                            if (wil.z > 0.0f) {
+                                wo = normalize(wil.x * Tv + wil.y * Bv + wil.z * iN);
+                                if (dot(N, wo) > 0.0f) {
+                                    mf_eval(mtype, alpha, color, wol, wil, &f, &pdf);
+                                    type = LT_GLOSSY;
+                                    valid = pdf > 0.0f;
+                                }
+                            }
+                        }
+                    } else {
+                        wo = normalize(tangent_to_world(cos_weighted(r3, r4), iN));
+                        if (dot(N, wo) > 0.0f) {
+                            type = LT_DIFFUSE;
+                            pdf = gmax(0.0f, dot(wo, iN)) * CT_INV_PI_F;
+                            f = color * CT_INV_PI_F;
+                            valid = true;
+                        }
+                    }
+                    if (valid) {
+                        // next-event estimation (Kernel.cu:161-195)
+                        if (type != LT_SPECULAR && so.shadow && sc.n_lights > 0) {
+                            int light = ct_rand_int_max(&seed, (int)sc.n_lights - 1);
+                            if (light > (int)sc.n_lights - 1) light = (int)sc.n_lights - 1;
+                            const float r1 = ct_rand_float(&seed);
+                            const float r2 = ct_rand_float(&seed);
+                            const float4* lr = sc.lights + 5 * (size_t)light;
+                            const float4 L0 = lr[0], L1 = lr[1], L2 = lr[2], LN = lr[3], LE = lr[4];
+                            const f3 lightPos = random_point(xyz(L0), xyz(L1), xyz(L2), r1, r2);
+                            f3 Lv = lightPos - org;
+                            const float sqDist = dot(Lv, Lv);
+                            const float distance = ct_sqrtf(sqDist);
+                            Lv = Lv / distance;
+                            const f3 lN = xyz(LN);
+                            const float NdotL = dot(iN, Lv);
+                            const float LNdotL = -dot(lN, Lv);
+                            if (NdotL > 0.0f && LNdotL > 0.0f) {
+                                const float ar = LN.w;
+                                const f3 emission = xyz(LE);
+                                f3 fe;
+                                float shadowPdf;
+                                if (type == LT_DIFFUSE) {
+                                    shadowPdf = ct_fabsf(dot(Lv, iN)) * CT_INV_PI_F;
+                                    fe = color * CT_INV_PI_F;
+                                } else {
+                                    f3 Tv, Bv;
+                                    tangent_frame(iN, Tv, Bv);
+                                    const f3 wol = mk3(dot(wiw, Tv), dot(wiw, Bv), dot(wiw, iN));
+                                    const f3 wil = mk3(dot(Lv, Tv), dot(Lv, Bv), dot(Lv, iN));
+                                    mf_eval(mtype, alpha, color, wol, wil, &fe, &shadowPdf);
+                                }
+                                if (shadowPdf > 0.0f) {
+                                    const float lightPdf = sqDist / (LNdotL * ar) * (float)sc.n_lights;
+                                    s_c = thr * emission * fe * (NdotL / (shadowPdf + lightPdf));
+                                    s_o = org + so.neps * Lv;
+                                    s_d = Lv;
+                                    s_tmax = distance - 2.0f * so.deps;
+                                    push_sh = true;
+                                }
+                            }
+                        }
+                        // continuation (Kernel.cu:197-205)
+                        if (!(pdf < 1e-6f || isnanf_(pdf) || isnanf_(thr.x) || isnanf_(thr.y) || isnanf_(thr.z) ||
+                              (thr.x <= 0.0f && thr.y <= 0.0f && thr.z <= 0.0f))) {
+                            thr = vmax0(thr * f * ct_fabsf(dot(iN, wo)));
+                            org = transmit ? org - N * so.neps : org + N * so.neps;
+                            bool alive = bounce + 1 < so.max_depth;
+                            if (alive && so.rr_start >= 0 && bounce >= so.rr_start) {
+                                const f3 tt = thr / pdf;
+                                const float qs = gmin(1.0f, gmax(tt.x, gmax(tt.y, tt.z)));
+                                const float rr = ct_rand_float(&seed);
+                                if (!(rr < qs)) alive = false;
+                                else thr = thr / qs;
+                            }
+                            if (alive) {
+                                push_ext = true;
+                                e_o = org;
+                                e_d = wo;
+                                e_thr = thr;
+                                e_pdf = pdf;
+                                e_type = type;
+                            }
+                        }
+                    }
+                }
+            }
+            if (contrib) {
+                float4 L = q.lpath[slot];
+                L.x = L.x + add.x;
+                L.y = L.y + add.y;
+                L.z = L.z + add.z;
+                q.lpath[slot] = L;
+            }
+        }
+        // wave-aggregated queue appends (all lanes of the wave reach here)
+        const uint32_t je = wave_append(push_ext, cnt_active(q.counters, bounce + 1));
+        if (push_ext) {
+            q.ray_o[nsel][je] = make_float4(e_o.x, e_o.y, e_o.z, 0.0f);
+            q.ray_d[nsel][je] = make_float4(e_d.x, e_d.y, e_d.z, 0.0f);
+            q.thr[nsel][je] = make_float4(e_thr.x, e_thr.y, e_thr.z, e_pdf);
+            q.path[nsel][je] = slot | (e_type << 30);
+        }
+        const uint32_t js = wave_append(push_sh, cnt_shadow(q.counters, bounce));
+        if (push_sh) {
+            q.sh_o[js] = make_float4(s_o.x, s_o.y, s_o.z, s_tmax);
+            q.sh_d[js] = make_float4(s_d.x, s_d.y, s_d.z, __uint_as_float(slot));
+            q.sh_c[js] = make_float4(s_c.x, s_c.y, s_c.z, 0.0f);
+        }
+    }
+}
+
+// -------------------------------------------------------------- resolve --
+// accum[pixel] += per-sample radiance, samples in increasing order.
+__global__ void k_resolve(const ViewDesc* __restrict__ views, const Segment* __restrict__ segs,
+                          const ChunkHdr* hdr, const float4* __restrict__ lpath, float4* accum) {
+    const uint32_t n_tasks = hdr->n_tasks;
+    const int nseg = (int)hdr->nseg;
+    for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < n_tasks; j += gridDim.x * blockDim.x) {
+        int lo = 0, hi = nseg - 1;
+        while (lo < hi) {
+            const int mid = (lo + hi + 1) >> 1;
+            if (segs[mid].pix_offset <= j) lo = mid;
+            else hi = mid - 1;
+        }
+        const Segment sg = segs[lo];
+        const ViewDesc& v = views[sg.view];
+        const uint32_t spp = (uint32_t)v.spp;
+        const uint32_t p = sg.first_path / spp + (j - sg.pix_offset);
+        const uint32_t g0 = max(p * spp, sg.first_path);
+        const uint32_t g1 = min((p + 1) * spp, sg.first_path + sg.n);
+        float4* a = accum + v.accum_offset + p;
+        float4 acc = *a;
+        for (uint32_t g = g0; g < g1; ++g) {
+            const float4 L = lpath[sg.chunk_offset + (g - sg.first_path)];
+            acc.x = acc.x + L.x;
+            acc.y = acc.y + L.y;
+            acc.z = acc.z + L.z;
+            acc.w = acc.w + 1.0f;
+        }
+        *a = acc;
+    }
+}
+
+// draw_framebuffer (Kernel.cu:258-270): rgb / samples, then x^(1/2.2).
+__global__ void k_tonemap(const float4* __restrict__ accum, uint32_t n, float samples, float4* out) {
+    const float e = 1.0f / 2.2f;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const float4 a = accum[i];
+        const f3 c = mk3(a.x, a.y, a.z) / samples;
+        out[i] = make_float4(ct_powf(c.x, e), ct_powf(c.y, e), ct_powf(c.z, e), 1.0f);
+    }
+}
+
+// Ray-level any-hit: occluded flag per ray (rays in sh_o / sh_d layout).
+__global__ void k_shadow_flags(DevScene sc, TraceOpts to, Queues q, uint32_t n, int lds_depth, uint8_t* out) {
+    extern __shared__ uint32_t lds_stack[];
+    uint32_t* fetch = cnt_fetch_sh(q.counters, 0);
+    const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
+    LaneStack st;
+    st.lds = lds_stack + threadIdx.x;
+    st.stride = blockDim.x;
+    st.depth = lds_depth;
+    st.spill = q.spill + (size_t)gtid * (size_t)(kMaxStack - lds_depth);
+    st.overflow = false;
+    while (true) {
+        uint32_t base = 0;
+        if (lane_id() == 0) base = atomicAdd(fetch, 64u);
+        base = __shfl(base, 0);
+        if (base >= n) break;
+        const uint32_t i = base + lane_id();
+        if (i < n) {
+            const float4 o4 = q.sh_o[i], d4 = q.sh_d[i];
+            float tmax = o4.w;
+            int dummy = -1;
+            const bool occ = trace_ray<true>(sc, to, xyz(o4), xyz(d4), tmax, dummy, st, nullptr);
+            if (st.overflow) { atomicOr(q.err, 1u); st.overflow = false; }
+            out[i] = occ ? 1 : 0;
+        }
+    }
+}
+
+}  // namespace ctd
+
+// ------------------------------------------------------------ launchers --
+namespace ctk {
+
+using namespace ctd;
+
+hipError_t launch_setup(uint32_t* counters, const ChunkHdr* hdr, hipStream_t s) {
+    hipLaunchKernelGGL(k_setup, dim3(1), dim3(256), 0, s, counters, hdr);
+    return hipGetLastError();
+}
+hipError_t launch_generate(const LaunchShape& ls, const ViewDesc* views, const Segment* segs,
+                           const ChunkHdr* hdr, uint32_t max_n, const Queues& q, hipStream_t s) {
+    const uint32_t blocks = std::max<uint32_t>(1u, std::min<uint32_t>((max_n + 255) / 256, (uint32_t)ls.max_blocks));
+    hipLaunchKernelGGL(k_generate, dim3(blocks), dim3(256), 0, s, views, segs, hdr, q);
+    return hipGetLastError();
+}
+hipError_t launch_extend(const LaunchShape& ls, const DevScene& sc, const TraceOpts& to, const Queues& q, int sel,
+                         int bounce, unsigned long long* wstats, hipStream_t s) {
+    const size_t lds = (size_t)ls.lds_depth * ls.block * 4;
+    hipLaunchKernelGGL(k_extend, dim3(ls.trace_blocks), dim3(ls.block), lds, s, sc, to, q, sel, bounce,
+                       ls.lds_depth, wstats);
+    return hipGetLastError();
+}
+hipError_t launch_connect(const LaunchShape& ls, const DevScene& sc, const TraceOpts& to, const Queues& q,
+                          int bounce, unsigned long long* wstats, hipStream_t s) {
+    const size_t lds = (size_t)ls.lds_depth * ls.block * 4;
+    hipLaunchKernelGGL(k_connect, dim3(ls.trace_blocks), dim3(ls.block), lds, s, sc, to, q, bounce, ls.lds_depth,
+                       wstats);
+    return hipGetLastError();
+}
+hipError_t launch_shade(const LaunchShape& ls, const DevScene& sc, const ShadeOpts& so, const ViewDesc* views,
+                        const Segment* segs, const ChunkHdr* hdr, const Queues& q, int sel, int bounce,
+                        hipStream_t s) {
+    hipLaunchKernelGGL(k_shade, dim3(ls.shade_blocks), dim3(256), 0, s, sc, so, views, segs, hdr, q, sel, bounce);
+    return hipGetLastError();
+}
+hipError_t launch_resolve(const LaunchShape& ls, const ViewDesc* views, const Segment* segs, const ChunkHdr* hdr,
+                          uint32_t max_tasks, const float4* lpath, float4* accum, hipStream_t s) {
+    const uint32_t blocks =
+        std::max<uint32_t>(1u, std::min<uint32_t>((max_tasks + 255) / 256, (uint32_t)ls.max_blocks));
+    hipLaunchKernelGGL(k_resolve, dim3(blocks), dim3(256), 0, s, views, segs, hdr, lpath, accum);
+    return hipGetLastError();
+}
+hipError_t launch_tonemap(const float4* accum, uint32_t n, float samples, float4* out, hipStream_t s) {
+    const uint32_t blocks = std::min<uint32_t>((n + 255) / 256, 65536u);
+    if (blocks == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_tonemap, dim3(blocks), dim3(256), 0, s, accum, n, samples, out);
+    return hipGetLastError();
+}
+hipError_t launch_shadow_flags(const LaunchShape& ls, const DevScene& sc, const TraceOpts& to, const Queues& q,
+                               uint32_t n, uint8_t* out, hipStream_t s) {
+    const size_t lds = (size_t)ls.lds_depth * ls.block * 4;
+    hipLaunchKernelGGL(k_shadow_flags, dim3(ls.trace_blocks), dim3(ls.block), lds, s, sc, to, q, n, ls.lds_depth,
+                       out);
+    return hipGetLastError();
+}
+
+}  // namespace ctk
+
+namespace ctk {
+// Resident blocks per CU of the persistent traversal kernels for a launch shape.
+int trace_blocks_per_cu(int block, int lds_depth) {
+    int nb = 0;
+    const size_t lds = (size_t)lds_depth * block * 4;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, reinterpret_cast<const void*>(&ctd::k_extend), block,
+                                                     lds) != hipSuccess)
+        return 0;
+    int nc = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nc, reinterpret_cast<const void*>(&ctd::k_connect), block,
+                                                     lds) != hipSuccess)
+        return 0;
+    return nb < nc ? nb : nc;
+}
+}  // namespace ctk

@@ -1,0 +1,40 @@
+// Host-side launchers of the wavefront kernels (kernels.hip).
+#pragma once
+
+#include <algorithm>
+
+#include "../../../include/cudatracer.h"
+#include "dev_common.h"
+
+namespace ctk {
+
+// Launch geometry chosen by the runtime (and the autotuner).
+struct LaunchShape {
+    int block = 256;          // threads per block for extend / connect
+    int lds_depth = 16;       // per-lane LDS stack entries
+    int trace_blocks = 1024;  // persistent grid of extend / connect
+    int shade_blocks = 1024;  // grid-stride grid of shade
+    int max_blocks = 65536;   // cap for the simple grid-stride kernels
+};
+
+// Resident blocks per CU of extend/connect (0 on error).
+int trace_blocks_per_cu(int block, int lds_depth);
+
+hipError_t launch_setup(uint32_t* counters, const ctd::ChunkHdr* hdr, hipStream_t s);
+hipError_t launch_generate(const LaunchShape& ls, const ctd::ViewDesc* views, const ctd::Segment* segs,
+                           const ctd::ChunkHdr* hdr, uint32_t max_n, const ctd::Queues& q, hipStream_t s);
+hipError_t launch_extend(const LaunchShape& ls, const ctd::DevScene& sc, const ctd::TraceOpts& to,
+                         const ctd::Queues& q, int sel, int bounce, unsigned long long* wstats, hipStream_t s);
+hipError_t launch_connect(const LaunchShape& ls, const ctd::DevScene& sc, const ctd::TraceOpts& to,
+                          const ctd::Queues& q, int bounce, unsigned long long* wstats, hipStream_t s);
+hipError_t launch_shade(const LaunchShape& ls, const ctd::DevScene& sc, const ctd::ShadeOpts& so,
+                        const ctd::ViewDesc* views, const ctd::Segment* segs, const ctd::ChunkHdr* hdr,
+                        const ctd::Queues& q, int sel, int bounce, hipStream_t s);
+hipError_t launch_resolve(const LaunchShape& ls, const ctd::ViewDesc* views, const ctd::Segment* segs,
+                          const ctd::ChunkHdr* hdr, uint32_t max_tasks, const float4* lpath, float4* accum,
+                          hipStream_t s);
+hipError_t launch_tonemap(const float4* accum, uint32_t n, float samples, float4* out, hipStream_t s);
+hipError_t launch_shadow_flags(const LaunchShape& ls, const ctd::DevScene& sc, const ctd::TraceOpts& to,
+                               const ctd::Queues& q, uint32_t n, uint8_t* out, hipStream_t s);
+
+}  // namespace ctk

@@ -1,0 +1,675 @@
+// C ABI entry points: scene upload (with the node/triangle export layouts),
+// resize/reset/material update, per-frame trace, accumulator access,
+// ray-level tracing, batched multi-view rendering and the autotuner.
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <sstream>
+
+#include "runtime.h"
+
+using ctd::Segment;
+using ctd::ViewDesc;
+using namespace ctr;
+
+#define HIPCHK(expr, what)                                     \
+    do {                                                       \
+        hipError_t e_ = (expr);                                \
+        if (e_ != hipSuccess) return ctr::hip_fail(c, e_, what); \
+    } while (0)
+
+namespace {
+
+bool finite3(const float* p) { return std::isfinite(p[0]) && std::isfinite(p[1]) && std::isfinite(p[2]); }
+
+// triangle::getArea (Triangle.cuh:89-96) in float, glm order.
+float heron_area(const float* p0, const float* p1, const float* p2) {
+    auto len = [](const float* a, const float* b) {
+        const float x = a[0] - b[0], y = a[1] - b[1], z = a[2] - b[2];
+        return std::sqrt(x * x + y * y + z * z);
+    };
+    const float a = len(p0, p1), b = len(p1, p2), c = len(p2, p0);
+    const float s = (a + b + c) / 2.0f;
+    return std::sqrt(s * (s - a) * (s - b) * (s - c));
+}
+
+float4 f4(const float* p, float w) { return make_float4(p[0], p[1], p[2], w); }
+float ubits(uint32_t u) {
+    float f;
+    std::memcpy(&f, &u, 4);
+    return f;
+}
+
+// Collect leaf ranges of a tree while checking that it is a proper tree
+// (every interior child in range and reached once).
+int collect_bvh2(const ct_scene_view* v, std::vector<std::pair<uint32_t, uint32_t>>& leaves, std::string& why) {
+    const uint32_t nn = v->n_bvh_nodes, N = v->n_triangles;
+    std::vector<uint8_t> seen(nn, 0);
+    std::vector<uint32_t> st{0};
+    seen[0] = 1;
+    while (!st.empty()) {
+        const uint32_t i = st.back();
+        st.pop_back();
+        const ct_bvh_node& n = v->bvh_nodes[i];
+        if (n.count >= 0) {
+            if ((uint64_t)(uint32_t)n.left_first + (uint64_t)n.count > N || n.left_first < 0) {
+                why = "binary leaf range outside the primitive list";
+                return 1;
+            }
+            if (n.count > 0) leaves.emplace_back((uint32_t)n.left_first, (uint32_t)n.count);
+            continue;
+        }
+        if (n.count != -1 || n.left_first < 1 || (uint64_t)n.left_first + 1 >= nn) {
+            why = "binary interior node with bad children";
+            return 1;
+        }
+        for (uint32_t k = 0; k < 2; ++k) {
+            const uint32_t ch = (uint32_t)n.left_first + k;
+            if (seen[ch]) { why = "binary tree has a shared or cyclic child"; return 1; }
+            seen[ch] = 1;
+            st.push_back(ch);
+        }
+    }
+    return 0;
+}
+
+int collect_bvh4(const ct_scene_view* v, std::vector<std::pair<uint32_t, uint32_t>>& leaves, std::string& why) {
+    const uint32_t nn = v->n_mbvh_nodes, N = v->n_triangles;
+    std::vector<uint8_t> seen(nn, 0);
+    std::vector<uint32_t> st{0};
+    seen[0] = 1;
+    while (!st.empty()) {
+        const uint32_t i = st.back();
+        st.pop_back();
+        const ct_mbvh_node& n = v->mbvh_nodes[i];
+        for (int s = 0; s < 4; ++s) {
+            const int cnt = n.counts[s], ch = n.childs[s];
+            if (cnt == 0) continue;
+            if (cnt == -1) {
+                if (ch < 1 || (uint32_t)ch >= nn) { why = "quad interior child out of range"; return 1; }
+                if (seen[ch]) { why = "quad tree has a shared or cyclic child"; return 1; }
+                seen[ch] = 1;
+                st.push_back((uint32_t)ch);
+            } else if (cnt > 0) {
+                if (ch < 0 || (uint64_t)(uint32_t)ch + (uint64_t)cnt > N) {
+                    why = "quad leaf range outside the primitive list";
+                    return 1;
+                }
+                leaves.emplace_back((uint32_t)ch, (uint32_t)cnt);
+            } else {
+                why = "quad slot with a negative count other than -1";
+                return 1;
+            }
+        }
+    }
+    return 0;
+}
+
+// Mark the last triangle of each leaf; leaves must not overlap.
+int leaf_flags(const std::vector<std::pair<uint32_t, uint32_t>>& leaves, uint32_t N, std::vector<uint8_t>& last,
+               std::string& why) {
+    std::vector<uint8_t> used(N, 0);
+    last.assign(N, 0);
+    for (auto& l : leaves) {
+        for (uint32_t k = l.first; k < l.first + l.second; ++k) {
+            if (used[k]) { why = "leaves overlap"; return 1; }
+            used[k] = 1;
+        }
+        last[l.first + l.second - 1] = 1;
+    }
+    return 0;
+}
+
+void free_scene(ct_ctx* c) {
+    for (void* p : c->scene_bufs) dfree(c, p);
+    c->scene_bufs.clear();
+    c->sc = ctd::DevScene{};
+    c->has_scene = c->has_bvh2 = c->has_bvh4 = false;
+}
+
+template <class T>
+int upload_vec(ct_ctx* c, const std::vector<T>& h, const float4** dst, const char* what) {
+    void* p = nullptr;
+    const int r = dalloc(c, &p, h.size() * sizeof(T), what);
+    if (r != CT_OK) return r;
+    c->scene_bufs.push_back(p);
+    if (!h.empty()) HIPCHK(hipMemcpyAsync(p, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice, c->stream), what);
+    *dst = reinterpret_cast<const float4*>(p);
+    return CT_OK;
+}
+
+void mat_rows(const ct_material& m, float4* rows) {
+    rows[0] = make_float4(m.albedo[0], m.albedo[1], m.albedo[2], m.roughness);
+    rows[1] = make_float4(ubits((uint32_t)m.type), m.refract_idx, std::max(m.roughness, 1e-3f), 0.0f);
+}
+
+void light_rows(const ct_scene_view* v, uint32_t tri, const ct_material* mats, float4* rows) {
+    const float* p0 = v->vertices + 3 * (size_t)v->indices[3 * (size_t)tri];
+    const float* p1 = v->vertices + 3 * (size_t)v->indices[3 * (size_t)tri + 1];
+    const float* p2 = v->vertices + 3 * (size_t)v->indices[3 * (size_t)tri + 2];
+    rows[0] = f4(p0, 0.0f);
+    rows[1] = f4(p1, 0.0f);
+    rows[2] = f4(p2, 0.0f);
+    rows[3] = f4(v->center_normals + 3 * (size_t)tri, heron_area(p0, p1, p2));
+    rows[4] = f4(mats[v->material_idx[tri]].albedo, 0.0f);
+}
+
+}  // namespace
+
+extern "C" {
+
+int ct_scene_upload(ct_ctx* c, const ct_scene_view* v) {
+    if (!c || !v) return CT_ERR_INVALID_ARG;
+    (void)hipSetDevice(c->device);
+    const uint32_t N = v->n_triangles, V = v->n_vertices;
+    if (N == 0) return fail(c, CT_ERR_INVALID_SCENE, "scene has no triangles");
+    if (N >= (1u << 31)) return fail(c, CT_ERR_INVALID_SCENE, "too many triangles");
+    if (!v->vertices || !v->normals || !v->indices || !v->center_normals || !v->material_idx || !v->materials ||
+        !v->prim_indices || v->n_materials == 0 || (v->n_lights && !v->light_indices))
+        return fail(c, CT_ERR_INVALID_ARG, "scene view is missing an array");
+    if (v->n_bvh_nodes == 0 && v->n_mbvh_nodes == 0)
+        return fail(c, CT_ERR_INVALID_ARG, "scene view has no acceleration tree");
+    for (uint64_t i = 0; i < 3ull * N; ++i)
+        if (v->indices[i] >= V) return fail(c, CT_ERR_INVALID_SCENE, "vertex index out of range");
+    for (uint32_t i = 0; i < V; ++i)
+        if (!finite3(v->vertices + 3 * (size_t)i)) return fail(c, CT_ERR_INVALID_SCENE, "non-finite vertex");
+    for (uint32_t i = 0; i < N; ++i)
+        if (v->material_idx[i] >= v->n_materials) return fail(c, CT_ERR_INVALID_SCENE, "material index out of range");
+    for (uint32_t i = 0; i < v->n_lights; ++i)
+        if (v->light_indices[i] >= N) return fail(c, CT_ERR_INVALID_SCENE, "light index out of range");
+    {
+        std::vector<uint8_t> seen(N, 0);
+        for (uint32_t i = 0; i < N; ++i) {
+            const uint32_t p = v->prim_indices[i];
+            if (p >= N || seen[p]) return fail(c, CT_ERR_INVALID_SCENE, "prim_indices is not a permutation");
+            seen[p] = 1;
+        }
+    }
+    std::string why;
+    std::vector<uint8_t> last, last2;
+    bool have_flags = false;
+    if (v->n_mbvh_nodes) {
+        std::vector<std::pair<uint32_t, uint32_t>> lv;
+        if (collect_bvh4(v, lv, why) || leaf_flags(lv, N, last, why)) return fail(c, CT_ERR_INVALID_SCENE, why);
+        have_flags = true;
+    }
+    if (v->n_bvh_nodes) {
+        std::vector<std::pair<uint32_t, uint32_t>> lv;
+        if (collect_bvh2(v, lv, why) || leaf_flags(lv, N, last2, why)) return fail(c, CT_ERR_INVALID_SCENE, why);
+        if (have_flags && last2 != last)
+            return fail(c, CT_ERR_INVALID_SCENE, "binary and quad trees have different leaves");
+        if (!have_flags) last.swap(last2);
+    }
+
+    // host staging of the device layouts
+    std::vector<float4> tris((size_t)N * 3), sh_pos((size_t)N * 3), sh_nrm((size_t)N * 4);
+    for (uint32_t k = 0; k < N; ++k) {
+        const uint32_t prim = v->prim_indices[k];
+        const float* p0 = v->vertices + 3 * (size_t)v->indices[3 * (size_t)prim];
+        const float* p1 = v->vertices + 3 * (size_t)v->indices[3 * (size_t)prim + 1];
+        const float* p2 = v->vertices + 3 * (size_t)v->indices[3 * (size_t)prim + 2];
+        tris[3 * (size_t)k] = f4(p0, ubits(prim | (last[k] ? 0x80000000u : 0u)));
+        tris[3 * (size_t)k + 1] = make_float4(p1[0] - p0[0], p1[1] - p0[1], p1[2] - p0[2], 0.0f);
+        tris[3 * (size_t)k + 2] = make_float4(p2[0] - p0[0], p2[1] - p0[1], p2[2] - p0[2], 0.0f);
+    }
+    for (uint32_t prim = 0; prim < N; ++prim) {
+        for (int j = 0; j < 3; ++j) {
+            const uint32_t vi = v->indices[3 * (size_t)prim + j];
+            sh_pos[3 * (size_t)prim + j] = f4(v->vertices + 3 * (size_t)vi, j == 0 ? ubits(v->material_idx[prim]) : 0.0f);
+            sh_nrm[4 * (size_t)prim + j] = f4(v->normals + 3 * (size_t)vi, 0.0f);
+        }
+        sh_nrm[4 * (size_t)prim + 3] = f4(v->center_normals + 3 * (size_t)prim, 0.0f);
+    }
+    std::vector<float4> mats((size_t)v->n_materials * 2);
+    for (uint32_t m = 0; m < v->n_materials; ++m) mat_rows(v->materials[m], &mats[2 * (size_t)m]);
+    std::vector<float4> lights((size_t)std::max<uint32_t>(v->n_lights, 1u) * 5);
+    for (uint32_t l = 0; l < v->n_lights; ++l) light_rows(v, v->light_indices[l], v->materials, &lights[5 * (size_t)l]);
+    std::vector<ct_mbvh_node> n4(v->mbvh_nodes, v->mbvh_nodes + v->n_mbvh_nodes);
+    std::vector<ct_bvh_node> n2(v->bvh_nodes, v->bvh_nodes + v->n_bvh_nodes);
+
+    HIPCHK(hipStreamSynchronize(c->stream), "scene upload");
+    free_scene(c);
+    ctd::DevScene sc{};
+    int r = upload_vec(c, tris, &sc.tris, "triangles");
+    if (r == CT_OK) r = upload_vec(c, sh_pos, &sc.sh_pos, "shading positions");
+    if (r == CT_OK) r = upload_vec(c, sh_nrm, &sc.sh_nrm, "shading normals");
+    if (r == CT_OK) r = upload_vec(c, mats, &sc.mats, "materials");
+    if (r == CT_OK) r = upload_vec(c, lights, &sc.lights, "lights");
+    if (r == CT_OK && !n4.empty()) r = upload_vec(c, n4, &sc.nodes4, "quad nodes");
+    if (r == CT_OK && !n2.empty()) r = upload_vec(c, n2, &sc.nodes2, "binary nodes");
+    if (r != CT_OK) { free_scene(c); return r; }
+    HIPCHK(hipStreamSynchronize(c->stream), "scene upload");
+    sc.n_tris = N;
+    sc.n_nodes4 = v->n_mbvh_nodes;
+    sc.n_nodes2 = v->n_bvh_nodes;
+    sc.n_fnodes = 0;
+    sc.n_mats = v->n_materials;
+    sc.n_lights = v->n_lights;
+    sc.sky_enabled = v->sky_enabled ? 1 : 0;
+    sc.sky_r = v->sky[0];
+    sc.sky_g = v->sky[1];
+    sc.sky_b = v->sky[2];
+    c->sc = sc;
+    c->has_bvh4 = v->n_mbvh_nodes > 0;
+    c->has_bvh2 = v->n_bvh_nodes > 0;
+    c->has_scene = true;
+    c->h_mats.assign(v->materials, v->materials + v->n_materials);
+    c->h_mat_idx.assign(v->material_idx, v->material_idx + N);
+    c->h_light_idx.assign(v->light_indices, v->light_indices + v->n_lights);
+    c->h_light_rows.assign(reinterpret_cast<const float*>(lights.data()),
+                           reinterpret_cast<const float*>(lights.data()) + lights.size() * 4);
+    c->graph_key.clear();
+    return CT_OK;
+}
+
+int ct_material_update(ct_ctx* c, uint32_t index, const ct_material* mat) {
+    if (!c || !mat) return CT_ERR_INVALID_ARG;
+    if (!c->has_scene) return fail(c, CT_ERR_NO_SCENE, "no scene uploaded");
+    if (index >= c->h_mats.size()) return fail(c, CT_ERR_INVALID_ARG, "material index out of range");
+    (void)hipSetDevice(c->device);
+    c->h_mats[index] = *mat;
+    float4 rows[2];
+    mat_rows(*mat, rows);
+    HIPCHK(hipMemcpyAsync((void*)(c->sc.mats + 2 * (size_t)index), rows, sizeof(rows), hipMemcpyHostToDevice,
+                          c->stream),
+           "material update");
+    // lights whose triangle uses this material carry its emission
+    const float4 e = make_float4(mat->albedo[0], mat->albedo[1], mat->albedo[2], 0.0f);
+    for (size_t l = 0; l < c->h_light_idx.size(); ++l)
+        if (c->h_mat_idx[c->h_light_idx[l]] == index)
+            HIPCHK(hipMemcpyAsync((void*)(c->sc.lights + 5 * l + 4), &e, sizeof(e), hipMemcpyHostToDevice, c->stream),
+                   "light emission update");
+    HIPCHK(hipStreamSynchronize(c->stream), "material update");
+    return CT_OK;
+}
+
+int ct_resize(ct_ctx* c, int32_t width, int32_t height) {
+    if (!c) return CT_ERR_INVALID_ARG;
+    if (width <= 0 || height <= 0 || (int64_t)width * height > (1ll << 28))
+        return fail(c, CT_ERR_INVALID_ARG, "bad frame size");
+    (void)hipSetDevice(c->device);
+    HIPCHK(hipStreamSynchronize(c->stream), "resize");
+    dfree(c, c->accum);
+    c->accum = nullptr;
+    c->graph_key.clear();
+    const int r = dalloc(c, (void**)&c->accum, (size_t)width * height * sizeof(float4), "accumulator");
+    if (r != CT_OK) { c->width = c->height = 0; return r; }
+    c->width = width;
+    c->height = height;
+    return ct_reset(c);
+}
+
+int ct_reset(ct_ctx* c) {
+    if (!c) return CT_ERR_INVALID_ARG;
+    if (!c->accum) return fail(c, CT_ERR_INVALID_ARG, "no frame buffer (call ct_resize)");
+    (void)hipSetDevice(c->device);
+    HIPCHK(hipMemsetAsync(c->accum, 0, (size_t)c->width * c->height * sizeof(float4), c->stream), "reset");
+    HIPCHK(hipStreamSynchronize(c->stream), "reset");
+    c->samples = 0;
+    return CT_OK;
+}
+
+static bool camera_ok(const ct_camera* cam) {
+    if (!cam) return false;
+    if (!finite3(cam->position) || !finite3(cam->forward) || !finite3(cam->up) || !std::isfinite(cam->plane_distance))
+        return false;
+    const float l2 = cam->forward[0] * cam->forward[0] + cam->forward[1] * cam->forward[1] +
+                     cam->forward[2] * cam->forward[2];
+    return l2 > 0.0f && cam->plane_distance > 0.0f;
+}
+
+static void fill_stats(const ChunkStats& s, ct_frame_stats* out) {
+    if (!out) return;
+    out->primary_rays = s.primary;
+    out->extension_rays = s.extension;
+    out->shadow_rays = s.shadow;
+    out->samples = s.samples;
+    out->gpu_ms = s.total_ms;
+    out->extend_ms = s.extend_ms;
+    out->shade_ms = s.shade_ms;
+    out->connect_ms = s.connect_ms;
+    out->other_ms = s.other_ms;
+}
+
+int ct_trace_frame(ct_ctx* c, const ct_camera* cam, const ct_frame_opts* opts, ct_frame_stats* stats) {
+    if (!c) return CT_ERR_INVALID_ARG;
+    ct_frame_opts o;
+    if (opts) o = *opts;
+    else ct_frame_opts_default(&o);
+    int r = validate_opts(c, o);
+    if (r != CT_OK) return r;
+    if (!camera_ok(cam)) return fail(c, CT_ERR_INVALID_ARG, "camera is not finite or degenerate");
+    if (!c->accum) return fail(c, CT_ERR_INVALID_ARG, "no frame buffer (call ct_resize)");
+    (void)hipSetDevice(c->device);
+    const uint64_t total = (uint64_t)c->width * c->height * (uint64_t)o.spp;
+    if (total >= (1ull << 32)) return fail(c, CT_ERR_INVALID_ARG, "width*height*spp must be < 2^32");
+    std::vector<ViewDesc> views{make_view(*cam, c->width, c->height, o.spp, (uint32_t)c->samples, 0)};
+    std::vector<Run> runs{{0u, 0u, (uint32_t)total}};
+    const uint32_t cap = (uint32_t)std::min<uint64_t>(total, c->chunk_default);
+    ChunkStats st;
+    r = run_paths(c, views, runs, cap, c->accum, o, stats != nullptr, false, false, &st);
+    if (r != CT_OK) return r;
+    c->samples += o.spp;
+    fill_stats(st, stats);
+    return CT_OK;
+}
+
+// Frame with work counters (node / triangle tests per stage), for the
+// benchmark's algorithmic-byte accounting.  Not part of the public header.
+int ct_trace_frame_counted(ct_ctx* c, const ct_camera* cam, const ct_frame_opts* opts, ct_frame_stats* stats,
+                           uint64_t work[4]) {
+    if (!c || !work) return CT_ERR_INVALID_ARG;
+    ct_frame_opts o;
+    if (opts) o = *opts;
+    else ct_frame_opts_default(&o);
+    int r = validate_opts(c, o);
+    if (r != CT_OK) return r;
+    if (!camera_ok(cam)) return fail(c, CT_ERR_INVALID_ARG, "camera is not finite or degenerate");
+    if (!c->accum) return fail(c, CT_ERR_INVALID_ARG, "no frame buffer (call ct_resize)");
+    (void)hipSetDevice(c->device);
+    const uint64_t total = (uint64_t)c->width * c->height * (uint64_t)o.spp;
+    if (total >= (1ull << 32)) return fail(c, CT_ERR_INVALID_ARG, "width*height*spp must be < 2^32");
+    std::vector<ViewDesc> views{make_view(*cam, c->width, c->height, o.spp, (uint32_t)c->samples, 0)};
+    std::vector<Run> runs{{0u, 0u, (uint32_t)total}};
+    const uint32_t cap = (uint32_t)std::min<uint64_t>(total, c->chunk_default);
+    ChunkStats st;
+    r = run_paths(c, views, runs, cap, c->accum, o, false, true, false, &st);
+    if (r != CT_OK) return r;
+    c->samples += o.spp;
+    fill_stats(st, stats);
+    for (int k = 0; k < 4; ++k) work[k] = st.work[k];
+    return CT_OK;
+}
+
+int ct_read_accum(ct_ctx* c, float* rgba, size_t n_floats, int32_t* samples) {
+    if (!c || !rgba) return CT_ERR_INVALID_ARG;
+    if (!c->accum) return fail(c, CT_ERR_INVALID_ARG, "no frame buffer");
+    const size_t need = (size_t)c->width * c->height * 4;
+    if (n_floats < need) return fail(c, CT_ERR_INVALID_ARG, "output buffer too small");
+    (void)hipSetDevice(c->device);
+    HIPCHK(hipMemcpyAsync(rgba, c->accum, need * 4, hipMemcpyDeviceToHost, c->stream), "read accumulator");
+    HIPCHK(hipStreamSynchronize(c->stream), "read accumulator");
+    if (samples) *samples = c->samples;
+    return CT_OK;
+}
+
+int ct_write_accum(ct_ctx* c, const float* rgba, size_t n_floats, int32_t samples) {
+    if (!c || !rgba || samples < 0) return CT_ERR_INVALID_ARG;
+    if (!c->accum) return fail(c, CT_ERR_INVALID_ARG, "no frame buffer");
+    const size_t need = (size_t)c->width * c->height * 4;
+    if (n_floats < need) return fail(c, CT_ERR_INVALID_ARG, "input buffer too small");
+    (void)hipSetDevice(c->device);
+    HIPCHK(hipMemcpyAsync(c->accum, rgba, need * 4, hipMemcpyHostToDevice, c->stream), "write accumulator");
+    HIPCHK(hipStreamSynchronize(c->stream), "write accumulator");
+    c->samples = samples;
+    return CT_OK;
+}
+
+int ct_read_image(ct_ctx* c, float* rgba, size_t n_floats) {
+    if (!c || !rgba) return CT_ERR_INVALID_ARG;
+    if (!c->accum) return fail(c, CT_ERR_INVALID_ARG, "no frame buffer");
+    const size_t npix = (size_t)c->width * c->height;
+    if (n_floats < npix * 4) return fail(c, CT_ERR_INVALID_ARG, "output buffer too small");
+    (void)hipSetDevice(c->device);
+    void* tmp = nullptr;
+    int r = dalloc(c, &tmp, npix * sizeof(float4), "image");
+    if (r != CT_OK) return r;
+    hipError_t e = ctk::launch_tonemap(c->accum, (uint32_t)npix, (float)std::max(c->samples, 1), (float4*)tmp, c->stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(rgba, tmp, npix * sizeof(float4), hipMemcpyDeviceToHost, c->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    dfree(c, tmp);
+    if (e != hipSuccess) return hip_fail(c, e, "tonemap");
+    return CT_OK;
+}
+
+static int ray_prologue(ct_ctx* c, const ct_frame_opts* opts, ct_frame_opts& o) {
+    if (opts) o = *opts;
+    else ct_frame_opts_default(&o);
+    if (o.spp < 1) o.spp = 1;
+    if (o.max_depth < 1) o.max_depth = 1;
+    return validate_opts(c, o);
+}
+
+int ct_trace_rays(ct_ctx* c, const float* origins, const float* dirs, uint32_t n, const ct_frame_opts* opts,
+                  float* t_out, int32_t* prim_out) {
+    if (!c || (n && (!origins || !dirs || !t_out || !prim_out))) return CT_ERR_INVALID_ARG;
+    ct_frame_opts o;
+    int r = ray_prologue(c, opts, o);
+    if (r != CT_OK) return r;
+    if (n == 0) return CT_OK;
+    (void)hipSetDevice(c->device);
+    const uint32_t cap = std::min<uint32_t>(n, 1u << 22);
+    r = ensure_paths(c, cap);
+    if (r != CT_OK) return r;
+    ctd::TraceOpts to{o.accel, o.traversal, o.triangle_epsilon};
+    std::vector<float4> ro(cap), rd(cap);
+    std::vector<float2> hits(cap);
+    for (uint32_t base = 0; base < n; base += cap) {
+        const uint32_t m = std::min(cap, n - base);
+        for (uint32_t i = 0; i < m; ++i) {
+            ro[i] = f4(origins + 3 * (size_t)(base + i), 0.0f);
+            rd[i] = f4(dirs + 3 * (size_t)(base + i), 0.0f);
+        }
+        const ctd::ChunkHdr h{m, 0u, 0u, 0u};
+        HIPCHK(hipMemcpyAsync(c->q.ray_o[0], ro.data(), m * sizeof(float4), hipMemcpyHostToDevice, c->stream), "rays");
+        HIPCHK(hipMemcpyAsync(c->q.ray_d[0], rd.data(), m * sizeof(float4), hipMemcpyHostToDevice, c->stream), "rays");
+        HIPCHK(hipMemcpyAsync(c->d_hdr, &h, sizeof(h), hipMemcpyHostToDevice, c->stream), "header");
+        HIPCHK(ctk::launch_setup(c->q.counters, c->d_hdr, c->stream), "setup");
+        HIPCHK(ctk::launch_extend(c->shape, c->sc, to, c->q, 0, 0, nullptr, c->stream), "extend");
+        uint32_t errflag = 0;
+        HIPCHK(hipMemcpyAsync(hits.data(), c->q.hit, m * sizeof(float2), hipMemcpyDeviceToHost, c->stream), "hits");
+        HIPCHK(hipMemcpyAsync(&errflag, c->q.err, 4, hipMemcpyDeviceToHost, c->stream), "flags");
+        HIPCHK(hipStreamSynchronize(c->stream), "trace rays");
+        if (errflag) {
+            (void)hipMemsetAsync(c->q.err, 0, 4, c->stream);
+            return fail(c, CT_ERR_STACK_OVERFLOW, "traversal stack bound exceeded");
+        }
+        for (uint32_t i = 0; i < m; ++i) {
+            t_out[base + i] = hits[i].x;
+            int32_t p;
+            std::memcpy(&p, &hits[i].y, 4);
+            prim_out[base + i] = p;
+        }
+    }
+    return CT_OK;
+}
+
+int ct_trace_shadow_rays(ct_ctx* c, const float* origins, const float* dirs, const float* tmax, uint32_t n,
+                         const ct_frame_opts* opts, uint8_t* occluded_out) {
+    if (!c || (n && (!origins || !dirs || !tmax || !occluded_out))) return CT_ERR_INVALID_ARG;
+    ct_frame_opts o;
+    int r = ray_prologue(c, opts, o);
+    if (r != CT_OK) return r;
+    if (n == 0) return CT_OK;
+    (void)hipSetDevice(c->device);
+    const uint32_t cap = std::min<uint32_t>(n, 1u << 22);
+    r = ensure_paths(c, cap);
+    if (r != CT_OK) return r;
+    void* dout = nullptr;
+    r = dalloc(c, &dout, cap, "occlusion flags");
+    if (r != CT_OK) return r;
+    ctd::TraceOpts to{o.accel, o.traversal, o.triangle_epsilon};
+    std::vector<float4> so(cap), sd(cap);
+    hipError_t e = hipSuccess;
+    uint32_t errflag = 0;
+    for (uint32_t base = 0; base < n && e == hipSuccess; base += cap) {
+        const uint32_t m = std::min(cap, n - base);
+        for (uint32_t i = 0; i < m; ++i) {
+            so[i] = f4(origins + 3 * (size_t)(base + i), tmax[base + i]);
+            sd[i] = f4(dirs + 3 * (size_t)(base + i), 0.0f);
+        }
+        const ctd::ChunkHdr h{m, 0u, 0u, 0u};
+        e = hipMemcpyAsync(c->q.sh_o, so.data(), m * sizeof(float4), hipMemcpyHostToDevice, c->stream);
+        if (e == hipSuccess) e = hipMemcpyAsync(c->q.sh_d, sd.data(), m * sizeof(float4), hipMemcpyHostToDevice, c->stream);
+        if (e == hipSuccess) e = hipMemcpyAsync(c->d_hdr, &h, sizeof(h), hipMemcpyHostToDevice, c->stream);
+        if (e == hipSuccess) e = ctk::launch_setup(c->q.counters, c->d_hdr, c->stream);
+        if (e == hipSuccess) e = ctk::launch_shadow_flags(c->shape, c->sc, to, c->q, m, (uint8_t*)dout, c->stream);
+        if (e == hipSuccess) e = hipMemcpyAsync(occluded_out + base, dout, m, hipMemcpyDeviceToHost, c->stream);
+        if (e == hipSuccess) e = hipMemcpyAsync(&errflag, c->q.err, 4, hipMemcpyDeviceToHost, c->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+        if (errflag) break;
+    }
+    dfree(c, dout);
+    if (e != hipSuccess) return hip_fail(c, e, "trace shadow rays");
+    if (errflag) {
+        (void)hipMemsetAsync(c->q.err, 0, 4, c->stream);
+        return fail(c, CT_ERR_STACK_OVERFLOW, "traversal stack bound exceeded");
+    }
+    return CT_OK;
+}
+
+int ct_trace_rays_device(ct_ctx* c, const void* d_rays_o, const void* d_rays_d, uint32_t n,
+                         const ct_frame_opts* opts, void* d_hits, float* kernel_ms) {
+    if (!c || (n && (!d_rays_o || !d_rays_d || !d_hits))) return CT_ERR_INVALID_ARG;
+    ct_frame_opts o;
+    int r = ray_prologue(c, opts, o);
+    if (r != CT_OK) return r;
+    (void)hipSetDevice(c->device);
+    r = ensure_paths(c, 64);
+    if (r != CT_OK) return r;
+    ctd::Queues q = c->q;
+    q.ray_o[0] = (float4*)d_rays_o;
+    q.ray_d[0] = (float4*)d_rays_d;
+    q.hit = (float2*)d_hits;
+    ctd::TraceOpts to{o.accel, o.traversal, o.triangle_epsilon};
+    const ctd::ChunkHdr h{n, 0u, 0u, 0u};
+    HIPCHK(hipMemcpyAsync(c->d_hdr, &h, sizeof(h), hipMemcpyHostToDevice, c->stream), "header");
+    HIPCHK(ctk::launch_setup(c->q.counters, c->d_hdr, c->stream), "setup");
+    HIPCHK(hipEventRecord(c->ev[0], c->stream), "event");
+    HIPCHK(ctk::launch_extend(c->shape, c->sc, to, q, 0, 0, nullptr, c->stream), "extend");
+    HIPCHK(hipEventRecord(c->ev[1], c->stream), "event");
+    HIPCHK(hipStreamSynchronize(c->stream), "trace rays");
+    float ms = 0;
+    HIPCHK(hipEventElapsedTime(&ms, c->ev[0], c->ev[1]), "elapsed");
+    if (kernel_ms) *kernel_ms = ms;
+    return CT_OK;
+}
+
+int ct_render_batch(ct_ctx* c, const ct_view* views, int32_t n_views, const ct_frame_opts* opts,
+                    ct_batch_stats* stats) {
+    if (!c || !views || n_views <= 0) return CT_ERR_INVALID_ARG;
+    ct_frame_opts o;
+    if (opts) o = *opts;
+    else ct_frame_opts_default(&o);
+    o.spp = 1;
+    int r = validate_opts(c, o);
+    if (r != CT_OK) return r;
+    (void)hipSetDevice(c->device);
+    std::vector<ViewDesc> vd;
+    std::vector<Run> runs;
+    uint64_t pix = 0, total = 0;
+    for (int32_t i = 0; i < n_views; ++i) {
+        const ct_view& v = views[i];
+        if (!camera_ok(&v.camera) || v.width <= 0 || v.height <= 0 || v.spp <= 0 || !v.accum_out)
+            return fail(c, CT_ERR_INVALID_ARG, "bad view");
+        const uint64_t np = (uint64_t)v.width * v.height * (uint64_t)v.spp;
+        if (np >= (1ull << 32)) return fail(c, CT_ERR_INVALID_ARG, "view too large");
+        vd.push_back(make_view(v.camera, v.width, v.height, v.spp, v.frame_base, pix));
+        runs.push_back({(uint32_t)i, 0u, (uint32_t)np});
+        pix += (uint64_t)v.width * v.height;
+        total += np;
+    }
+    float4* arena = nullptr;
+    r = dalloc(c, (void**)&arena, pix * sizeof(float4), "batch accumulators");
+    if (r != CT_OK) return r;
+    hipError_t e = hipMemsetAsync(arena, 0, pix * sizeof(float4), c->stream);
+    if (e != hipSuccess) { dfree(c, arena); return hip_fail(c, e, "clear batch accumulators"); }
+    const uint32_t cap = (uint32_t)std::min<uint64_t>(total, c->chunk_default);
+    ChunkStats st;
+    int chunks_before = 0;
+    (void)chunks_before;
+    r = run_paths(c, vd, runs, cap, arena, o, false, false, true, &st);
+    if (r == CT_OK) {
+        for (int32_t i = 0; i < n_views && e == hipSuccess; ++i)
+            e = hipMemcpyAsync(views[i].accum_out, arena + vd[i].accum_offset,
+                               (size_t)views[i].width * views[i].height * sizeof(float4), hipMemcpyDeviceToHost,
+                               c->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+        if (e != hipSuccess) r = hip_fail(c, e, "read batch accumulators");
+    }
+    // the captured graph refers to the arena; drop it with the arena
+    if (c->graph_exec) { (void)hipGraphExecDestroy(c->graph_exec); c->graph_exec = nullptr; c->graph_key.clear(); }
+    dfree(c, arena);
+    if (r != CT_OK) return r;
+    if (stats) {
+        stats->primary_rays = st.primary;
+        stats->extension_rays = st.extension;
+        stats->shadow_rays = st.shadow;
+        stats->samples = st.samples;
+        stats->gpu_ms = st.total_ms;
+        stats->chunks = (int32_t)((total + cap - 1) / cap);
+    }
+    return CT_OK;
+}
+
+int ct_autotune(ct_ctx* c, const ct_camera* cam, const ct_frame_opts* opts, int32_t probe_width,
+                int32_t probe_height, ct_tune_result* out) {
+    if (!c || !camera_ok(cam) || probe_width <= 0 || probe_height <= 0) return CT_ERR_INVALID_ARG;
+    ct_frame_opts o;
+    if (opts) o = *opts;
+    else ct_frame_opts_default(&o);
+    o.spp = 1;
+    int r = validate_opts(c, o);
+    if (r != CT_OK) return r;
+    (void)hipSetDevice(c->device);
+    const uint64_t npix = (uint64_t)probe_width * probe_height;
+    float4* acc = nullptr;
+    r = dalloc(c, (void**)&acc, npix * sizeof(float4), "autotune accumulator");
+    if (r != CT_OK) return r;
+    const ctk::LaunchShape saved = c->shape;
+    const uint32_t saved_chunk = c->chunk_default;
+    std::vector<ViewDesc> views{make_view(*cam, probe_width, probe_height, 1, 0u, 0)};
+    std::vector<Run> runs{{0u, 0u, (uint32_t)npix}};
+    double best = 1e30;
+    int best_block = saved.block, best_lds = saved.lds_depth, trials = 0;
+    const int blocks[] = {64, 128, 256};
+    const int ldss[] = {8, 16, 24, 32};
+    for (int b : blocks)
+        for (int l : ldss) {
+            c->shape.block = b;
+            c->shape.lds_depth = l;
+            int per_cu = ctk::trace_blocks_per_cu(b, l);
+            if (per_cu <= 0) continue;
+            c->shape.trace_blocks = std::max(1, c->num_cus * per_cu);
+            double t = 0;
+            bool ok = true;
+            for (int rep = 0; rep < 3 && ok; ++rep) {
+                ChunkStats st;
+                r = run_paths(c, views, runs, (uint32_t)npix, acc, o, false, false, false, &st);
+                if (r != CT_OK) { ok = false; break; }
+                if (rep > 0) t += st.total_ms;  // first run warms up
+            }
+            if (!ok) { c->shape = saved; dfree(c, acc); return r; }
+            ++trials;
+            if (t < best) { best = t; best_block = b; best_lds = l; }
+        }
+    ctr_apply_shape(c, best_block, best_lds);
+    // chunk size: paths per wavefront chunk for multi-sample work
+    const uint32_t chunks[] = {1u << 20, 1u << 21, 1u << 22};
+    const int spp_probe = (int)std::max<uint64_t>(1, (1u << 22) / npix);
+    std::vector<ViewDesc> v2{make_view(*cam, probe_width, probe_height, spp_probe, 0u, 0)};
+    std::vector<Run> r2{{0u, 0u, (uint32_t)(npix * spp_probe)}};
+    double best_c = 1e30;
+    uint32_t best_chunk = saved_chunk;
+    for (uint32_t ch : chunks) {
+        ChunkStats st;
+        r = run_paths(c, v2, r2, ch, acc, o, false, false, false, &st);
+        if (r != CT_OK) break;
+        ++trials;
+        if (st.total_ms < best_c) { best_c = st.total_ms; best_chunk = ch; }
+    }
+    dfree(c, acc);
+    if (r != CT_OK) { c->shape = saved; c->chunk_default = saved_chunk; return r; }
+    c->chunk_default = best_chunk;
+    if (out) {
+        out->block_size = best_block;
+        out->stack_lds = best_lds;
+        out->chunk_paths = (int32_t)best_chunk;
+        out->best_ms = best / 2.0;
+        out->trials = trials;
+    }
+    return CT_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,98 @@
+// Internal runtime of the C ABI: the context, its device buffers and the
+// chunk driver shared by the plain frame loop, the batch API and the server.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../../include/cudatracer.h"
+#include "../device/dev_common.h"
+#include "../device/launch.h"
+
+struct ChunkStats {
+    uint64_t primary = 0, extension = 0, shadow = 0, samples = 0;
+    double extend_ms = 0, shade_ms = 0, connect_ms = 0, other_ms = 0, total_ms = 0;
+    uint64_t work[4] = {0, 0, 0, 0};  // closest-hit node/tri tests, shadow node/tri tests
+    int extend_launches = 0;
+};
+
+struct ct_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    std::string name;
+    int num_cus = 0;
+    uint64_t budget = 0, used = 0;
+    std::vector<std::pair<void*, size_t>> allocs;
+
+    ctk::LaunchShape shape;
+    uint32_t chunk_default = 1u << 22;  // paths per chunk
+
+    // scene
+    bool has_scene = false;
+    bool has_bvh2 = false, has_bvh4 = false;
+    ctd::DevScene sc{};
+    std::vector<void*> scene_bufs;
+    std::vector<ct_material> h_mats;
+    std::vector<uint32_t> h_mat_idx, h_light_idx;
+    std::vector<float> h_light_rows;  // 20 floats per light
+
+    // plain frame accumulator
+    int width = 0, height = 0;
+    int32_t samples = 0;
+    float4* accum = nullptr;
+
+    // path state
+    uint32_t cap = 0;
+    ctd::Queues q{};
+    std::vector<void*> path_bufs;
+    uint32_t spill_threads = 0;
+    unsigned long long* wstats = nullptr;
+    ctd::ViewDesc* d_views = nullptr;
+    uint32_t d_views_cap = 0;
+    ctd::Segment* d_segs = nullptr;
+    uint32_t d_segs_cap = 0;
+    ctd::ChunkHdr* d_hdr = nullptr;
+
+    // captured bounce loop
+    hipGraphExec_t graph_exec = nullptr;
+    std::string graph_key;
+
+    hipEvent_t ev[2 * (3 * ctd::kMaxBounce + 8)] = {};
+    int n_events = 0;
+};
+
+namespace ctr {
+
+int fail(ct_ctx* c, int code, const std::string& msg);
+int hip_fail(ct_ctx* c, hipError_t e, const char* what);
+int dalloc(ct_ctx* c, void** p, size_t bytes, const char* what);
+void dfree(ct_ctx* c, void* p);
+
+int validate_opts(ct_ctx* c, const ct_frame_opts& o);
+int ensure_paths(ct_ctx* c, uint32_t cap);
+int ensure_tables(ct_ctx* c, uint32_t nviews, uint32_t nsegs);
+ctd::ViewDesc make_view(const ct_camera& cam, int width, int height, int spp, uint32_t frame_base,
+                        uint64_t accum_offset);
+
+// Run one chunk: views/segments already describe the chunk (host copies are
+// uploaded here).  use_graph replays a captured bounce loop.
+int run_chunk(ct_ctx* c, const std::vector<ctd::ViewDesc>& views, const std::vector<ctd::Segment>& segs,
+              uint32_t n, uint32_t n_tasks, float4* accum, const ct_frame_opts& o, bool timed, bool count,
+              bool use_graph, ChunkStats* st);
+
+// Split a list of (view, first_path, n) runs into chunks of at most cap paths
+// and run them in order.
+struct Run {
+    uint32_t view, first_path, n;
+};
+int run_paths(ct_ctx* c, const std::vector<ctd::ViewDesc>& views, const std::vector<Run>& runs, uint32_t cap,
+              float4* accum, const ct_frame_opts& o, bool timed, bool count, bool use_graph, ChunkStats* st);
+
+}  // namespace ctr
+
+// Set the launch shape (block, LDS stack depth) and size the persistent grid.
+extern "C" void ctr_apply_shape(ct_ctx* c, int block, int lds_depth);

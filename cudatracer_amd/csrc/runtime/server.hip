@@ -1,0 +1,366 @@
+// Render server: continuous batching of render requests on one context.
+//
+// The reference renders one view per launchKernels call from its UI loop
+// (Application.cpp:101-125); it has no request queue.  Here a scheduler
+// thread owns the context:
+//   - submit() queues a request (camera, size, spp, frame base); a request
+//     whose footprint can never fit the budget is refused (CT_ERR_ADMISSION);
+//   - requests are admitted in arrival order while their footprint
+//     (accumulator + the path-state queue slots they can occupy) fits the
+//     memory budget, so requests that arrive while others are on the GPU join
+//     the next chunk;
+//   - every chunk packs path samples of all admitted requests (oldest first)
+//     into one wavefront: one generate, depth x (extend, shade, connect),
+//     one resolve — replayed from a captured hipGraph;
+//   - a request is finished when its last path sample has been resolved; its
+//     accumulator is copied out and its memory released.
+// A request's result does not depend on what it was batched with: the RNG is
+// keyed by (pixel, frame_base + sample, bounce) and samples are summed in
+// sample order.
+#include <algorithm>
+#include <chrono>
+#include <condition_variable>
+#include <cstring>
+#include <deque>
+#include <map>
+#include <mutex>
+#include <thread>
+
+#include "runtime.h"
+
+using ctd::ViewDesc;
+using namespace ctr;
+
+namespace {
+
+constexpr uint64_t kBytesPerPath = 176;  // path-state queue bytes per in-flight path sample
+
+enum ReqState { PENDING = 0, ACTIVE = 1, DONE = 2 };
+
+struct Req {
+    uint64_t id = 0;
+    ct_request r{};
+    int state = PENDING;
+    int status = CT_OK;
+    uint64_t total = 0, next = 0;  // path samples
+    uint64_t footprint = 0;
+    uint64_t acc_off = 0;          // float4 offset in the arena
+    double submit_ms = 0, admit_ms = 0, done_ms = 0;
+    uint64_t rays = 0;
+    bool cancel = false;
+    std::vector<float> result;
+};
+
+// First-fit allocator over the accumulator arena (units: float4).
+struct Arena {
+    std::map<uint64_t, uint64_t> free_;  // offset -> length
+    explicit Arena(uint64_t n = 0) { if (n) free_[0] = n; }
+    bool alloc(uint64_t n, uint64_t* off) {
+        for (auto it = free_.begin(); it != free_.end(); ++it)
+            if (it->second >= n) {
+                *off = it->first;
+                const uint64_t rest = it->second - n, at = it->first + n;
+                free_.erase(it);
+                if (rest) free_[at] = rest;
+                return true;
+            }
+        return false;
+    }
+    void release(uint64_t off, uint64_t n) {
+        auto it = free_.emplace(off, n).first;
+        auto nx = std::next(it);
+        if (nx != free_.end() && it->first + it->second == nx->first) {
+            it->second += nx->second;
+            free_.erase(nx);
+        }
+        if (it != free_.begin()) {
+            auto pv = std::prev(it);
+            if (pv->first + pv->second == it->first) {
+                pv->second += it->second;
+                free_.erase(it);
+            }
+        }
+    }
+};
+
+}  // namespace
+
+struct ct_server {
+    ct_ctx* ctx = nullptr;
+    ct_server_opts opts{};
+    ct_frame_opts fopts{};
+    bool have_fopts = false;
+    std::mutex mu;
+    std::condition_variable cv;
+    std::thread worker;
+    bool stop = false;
+    uint64_t next_id = 1;
+    std::map<uint64_t, Req> reqs;
+    std::deque<uint64_t> pending;
+    std::vector<uint64_t> active;  // admission order
+    uint64_t in_use = 0;
+    float4* arena_ptr = nullptr;
+    uint64_t arena_len = 0;  // float4
+    Arena arena;
+    uint32_t chunk = 0;
+    std::chrono::steady_clock::time_point t0;
+    int fatal = CT_OK;
+    std::string fatal_msg;
+
+    double now() const {
+        return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    }
+    void loop();
+};
+
+void ct_server::loop() {
+    (void)hipSetDevice(ctx->device);
+    std::vector<ViewDesc> views;
+    std::vector<Run> runs;
+    std::vector<uint64_t> ids;
+    while (true) {
+        {
+            std::unique_lock<std::mutex> lk(mu);
+            // drop cancelled requests
+            for (auto it = active.begin(); it != active.end();) {
+                Req& q = reqs[*it];
+                if (q.cancel) {
+                    arena.release(q.acc_off, (uint64_t)q.r.width * q.r.height);
+                    in_use -= q.footprint;
+                    q.state = DONE;
+                    q.status = CT_ERR_CANCELLED;
+                    q.done_ms = now();
+                    it = active.erase(it);
+                } else {
+                    ++it;
+                }
+            }
+            // admit in arrival order while the footprint fits
+            while (!pending.empty() && (opts.max_active <= 0 || (int)active.size() < opts.max_active)) {
+                Req& q = reqs[pending.front()];
+                if (q.cancel) {
+                    q.state = DONE;
+                    q.status = CT_ERR_CANCELLED;
+                    q.done_ms = now();
+                    pending.pop_front();
+                    continue;
+                }
+                if (in_use + q.footprint > opts.mem_budget) break;
+                uint64_t off = 0;
+                if (!arena.alloc((uint64_t)q.r.width * q.r.height, &off)) break;
+                q.acc_off = off;
+                q.state = ACTIVE;
+                q.admit_ms = now();
+                in_use += q.footprint;
+                active.push_back(q.id);
+                pending.pop_front();
+                cv.notify_all();
+            }
+            if (active.empty()) {
+                if (stop) return;
+                cv.wait(lk);
+                continue;
+            }
+            // pack one chunk, oldest request first
+            views.clear();
+            runs.clear();
+            ids.clear();
+            uint64_t room = chunk;
+            for (uint64_t id : active) {
+                if (room == 0) break;
+                Req& q = reqs[id];
+                const uint64_t take = std::min<uint64_t>(room, q.total - q.next);
+                if (take == 0) continue;
+                views.push_back(make_view(q.r.camera, q.r.width, q.r.height, q.r.spp, q.r.frame_base, q.acc_off));
+                runs.push_back({(uint32_t)(views.size() - 1), (uint32_t)q.next, (uint32_t)take});
+                ids.push_back(id);
+                q.next += take;
+                room -= take;
+            }
+        }
+        // zero accumulators of requests starting in this chunk, then trace
+        int rc = CT_OK;
+        for (size_t k = 0; k < ids.size() && rc == CT_OK; ++k)
+            if (runs[k].first_path == 0) {
+                const Req& q = reqs[ids[k]];
+                if (hipMemsetAsync(arena_ptr + q.acc_off, 0, (size_t)q.r.width * q.r.height * sizeof(float4),
+                                   ctx->stream) != hipSuccess)
+                    rc = CT_ERR_DEVICE;
+            }
+        ChunkStats st;
+        if (rc == CT_OK)
+            rc = run_paths(ctx, views, runs, chunk, arena_ptr, fopts, false, false, true, &st);
+        std::unique_lock<std::mutex> lk(mu);
+        if (rc != CT_OK) {
+            fatal = rc;
+            fatal_msg = ctx->err;
+            for (uint64_t id : active) {
+                Req& q = reqs[id];
+                q.state = DONE;
+                q.status = rc;
+                q.done_ms = now();
+            }
+            active.clear();
+            cv.notify_all();
+            continue;
+        }
+        // rays are attributed to requests in proportion to their path samples
+        uint64_t paths = 0;
+        for (const Run& r : runs) paths += r.n;
+        const uint64_t rays = st.primary + st.extension + st.shadow;
+        for (size_t k = 0; k < ids.size(); ++k)
+            reqs[ids[k]].rays += paths ? rays * runs[k].n / paths : 0;
+        for (auto it = active.begin(); it != active.end();) {
+            Req& q = reqs[*it];
+            if (q.next >= q.total) {
+                const size_t n = (size_t)q.r.width * q.r.height * 4;
+                q.result.resize(n);
+                lk.unlock();
+                const hipError_t e = hipMemcpy(q.result.data(), arena_ptr + q.acc_off, n * sizeof(float),
+                                               hipMemcpyDeviceToHost);
+                lk.lock();
+                q.status = e == hipSuccess ? CT_OK : CT_ERR_DEVICE;
+                q.state = DONE;
+                q.done_ms = now();
+                arena.release(q.acc_off, (uint64_t)q.r.width * q.r.height);
+                in_use -= q.footprint;
+                it = active.erase(it);
+                cv.notify_all();
+            } else {
+                ++it;
+            }
+        }
+    }
+}
+
+extern "C" {
+
+int ct_server_create(ct_ctx* ctx, const ct_server_opts* opts, ct_server** out) {
+    if (!ctx || !out) return CT_ERR_INVALID_ARG;
+    *out = nullptr;
+    if (!ctx->has_scene) return fail(ctx, CT_ERR_NO_SCENE, "no scene uploaded");
+    ct_server* s = new ct_server();
+    s->ctx = ctx;
+    if (opts) s->opts = *opts;
+    if (s->opts.mem_budget == 0) s->opts.mem_budget = 8ull << 30;
+    s->chunk = s->opts.chunk_paths > 0 ? (uint32_t)s->opts.chunk_paths : ctx->chunk_default;
+    (void)hipSetDevice(ctx->device);
+    int r = ensure_paths(ctx, s->chunk);
+    if (r == CT_OK) {
+        s->arena_len = s->opts.mem_budget / sizeof(float4);
+        r = dalloc(ctx, (void**)&s->arena_ptr, s->arena_len * sizeof(float4), "server accumulator arena");
+    }
+    if (r != CT_OK) {
+        delete s;
+        return r;
+    }
+    s->arena = Arena(s->arena_len);
+    s->t0 = std::chrono::steady_clock::now();
+    s->worker = std::thread([s] { s->loop(); });
+    *out = s;
+    return CT_OK;
+}
+
+int ct_server_submit(ct_server* s, const ct_request* req, uint64_t* id) {
+    if (!s || !req || !id) return CT_ERR_INVALID_ARG;
+    const ct_request& r = *req;
+    if (r.width <= 0 || r.height <= 0 || r.spp <= 0) return CT_ERR_INVALID_ARG;
+    const uint64_t total = (uint64_t)r.width * r.height * (uint64_t)r.spp;
+    if (total >= (1ull << 32)) return CT_ERR_INVALID_ARG;
+    const uint64_t fp = (uint64_t)r.width * r.height * sizeof(float4) + std::min<uint64_t>(total, s->chunk) * kBytesPerPath;
+    std::lock_guard<std::mutex> lk(s->mu);
+    if (fp > s->opts.mem_budget) return fail(s->ctx, CT_ERR_ADMISSION, "request can never fit the memory budget");
+    if (s->fatal != CT_OK) return s->fatal;
+    if (!s->have_fopts) {
+        ct_frame_opts o = r.opts;
+        o.spp = 1;
+        const int v = validate_opts(s->ctx, o);
+        if (v != CT_OK) return v;
+        s->fopts = o;
+        s->have_fopts = true;
+    } else if (r.opts.max_depth != s->fopts.max_depth || r.opts.accel != s->fopts.accel ||
+               r.opts.traversal != s->fopts.traversal || r.opts.bsdf_mode != s->fopts.bsdf_mode ||
+               r.opts.rr_start != s->fopts.rr_start || r.opts.shadow != s->fopts.shadow) {
+        return fail(s->ctx, CT_ERR_INVALID_ARG, "request options differ from the server's");
+    }
+    Req q;
+    q.id = s->next_id++;
+    q.r = r;
+    q.total = total;
+    q.footprint = fp;
+    q.submit_ms = s->now();
+    s->reqs[q.id] = q;
+    s->pending.push_back(q.id);
+    *id = q.id;
+    s->cv.notify_all();
+    return CT_OK;
+}
+
+static void fill_result(const Req& q, ct_result* res) {
+    if (!res) return;
+    res->status = q.status;
+    res->width = q.r.width;
+    res->height = q.r.height;
+    res->spp = q.r.spp;
+    res->submit_ms = q.submit_ms;
+    res->admit_ms = q.admit_ms;
+    res->done_ms = q.done_ms;
+    res->rays = q.rays;
+}
+
+int ct_server_poll(ct_server* s, uint64_t id, int32_t* done, ct_result* res, float* accum_out, size_t n_floats) {
+    if (!s || !done) return CT_ERR_INVALID_ARG;
+    std::lock_guard<std::mutex> lk(s->mu);
+    auto it = s->reqs.find(id);
+    if (it == s->reqs.end()) return CT_ERR_NOT_FOUND;
+    Req& q = it->second;
+    *done = q.state == DONE ? 1 : 0;
+    if (q.state != DONE) return CT_OK;
+    fill_result(q, res);
+    if (accum_out && q.status == CT_OK) {
+        if (n_floats < q.result.size()) return CT_ERR_INVALID_ARG;
+        std::memcpy(accum_out, q.result.data(), q.result.size() * sizeof(float));
+    }
+    return CT_OK;
+}
+
+int ct_server_wait(ct_server* s, uint64_t id, ct_result* res, float* accum_out, size_t n_floats) {
+    if (!s) return CT_ERR_INVALID_ARG;
+    {
+        std::unique_lock<std::mutex> lk(s->mu);
+        auto it = s->reqs.find(id);
+        if (it == s->reqs.end()) return CT_ERR_NOT_FOUND;
+        s->cv.wait(lk, [&] { return s->reqs[id].state == DONE; });
+    }
+    int32_t done = 0;
+    return ct_server_poll(s, id, &done, res, accum_out, n_floats);
+}
+
+int ct_server_cancel(ct_server* s, uint64_t id) {
+    if (!s) return CT_ERR_INVALID_ARG;
+    std::lock_guard<std::mutex> lk(s->mu);
+    auto it = s->reqs.find(id);
+    if (it == s->reqs.end()) return CT_ERR_NOT_FOUND;
+    it->second.cancel = true;
+    s->cv.notify_all();
+    return CT_OK;
+}
+
+double ct_server_clock_ms(const ct_server* s) { return s ? s->now() : 0.0; }
+
+void ct_server_destroy(ct_server* s) {
+    if (!s) return;
+    {
+        std::lock_guard<std::mutex> lk(s->mu);
+        s->stop = true;
+        for (uint64_t id : s->pending) s->reqs[id].cancel = true;
+        for (uint64_t id : s->active) s->reqs[id].cancel = true;
+        s->cv.notify_all();
+    }
+    if (s->worker.joinable()) s->worker.join();
+    (void)hipStreamSynchronize(s->ctx->stream);
+    dfree(s->ctx, s->arena_ptr);
+    delete s;
+}
+
+}  // extern "C"

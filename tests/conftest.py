@@ -1,0 +1,117 @@
+"""Shared fixtures: config scenes, their oracle twins, seeded ray sets and a
+GPU context (session scoped, created only by tests marked gpu)."""
+from __future__ import annotations
+
+import sys
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+ROOT = Path(__file__).resolve().parents[1]
+sys.path.insert(0, str(ROOT))
+
+
+def pytest_configure(config):
+    config.addinivalue_line("markers", "gpu: needs an MI355X (gfx950) GPU")
+    config.addinivalue_line("markers", "slow: long-running")
+
+
+@pytest.fixture(scope="session")
+def ct():
+    import cudatracer_amd
+    return cudatracer_amd
+
+
+@pytest.fixture(scope="session")
+def orc():
+    from oracle import oracle
+    return oracle
+
+
+_SCENES = {}
+
+
+def get_scene(ct, orc, config: int, seed: int = 7):
+    key = (config, seed)
+    if key not in _SCENES:
+        s = ct.HostScene.generate(config, seed)
+        arrays = s.arrays()
+        _SCENES[key] = (s, arrays, orc.Scene(arrays))
+    return _SCENES[key]
+
+
+@pytest.fixture(scope="session")
+def c1(ct, orc):
+    return get_scene(ct, orc, 1)
+
+
+@pytest.fixture(scope="session")
+def c2(ct, orc):
+    return get_scene(ct, orc, 2)
+
+
+@pytest.fixture(scope="session")
+def gpu_ctx(ct):
+    ctx = ct.Context(0)
+    yield ctx
+    ctx.close()
+
+
+def camera_rays(cam, width: int, height: int, n: int, seed: int):
+    """Jittered pinhole rays of the camera (same basis as the kernels)."""
+    rng = np.random.default_rng(seed)
+    pos = np.array(cam.position, np.float32)
+    w = np.array(cam.forward, np.float32)
+    up = np.array(cam.up, np.float32)
+    u = np.cross(w, up)
+    u = u / np.linalg.norm(u)
+    v = np.cross(u, w)
+    v = v / np.linalg.norm(v)
+    hor = u * cam.plane_distance * (width / height if width > height else 1.0)
+    ver = v * cam.plane_distance * (1.0 if width > height else width / height)
+    px = rng.uniform(0, width, n).astype(np.float32)
+    py = rng.uniform(0, height, n).astype(np.float32)
+    sx = 2.0 * px / width - 1.0
+    sy = 1.0 - 2.0 * py / height
+    d = w[None] + hor[None] * sx[:, None] + ver[None] * sy[:, None]
+    d = (d / np.linalg.norm(d, axis=1, keepdims=True)).astype(np.float32)
+    o = np.repeat(pos[None], n, axis=0).astype(np.float32)
+    return o, d
+
+
+def surface_rays(arrays: dict, n: int, seed: int, offset: float = 1e-3):
+    """Cosine-distributed rays leaving seeded points on random triangles."""
+    rng = np.random.default_rng(seed)
+    idx = arrays["indices"]
+    V = arrays["vertices"].astype(np.float64)
+    tri = rng.integers(0, idx.shape[0], n)
+    r1 = rng.random(n)
+    r2 = rng.random(n)
+    flip = r1 + r2 > 1
+    r1[flip], r2[flip] = 1 - r1[flip], 1 - r2[flip]
+    p0, p1, p2 = V[idx[tri, 0]], V[idx[tri, 1]], V[idx[tri, 2]]
+    p = (1 - r1 - r2)[:, None] * p0 + r1[:, None] * p1 + r2[:, None] * p2
+    nrm = np.cross(p1 - p0, p2 - p0)
+    nrm /= np.linalg.norm(nrm, axis=1, keepdims=True)
+    side = np.where(rng.random(n) < 0.5, -1.0, 1.0)
+    nrm *= side[:, None]
+    a = np.where(np.abs(nrm[:, 0:1]) > 0.9, np.array([[0.0, 1.0, 0.0]]), np.array([[1.0, 0.0, 0.0]]))
+    t = np.cross(nrm, a)
+    t /= np.linalg.norm(t, axis=1, keepdims=True)
+    b = np.cross(nrm, t)
+    u1, u2 = rng.random(n), rng.random(n)
+    r = np.sqrt(u1)
+    phi = 2 * np.pi * u2
+    d = (r * np.cos(phi))[:, None] * t + (r * np.sin(phi))[:, None] * b + np.sqrt(1 - u1)[:, None] * nrm
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    o = p + offset * nrm
+    return o.astype(np.float32), d.astype(np.float32)
+
+
+def ray_set(scene, arrays, n: int, seed: int):
+    """Half camera rays, half surface rays (the parity ray set)."""
+    cam = scene.camera(640, 480)
+    o1, d1 = camera_rays(cam, 640, 480, n // 2, seed)
+    o2, d2 = surface_rays(arrays, n - n // 2, seed + 1)
+    return np.concatenate([o1, o2]), np.concatenate([d1, d2])

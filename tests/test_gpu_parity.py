@@ -1,0 +1,187 @@
+"""GPU parity: the HIP kernels (through the C ABI) against the CPU oracle.
+
+Bars (exact traversal): closest-hit primitive ids and hit distances
+bit-identical; shadow (any-hit) flags identical; per-pixel FP32 radiance
+within 1e-4 relative (observed: bit-identical) and relative MSE against the
+FP64 oracle <= 1e-3.
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from conftest import ray_set, surface_rays
+
+pytestmark = pytest.mark.gpu
+
+N_RAYS = 1 << 20
+
+
+def _upload(gpu_ctx, scene):
+    gpu_ctx.upload(scene)
+
+
+@pytest.mark.parametrize("config,accel", [(1, 0), (1, 1), (2, 1)])
+def test_closest_hit_bit_exact(ct, orc, gpu_ctx, config, accel):
+    from conftest import get_scene
+    scene, arrays, osc = get_scene(ct, orc, config)
+    _upload(gpu_ctx, scene)
+    o, d = ray_set(scene, arrays, N_RAYS, seed=100 + config)
+    opts, _, _ = scene.frame_opts()
+    opts.accel = accel
+    opts.traversal = ct.TRAV_EXACT
+    t_gpu, p_gpu = gpu_ctx.trace_rays(o, d, opts)
+    t_ref, p_ref = osc.trace(o, d, accel, 32, opts.triangle_epsilon, threads=16)
+    t_ref32 = t_ref.astype(np.float32)
+    mismatch = np.flatnonzero(p_gpu != p_ref)
+    print(f"config {config} accel {accel}: rays {N_RAYS}, hits {int(np.sum(p_ref >= 0))}, "
+          f"id mismatches {mismatch.size}")
+    assert mismatch.size == 0, f"first mismatches {mismatch[:8]} gpu {p_gpu[mismatch[:8]]} ref {p_ref[mismatch[:8]]}"
+    assert np.array_equal(t_gpu.view(np.uint32), t_ref32.view(np.uint32))
+
+
+def test_closest_hit_matches_brute_force_c1(ct, orc, gpu_ctx, c1):
+    """Against exhaustive search: same t everywhere, ids equal except exact ties."""
+    scene, arrays, osc = c1
+    _upload(gpu_ctx, scene)
+    o, d = ray_set(scene, arrays, 1 << 14, seed=5)
+    opts, _, _ = scene.frame_opts()
+    opts.accel = ct.ACCEL_BVH4
+    t_gpu, p_gpu = gpu_ctx.trace_rays(o, d, opts)
+    t_bf, p_bf = osc.trace(o, d, 2, 32, opts.triangle_epsilon, threads=16)
+    assert np.array_equal(t_gpu, t_bf.astype(np.float32))
+    ties = np.flatnonzero(p_gpu != p_bf)
+    print("exact ties (same t, different id):", ties.size)
+    assert ties.size <= 4
+
+
+@pytest.mark.parametrize("config,accel", [(1, 0), (1, 1), (2, 1)])
+def test_shadow_any_hit(ct, orc, gpu_ctx, config, accel):
+    from conftest import get_scene
+    scene, arrays, osc = get_scene(ct, orc, config)
+    _upload(gpu_ctx, scene)
+    n = 1 << 18
+    o, _ = surface_rays(arrays, n, seed=11)
+    rng = np.random.default_rng(12)
+    # targets: random points on light triangles or random surface points
+    lidx = arrays["light_indices"]
+    tri = lidx[rng.integers(0, lidx.size, n)]
+    V = arrays["vertices"].astype(np.float64)
+    I = arrays["indices"]
+    r1, r2 = rng.random(n), rng.random(n)
+    f = r1 + r2 > 1
+    r1[f], r2[f] = 1 - r1[f], 1 - r2[f]
+    tgt = (1 - r1 - r2)[:, None] * V[I[tri, 0]] + r1[:, None] * V[I[tri, 1]] + r2[:, None] * V[I[tri, 2]]
+    dv = tgt - o
+    dist = np.linalg.norm(dv, axis=1)
+    d = (dv / dist[:, None]).astype(np.float32)
+    tmax = (dist - 2e-3).astype(np.float32)
+    opts, _, _ = scene.frame_opts()
+    opts.accel = accel
+    occ_gpu = gpu_ctx.trace_shadow_rays(o, d, tmax, opts)
+    occ_ref = osc.trace_shadow(o, d, tmax, accel, 32, opts.triangle_epsilon, threads=16)
+    print(f"config {config} accel {accel}: occluded {int(occ_ref.sum())} of {n}")
+    assert np.array_equal(occ_gpu, occ_ref)
+
+
+def _relmse(x, ref):
+    return float(np.mean((x - ref) ** 2 / (ref ** 2 + 1e-2)))
+
+
+def test_radiance_c1_16_frames(ct, orc, gpu_ctx, c1):
+    """Config 1: 256x256, 1 spp x 16 frames, depth 4, NEE, binary tree."""
+    scene, arrays, osc = c1
+    _upload(gpu_ctx, scene)
+    opts, w, h = scene.frame_opts()
+    cam = scene.camera(w, h)
+    gpu_ctx.resize(w, h)
+    for _ in range(16):
+        gpu_ctx.trace_frame(cam, opts)
+    acc, samples = gpu_ctx.read_accum()
+    assert samples == 16
+    ref = np.zeros((h, w, 4), np.float32)
+    for f in range(16):
+        osc.render(cam, opts, w, h, f, 1, threads=16, accum=ref)
+    diff = np.abs(acc - ref)
+    rel = diff / np.maximum(np.abs(ref), 1e-30)
+    bad = int(np.sum((rel > 1e-4) & (diff > 1e-7)))
+    exact = int(np.sum(acc.view(np.uint32) == ref.view(np.uint32)))
+    print(f"C1 16 frames: pixels over 1e-4 rel {bad}, bit-identical values {exact}/{acc.size}")
+    assert bad == 0
+    ref64, _ = osc.render(cam, opts, w, h, 0, 16, precision=64, threads=16)
+    img = acc[..., :3] / acc[..., 3:]
+    img64 = ref64[..., :3] / ref64[..., 3:]
+    r = _relmse(img, img64)
+    print(f"C1 rel-MSE vs FP64 oracle: {r:.3e}")
+    assert r <= 1e-3
+
+
+def test_radiance_c2_typed_materials(ct, orc, gpu_ctx, c2):
+    """Config 2 scene (GGX, mirror, dielectric, Lambert, sky) at reduced size."""
+    scene, arrays, osc = c2
+    _upload(gpu_ctx, scene)
+    opts, _, _ = scene.frame_opts()
+    w, h = 192, 108
+    opts.spp = 2
+    cam = scene.camera(w, h)
+    gpu_ctx.resize(w, h)
+    gpu_ctx.trace_frame(cam, opts)
+    acc, _ = gpu_ctx.read_accum()
+    ref, _ = osc.render(cam, opts, w, h, 0, 2, threads=16)
+    diff = np.abs(acc - ref)
+    rel = diff / np.maximum(np.abs(ref), 1e-30)
+    bad = int(np.sum((rel > 1e-4) & (diff > 1e-7)))
+    print(f"C2 typed: pixels over 1e-4 rel {bad} of {acc.size}")
+    assert bad == 0
+    ref64, _ = osc.render(cam, opts, w, h, 0, 2, precision=64, threads=16)
+    r = _relmse(acc[..., :3] / acc[..., 3:], ref64[..., :3] / ref64[..., 3:])
+    print(f"C2 rel-MSE vs FP64 oracle: {r:.3e}")
+    assert r <= 1e-3
+
+
+def test_batch_equals_single_views(ct, orc, gpu_ctx, c1):
+    scene, arrays, osc = c1
+    _upload(gpu_ctx, scene)
+    opts, _, _ = scene.frame_opts()
+    opts.max_depth = 8
+    views = []
+    rng = np.random.default_rng(3)
+    for i, (w, h, spp) in enumerate([(96, 72, 2), (64, 64, 1), (160, 120, 4), (33, 17, 3)]):
+        ang = rng.uniform(0, 2 * np.pi)
+        pos = [30 * np.cos(ang), rng.uniform(-20, 20), 30 * np.sin(ang)]
+        cam = ct.look_at(pos, [0, 0, 0], 70.0)
+        views.append((cam, w, h, spp, 5 * i))
+    outs, st = gpu_ctx.render_batch(views, opts)
+    for (cam, w, h, spp, fb), out in zip(views, outs):
+        gpu_ctx.resize(w, h)
+        o2 = opts.copy()
+        o2.spp = 1
+        acc = np.zeros((h, w, 4), np.float32)
+        # single-view reference: ct_trace_frame with frame indices fb .. fb+spp-1
+        gpu_ctx.write_accum(acc, fb)
+        o2.spp = spp
+        gpu_ctx.trace_frame(cam, o2)
+        single, _ = gpu_ctx.read_accum()
+        assert np.array_equal(out, single), f"view {w}x{h} spp {spp} differs"
+    print("batch stats", st.primary_rays, st.extension_rays, st.shadow_rays, st.chunks)
+
+
+def test_batch_chunked_graph_replay(ct, orc, gpu_ctx, c1):
+    """Many chunks through the captured graph equal one big chunk."""
+    scene, arrays, osc = c1
+    _upload(gpu_ctx, scene)
+    opts, _, _ = scene.frame_opts()
+    cam = scene.camera(128, 128)
+    views = [(cam, 128, 128, 8, 0)]
+    outs_small = None
+    small = ct.Context(0, chunk_paths=1 << 14)
+    try:
+        small.upload(scene)
+        outs_small, st = small.render_batch(views, opts)
+        assert st.chunks == 8
+    finally:
+        small.close()
+    outs_big, _ = gpu_ctx.render_batch(views, opts)
+    assert np.array_equal(outs_small[0], outs_big[0])
+    ref, _ = osc.render(cam, opts, 128, 128, 0, 8, threads=16)
+    assert np.array_equal(outs_big[0], ref)
