@@ -13,7 +13,7 @@ untimed frame.  achieved = those bytes / the extend launches' device time
 frames).
 
 cpu_baseline: the CPU oracle (FP32 restatement of the reference loop, OpenMP
-over pixels) on a horizontal strip of the same frame.
+over pixels) on whole frames of the same workload, about 10 s of CPU work.
 """
 from __future__ import annotations
 
@@ -38,10 +38,17 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--seed", type=int, default=7)
     ap.add_argument("--traversal", default="exact", choices=["exact", "fast"])
-    ap.add_argument("--cpu-rows", type=int, default=24, help="rows of the frame the CPU baseline traces")
+    ap.add_argument("--cpu-seconds", type=float, default=10.0, help="minimum CPU-baseline work")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="only run a plain device copy of MIB MiB (calibrates FETCH_SIZE / WRITE_SIZE)")
+    ap.add_argument("--summarize", nargs="+", metavar="DB", help="reduce rocprofv3 result databases and exit")
     args = ap.parse_args()
+    if args.summarize:
+        return summarize(args.summarize)
+    if args.copy_probe:
+        return copy_probe(args.copy_probe)
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU form "
               "(image tiles or requests per GPU) is not built.", file=sys.stderr)
@@ -140,25 +147,71 @@ def main() -> int:
     }
 
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(scene, cam, opts, W, H, args.cpu_rows)
+        out["cpu_baseline"] = cpu_baseline(scene, cam, opts, W, H, args.cpu_seconds)
     ctx.close()
     print(json.dumps(out))
     return 0
 
 
-def cpu_baseline(scene, cam, opts, W, H, rows) -> dict:
-    """Oracle FP32 on rows [r0, r0+rows) of the benchmark frame."""
+def summarize(paths: list) -> int:
+    """Reduce rocprofv3 result databases (rocpd SQLite) to per-kernel
+    duration statistics and per-dispatch counter averages."""
+    import sqlite3
+    for p in paths:
+        db = sqlite3.connect(p)
+        print(f"# {p}")
+        rows = db.execute("select name, count(*), sum(end-start), avg(end-start), min(end-start), max(end-start) "
+                          "from kernels group by name order by sum(end-start) desc").fetchall()
+        tot = sum(r[2] for r in rows) or 1
+        print("| kernel | calls | total ms | avg us | min us | max us | % |")
+        print("|---|---|---|---|---|---|---|")
+        for r in rows:
+            print(f"| {r[0].split('(')[0]} | {r[1]} | {r[2] / 1e6:.3f} | {r[3] / 1e3:.1f} | {r[4] / 1e3:.1f} | "
+                  f"{r[5] / 1e3:.1f} | {100 * r[2] / tot:.1f} |")
+        pmc = db.execute("select k.name, p.counter_name, avg(p.counter_value), count(*) from pmc_events p "
+                         "join kernels k on k.dispatch_id = p.dispatch_id group by k.name, p.counter_name "
+                         "order by k.name").fetchall()
+        if pmc:
+            print("| kernel | counter | mean per dispatch | dispatches |")
+            print("|---|---|---|---|")
+            for r in pmc:
+                print(f"| {r[0].split('(')[0]} | {r[1]} | {r[2]:.1f} | {r[3]} |")
+    return 0
+
+
+def copy_probe(mib: int) -> int:
+    """A plain copy kernel of known size: reads and writes mib MiB, 5 times."""
+    import torch
+    n = mib * (1 << 20) // 4
+    x = torch.ones(n, dtype=torch.float32, device="cuda")
+    y = torch.empty_like(x)
+    for _ in range(5):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes": n * 4, "dispatches": 5}))
+    return 0
+
+
+def cpu_baseline(scene, cam, opts, W, H, min_seconds: float, max_frames: int = 64) -> dict:
+    """Oracle FP32 on whole benchmark frames (frame indices 0, 1, ...) until
+    at least min_seconds of CPU work, at most max_frames frames."""
+    import numpy as np
     from oracle import oracle as orc
     threads = min(16, os.cpu_count() or 1)
     osc = orc.Scene(scene.arrays())
-    r0 = H // 2 - rows // 2
+    acc = np.zeros((H, W, 4), np.float32)
+    rays = frames = 0
     t0 = time.perf_counter()
-    _, st = osc.render(cam, opts, W, H, 0, 1, rows=(r0, r0 + rows), threads=threads)
+    while frames < max_frames:
+        _, st = osc.render(cam, opts, W, H, frames, 1, threads=threads, accum=acc)
+        rays += st["primary_rays"] + st["extension_rays"] + st["shadow_rays"]
+        frames += 1
+        if time.perf_counter() - t0 >= min_seconds:
+            break
     dt = time.perf_counter() - t0
-    rays = st["primary_rays"] + st["extension_rays"] + st["shadow_rays"]
     return {"value": round(rays / dt / 1e6, 3), "unit": "Mrays/s", "cores": threads, "kind": "port",
-            "sample": f"rows {r0}..{r0 + rows - 1} of the 1920x1080 frame ({W * rows} paths, {rays} rays, "
-                      f"{dt:.1f} s on {threads} threads)"}
+            "sample": f"{frames} full 1920x1080 frames at 1 spp (frame indices 0..{frames - 1}), {rays} rays, "
+                      f"{dt:.1f} s on {threads} OpenMP threads; oracle built -O3 -march=x86-64-v3 without fast-math"}
 
 
 if __name__ == "__main__":

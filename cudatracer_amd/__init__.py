@@ -32,6 +32,7 @@ MAT_LIGHT, MAT_SPECULAR, MAT_FRESNEL, MAT_LAMBERTIAN, MAT_BECKMANN, MAT_GGX = 0,
 ACCEL_BVH2, ACCEL_BVH4 = 0, 1
 BSDF_REFERENCE, BSDF_TYPED = 0, 1
 TRAV_EXACT, TRAV_FAST = 0, 1
+LIGHTPDF_REFERENCE, LIGHTPDF_UNIFORM = 0, 1
 SCENE_C1, SCENE_C2, SCENE_C3 = 1, 2, 3
 
 
@@ -79,7 +80,8 @@ class FrameOpts(C.Structure):
     _fields_ = [("max_depth", C.c_int32), ("spp", C.c_int32), ("rr_start", C.c_int32), ("accel", C.c_int32),
                 ("bsdf_mode", C.c_int32), ("traversal", C.c_int32), ("shadow", C.c_int32),
                 ("indirect", C.c_int32), ("normal_epsilon", C.c_float), ("dist_epsilon", C.c_float),
-                ("triangle_epsilon", C.c_float), ("sort_by_material", C.c_int32)]
+                ("triangle_epsilon", C.c_float), ("sort_by_material", C.c_int32),
+                ("light_pdf", C.c_int32)]
 
     def copy(self) -> "FrameOpts":
         o = FrameOpts()
@@ -232,7 +234,8 @@ class HostScene:
         s.build()
         return s
 
-    def add_material(self, albedo, mtype=MAT_LAMBERTIAN, roughness=0.0, ior=1.0, keep_roughness=True) -> int:
+    def add_material(self, albedo, mtype=MAT_LAMBERTIAN, roughness=0.0, ior=1.0, keep_roughness=False) -> int:
+        """TriangleList::addMaterial: roughness is forced to 1 unless keep_roughness."""
         m = Material()
         m.albedo[:] = list(map(float, albedo))
         m.refract_idx = ior

@@ -99,7 +99,12 @@ struct TraceOpts {
 struct ShadeOpts {
     int max_depth, rr_start, bsdf_mode, shadow, indirect;
     float neps, deps, teps;
+    int light_pdf;
 };
+
+// Light-selection factor of the MIS light pdf (x count like Kernel.cu:128,184,
+// or / count for uniform selection).
+CT_DEV float light_pdf_scale(float pdf, uint32_t n, int mode) { return mode ? pdf / (float)n : pdf * (float)n; }
 
 // Queue buffers of one context (structure of arrays).
 struct Queues {

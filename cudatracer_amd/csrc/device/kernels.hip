@@ -244,7 +244,7 @@ __global__ void k_shade(DevScene sc, ShadeOpts so, const ViewDesc* __restrict__ 
                         if (lastType == LT_SPECULAR) {
                             c = thr * Le;
                         } else {
-                            const float lightPdf = t * t / (DdotNL * tri_area(p0, p1, p2)) * (float)sc.n_lights;
+                            const float lightPdf = light_pdf_scale(t * t / (DdotNL * tri_area(p0, p1, p2)), sc.n_lights, so.light_pdf);
                             c = thr * Le * (1.0f / (lastPdf + lightPdf));
                         }
                         add = vmax0(c);
@@ -354,7 +354,7 @@ __global__ void k_shade(DevScene sc, ShadeOpts so, const ViewDesc* __restrict__ 
                                     mf_eval(mtype, alpha, color, wol, wil, &fe, &shadowPdf);
                                 }
                                 if (shadowPdf > 0.0f) {
-                                    const float lightPdf = sqDist / (LNdotL * ar) * (float)sc.n_lights;
+                                    const float lightPdf = light_pdf_scale(sqDist / (LNdotL * ar), sc.n_lights, so.light_pdf);
                                     s_c = thr * emission * fe * (NdotL / (shadowPdf + lightPdf));
                                     s_o = org + so.neps * Lv;
                                     s_d = Lv;

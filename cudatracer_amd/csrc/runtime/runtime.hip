@@ -83,6 +83,8 @@ int validate_opts(ct_ctx* c, const ct_frame_opts& o) {
         return fail(c, CT_ERR_INVALID_ARG, "bad bsdf_mode");
     if (o.traversal != CT_TRAV_EXACT && o.traversal != CT_TRAV_FAST)
         return fail(c, CT_ERR_INVALID_ARG, "bad traversal");
+    if (o.light_pdf != CT_LIGHTPDF_REFERENCE && o.light_pdf != CT_LIGHTPDF_UNIFORM)
+        return fail(c, CT_ERR_INVALID_ARG, "bad light_pdf");
     if (!(o.triangle_epsilon >= 0.0f) || !(o.normal_epsilon >= 0.0f) || !(o.dist_epsilon >= 0.0f))
         return fail(c, CT_ERR_INVALID_ARG, "epsilons must be finite and >= 0");
     if (!c->has_scene) return fail(c, CT_ERR_NO_SCENE, "no scene uploaded");
@@ -224,6 +226,7 @@ static ctd::ShadeOpts shade_opts(const ct_frame_opts& o) {
     s.neps = o.normal_epsilon;
     s.deps = o.dist_epsilon;
     s.teps = o.triangle_epsilon;
+    s.light_pdf = o.light_pdf;
     return s;
 }
 
@@ -274,7 +277,7 @@ static std::string graph_key(const ct_ctx* c, const float4* accum, const ct_fram
     std::ostringstream os;
     os << (const void*)accum << ':' << c->cap << ':' << o.max_depth << ':' << o.rr_start << ':' << o.accel << ':'
        << o.bsdf_mode << ':' << o.traversal << ':' << o.shadow << ':' << o.indirect << ':' << o.normal_epsilon << ':'
-       << o.dist_epsilon << ':' << o.triangle_epsilon << ':' << c->shape.block << ':' << c->shape.lds_depth << ':'
+       << o.dist_epsilon << ':' << o.triangle_epsilon << ':' << o.light_pdf << ':' << c->shape.block << ':' << c->shape.lds_depth << ':'
        << c->shape.trace_blocks << ':' << c->shape.shade_blocks << ':' << (const void*)c->d_views << ':'
        << (const void*)c->d_segs << ':' << (const void*)c->sc.tris << ':' << (const void*)c->sc.mats;
     return os.str();
@@ -422,6 +425,7 @@ void ct_frame_opts_default(ct_frame_opts* o) {
     o->dist_epsilon = 1e-5f;
     o->triangle_epsilon = 1e-3f;
     o->sort_by_material = 0;
+    o->light_pdf = CT_LIGHTPDF_REFERENCE;
 }
 
 const char* ct_last_error(const ct_ctx* c) { return c ? c->err.c_str() : "null context"; }

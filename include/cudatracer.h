@@ -136,6 +136,12 @@ enum { CT_BSDF_REFERENCE = 0, CT_BSDF_TYPED = 1 };
 /* CT_TRAV_EXACT: reference node order and culling, bit-exact hit ids.
  * CT_TRAV_FAST: compressed 64-byte nodes; ids equal except exact t-ties. */
 enum { CT_TRAV_EXACT = 0, CT_TRAV_FAST = 1 };
+/* Light-selection pdf in the MIS weights.  CT_LIGHTPDF_REFERENCE multiplies
+ * the per-triangle solid-angle pdf by the light count as the reference does
+ * (Kernel.cu:128,184), which biases direct light by 1/count^2 when a scene
+ * has more than one light triangle; CT_LIGHTPDF_UNIFORM divides by it (the
+ * pdf of uniform light selection, unbiased).  Both agree for one light. */
+enum { CT_LIGHTPDF_REFERENCE = 0, CT_LIGHTPDF_UNIFORM = 1 };
 
 typedef struct ct_frame_opts {
     int32_t max_depth;        /* path segments (reference: 5, Kernel.cu:320)  */
@@ -150,6 +156,7 @@ typedef struct ct_frame_opts {
     float dist_epsilon;
     float triangle_epsilon;
     int32_t sort_by_material; /* material-key sort before shading             */
+    int32_t light_pdf;        /* CT_LIGHTPDF_*                                */
 } ct_frame_opts;
 
 typedef struct ct_frame_stats {
@@ -162,7 +169,8 @@ typedef struct ct_frame_stats {
 } ct_frame_stats;
 
 /* Fill with the reference defaults (depth 5, 1 spp, no RR, BVH4, reference BSDF,
- * exact traversal, NEE and indirect on, epsilons 1e-5/1e-5/1e-3, no sort). */
+ * exact traversal, NEE and indirect on, epsilons 1e-5/1e-5/1e-3, no sort,
+ * reference light pdf). */
 void ct_frame_opts_default(ct_frame_opts* opts);
 
 /* ------------------------------------------------------------ context */

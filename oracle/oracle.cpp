@@ -725,6 +725,12 @@ template <class T> T fresnel_dielectric(T cosThetaI, T eta) {
 // --------------------------------------------------------------- shading --
 enum { SPECULAR = 0, DIFFUSE = 1, GLOSSY = 2 };
 
+// Kernel.cu:128,184 multiply the light pdf by lightCount; uniform light
+// selection divides by it (CT_LIGHTPDF_UNIFORM).
+template <class T> T light_scale(T pdf, uint32_t n, int mode) {
+    return mode == CT_LIGHTPDF_UNIFORM ? pdf / T((float)n) : pdf * T((float)n);
+}
+
 template <class T> struct Camera3 {
     V3<T> pos, w, hor, ver;
     T invw, invh;
@@ -813,7 +819,7 @@ V3<T> path_sample(const or_scene& s, const Camera3<T>& cb, const ct_frame_opts& 
             if (lastType == SPECULAR) {
                 c = thr * Le;
             } else {
-                const T lightPdf = t * t / (DdotNL * tri_area(p0, p1, p2)) * T((float)lightCount);
+                const T lightPdf = light_scale(t * t / (DdotNL * tri_area(p0, p1, p2)), lightCount, o.light_pdf);
                 c = thr * Le * (T(1) / (lastPdf + lightPdf));
             }
             L = L + vmax0(c);
@@ -926,7 +932,7 @@ V3<T> path_sample(const or_scene& s, const Camera3<T>& cb, const ct_frame_opts& 
                     mf_eval(mtype, alpha, color, wol, wil, &fe, &shadowPdf);
                 }
                 if (shadowPdf > T(0)) {
-                    const T lightPdf = sqDist / (LNdotL * ar) * T((float)lightCount);
+                    const T lightPdf = light_scale(sqDist / (LNdotL * ar), lightCount, o.light_pdf);
                     const V3<T> col = thr * emission * fe * (NdotL / (shadowPdf + lightPdf));
                     const V3<T> so = org + neps * Lv;
                     T tmax = distance - T(2) * deps;

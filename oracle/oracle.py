@@ -29,7 +29,8 @@ class FrameOpts(C.Structure):
     _fields_ = [("max_depth", C.c_int32), ("spp", C.c_int32), ("rr_start", C.c_int32), ("accel", C.c_int32),
                 ("bsdf_mode", C.c_int32), ("traversal", C.c_int32), ("shadow", C.c_int32),
                 ("indirect", C.c_int32), ("normal_epsilon", C.c_float), ("dist_epsilon", C.c_float),
-                ("triangle_epsilon", C.c_float), ("sort_by_material", C.c_int32)]
+                ("triangle_epsilon", C.c_float), ("sort_by_material", C.c_int32),
+                ("light_pdf", C.c_int32)]
 
 
 class RenderStats(C.Structure):
