@@ -44,6 +44,8 @@ def main() -> int:
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (calibrates FETCH_SIZE / WRITE_SIZE)")
     ap.add_argument("--summarize", nargs="+", metavar="DB", help="reduce rocprofv3 result databases and exit")
+    ap.add_argument("--block", type=int, default=0, help="threads per block of the traversal kernels (0 = default)")
+    ap.add_argument("--stack-lds", type=int, default=0, help="per-lane LDS stack entries (0 = default)")
     args = ap.parse_args()
     if args.summarize:
         return summarize(args.summarize)
@@ -62,7 +64,7 @@ def main() -> int:
     opts.traversal = ct.TRAV_FAST if args.traversal == "fast" else ct.TRAV_EXACT
     cam = scene.camera(W, H)
     st_scene = scene.stats()
-    ctx = ct.Context(0)
+    ctx = ct.Context(0, block_size=args.block, stack_lds=args.stack_lds)
     ctx.upload(scene)
     ctx.resize(W, H)
 

@@ -126,8 +126,18 @@ struct Queues {
 constexpr int kMaxBounce = 64;
 CT_DEV uint32_t* cnt_active(uint32_t* c, int b) { return c + b; }
 CT_DEV uint32_t* cnt_shadow(uint32_t* c, int b) { return c + (kMaxBounce + 1) + b; }
-CT_DEV uint32_t* cnt_fetch_ext(uint32_t* c, int b) { return c + 2 * (kMaxBounce + 1) + b; }
-CT_DEV uint32_t* cnt_fetch_sh(uint32_t* c, int b) { return c + 3 * (kMaxBounce + 1) + b; }
-constexpr int kCounterWords = 4 * (kMaxBounce + 1) + 64;  // + misc
+// Work-fetch counters, [queue][range][bounce]: each XCD range owns a
+// 384-byte slot so ranges never share a cache line (same-line atomics from
+// different XCDs serialise).
+constexpr int kXcds = 8;
+#ifndef CT_FETCH_RANGES
+#define CT_FETCH_RANGES 1
+#endif
+constexpr int kFetchRanges = CT_FETCH_RANGES;
+constexpr int kFetchSlot = 96;
+constexpr int kFetchBase = 192;
+CT_DEV uint32_t* cnt_fetch_ext(uint32_t* c, int b) { return c + kFetchBase + b; }
+CT_DEV uint32_t* cnt_fetch_sh(uint32_t* c, int b) { return c + kFetchBase + kFetchRanges * kFetchSlot + b; }
+constexpr int kCounterWords = kFetchBase + 2 * kFetchRanges * kFetchSlot + 64;
 
 }  // namespace ctd

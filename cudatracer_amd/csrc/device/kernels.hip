@@ -21,7 +21,6 @@
 namespace ctd {
 
 // ------------------------------------------------------------ wave tools --
-CT_DEV uint32_t lane_id() { return __lane_id(); }
 
 // Append for all lanes with pred set: one atomic per wave, offsets by a
 // prefix popcount of the ballot.  Must be reached by every lane of the wave.
@@ -104,37 +103,53 @@ __global__ void k_generate(const ViewDesc* __restrict__ views, const Segment* __
 }
 
 // --------------------------------------------------------------- extend --
-// Persistent closest-hit: each wave takes 64 rays per atomic from the fetch
-// counter until the bounce's queue is drained.
-__global__ void k_extend(DevScene sc, TraceOpts to, Queues q, int sel, int bounce, int lds_depth,
-                         unsigned long long* wstats) {
-    extern __shared__ uint32_t lds_stack[];
+// Persistent closest-hit over the bounce's queue (see trace_persistent).
+template <int ACCEL, bool COUNT>
+CT_DEV void extend_impl(const DevScene& sc, const TraceOpts& to, const Queues& q, int sel, int bounce,
+                        LaneStack& st, TravCount& tc) {
     const uint32_t n = *cnt_active(q.counters, bounce);
-    uint32_t* fetch = cnt_fetch_ext(q.counters, bounce);
-    const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
-    LaneStack st;
-    st.lds = lds_stack + threadIdx.x;
-    st.stride = blockDim.x;
-    st.depth = lds_depth;
-    st.spill = q.spill + (size_t)gtid * (size_t)(kMaxStack - lds_depth);
-    st.overflow = false;
     const float4* __restrict__ ro = q.ray_o[sel];
     const float4* __restrict__ rd = q.ray_d[sel];
-    TravCount tc{0u, 0u};
-    while (true) {
-        uint32_t base = 0;
-        if (lane_id() == 0) base = atomicAdd(fetch, 64u);
-        base = __shfl(base, 0);
-        if (base >= n) break;
-        const uint32_t i = base + lane_id();
-        if (i < n) {
+    float2* hit = q.hit;
+    trace_persistent<ACCEL, false, COUNT>(
+        sc, to.teps, n, cnt_fetch_ext(q.counters, bounce), st, tc, q.err,
+        [&](uint32_t i) {
             const float4 o4 = ro[i], d4 = rd[i];
-            float t = 1e34f;
-            int prim = -1;
-            trace_ray<false>(sc, to, xyz(o4), xyz(d4), t, prim, st, wstats ? &tc : nullptr);
-            if (st.overflow) { atomicOr(q.err, 1u); st.overflow = false; t = 1e34f; prim = -1; }
-            q.hit[i] = make_float2(t, __int_as_float(prim));
-        }
+            return RayIn{xyz(o4), xyz(d4), 1e34f};
+        },
+        [&](uint32_t i, float t, int prim) {
+            if (prim == -2) { t = 1e34f; prim = -1; }
+            hit[i] = make_float2(t, __int_as_float(prim));
+        });
+}
+
+// GFX9 buffer resource word 3 (raw 32-bit access).
+constexpr int kBufferWord3 = 0x00020000;
+
+CT_DEV LaneStack make_stack(const Queues& q, int lds_depth) {
+    LaneStack st;
+    st.base = threadIdx.x;
+    st.stride = blockDim.x;
+    st.depth = lds_depth;
+    const uint32_t per_thread = (uint32_t)(kMaxStack - lds_depth);
+    const uint32_t threads = gridDim.x * blockDim.x;
+    st.spill = __builtin_amdgcn_make_buffer_rsrc(q.spill, 0, (int)(threads * per_thread * 4u), kBufferWord3);
+    st.spill_off = (blockIdx.x * blockDim.x + threadIdx.x) * per_thread * 4u;
+    st.sp = 0;
+    st.overflow = false;
+    return st;
+}
+
+__global__ void k_extend(DevScene sc, TraceOpts to, Queues q, int sel, int bounce, int lds_depth,
+                         unsigned long long* wstats) {
+    LaneStack st = make_stack(q, lds_depth);
+    TravCount tc{0u, 0u};
+    if (wstats) {
+        if (to.accel == 0) extend_impl<2, true>(sc, to, q, sel, bounce, st, tc);
+        else extend_impl<4, true>(sc, to, q, sel, bounce, st, tc);
+    } else {
+        if (to.accel == 0) extend_impl<2, false>(sc, to, q, sel, bounce, st, tc);
+        else extend_impl<4, false>(sc, to, q, sel, bounce, st, tc);
     }
     if (wstats) {
         atomicAdd(&wstats[0], (unsigned long long)tc.nodes);
@@ -143,41 +158,40 @@ __global__ void k_extend(DevScene sc, TraceOpts to, Queues q, int sel, int bounc
 }
 
 // -------------------------------------------------------------- connect --
+// Persistent any-hit over the bounce's shadow queue; unoccluded rays add
+// their colour to the path's radiance (Kernel.cu:247-256).
+template <int ACCEL, bool COUNT>
+CT_DEV void connect_impl(const DevScene& sc, const TraceOpts& to, const Queues& q, int bounce, LaneStack& st,
+                         TravCount& tc) {
+    const uint32_t n = *cnt_shadow(q.counters, bounce);
+    trace_persistent<ACCEL, true, COUNT>(
+        sc, to.teps, n, cnt_fetch_sh(q.counters, bounce), st, tc, q.err,
+        [&](uint32_t i) {
+            const float4 o4 = q.sh_o[i], d4 = q.sh_d[i];
+            return RayIn{xyz(o4), xyz(d4), o4.w};
+        },
+        [&](uint32_t i, float, int prim) {
+            if (prim != -1) return;  // occluded (or stack overflow, flagged)
+            const uint32_t slot = __float_as_uint(q.sh_d[i].w);
+            const float4 c = q.sh_c[i];
+            float4 L = q.lpath[slot];
+            L.x = L.x + c.x;
+            L.y = L.y + c.y;
+            L.z = L.z + c.z;
+            q.lpath[slot] = L;
+        });
+}
+
 __global__ void k_connect(DevScene sc, TraceOpts to, Queues q, int bounce, int lds_depth,
                           unsigned long long* wstats) {
-    extern __shared__ uint32_t lds_stack[];
-    const uint32_t n = *cnt_shadow(q.counters, bounce);
-    uint32_t* fetch = cnt_fetch_sh(q.counters, bounce);
-    const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
-    LaneStack st;
-    st.lds = lds_stack + threadIdx.x;
-    st.stride = blockDim.x;
-    st.depth = lds_depth;
-    st.spill = q.spill + (size_t)gtid * (size_t)(kMaxStack - lds_depth);
-    st.overflow = false;
+    LaneStack st = make_stack(q, lds_depth);
     TravCount tc{0u, 0u};
-    while (true) {
-        uint32_t base = 0;
-        if (lane_id() == 0) base = atomicAdd(fetch, 64u);
-        base = __shfl(base, 0);
-        if (base >= n) break;
-        const uint32_t i = base + lane_id();
-        if (i < n) {
-            const float4 o4 = q.sh_o[i], d4 = q.sh_d[i];
-            float tmax = o4.w;
-            int dummy = -1;
-            const bool occ = trace_ray<true>(sc, to, xyz(o4), xyz(d4), tmax, dummy, st, wstats ? &tc : nullptr);
-            if (st.overflow) { atomicOr(q.err, 1u); st.overflow = false; }
-            else if (!occ) {
-                const uint32_t slot = __float_as_uint(d4.w);
-                const float4 c = q.sh_c[i];
-                float4 L = q.lpath[slot];
-                L.x = L.x + c.x;
-                L.y = L.y + c.y;
-                L.z = L.z + c.z;
-                q.lpath[slot] = L;
-            }
-        }
+    if (wstats) {
+        if (to.accel == 0) connect_impl<2, true>(sc, to, q, bounce, st, tc);
+        else connect_impl<4, true>(sc, to, q, bounce, st, tc);
+    } else {
+        if (to.accel == 0) connect_impl<2, false>(sc, to, q, bounce, st, tc);
+        else connect_impl<4, false>(sc, to, q, bounce, st, tc);
     }
     if (wstats) {
         atomicAdd(&wstats[2], (unsigned long long)tc.nodes);
@@ -456,31 +470,23 @@ __global__ void k_tonemap(const float4* __restrict__ accum, uint32_t n, float sa
 }
 
 // Ray-level any-hit: occluded flag per ray (rays in sh_o / sh_d layout).
-__global__ void k_shadow_flags(DevScene sc, TraceOpts to, Queues q, uint32_t n, int lds_depth, uint8_t* out) {
-    extern __shared__ uint32_t lds_stack[];
-    uint32_t* fetch = cnt_fetch_sh(q.counters, 0);
-    const uint32_t gtid = blockIdx.x * blockDim.x + threadIdx.x;
-    LaneStack st;
-    st.lds = lds_stack + threadIdx.x;
-    st.stride = blockDim.x;
-    st.depth = lds_depth;
-    st.spill = q.spill + (size_t)gtid * (size_t)(kMaxStack - lds_depth);
-    st.overflow = false;
-    while (true) {
-        uint32_t base = 0;
-        if (lane_id() == 0) base = atomicAdd(fetch, 64u);
-        base = __shfl(base, 0);
-        if (base >= n) break;
-        const uint32_t i = base + lane_id();
-        if (i < n) {
+template <int ACCEL>
+CT_DEV void flags_impl(const DevScene& sc, const TraceOpts& to, const Queues& q, uint32_t n, LaneStack& st,
+                       uint8_t* out) {
+    TravCount tc{0u, 0u};
+    trace_persistent<ACCEL, true, false>(
+        sc, to.teps, n, cnt_fetch_sh(q.counters, 0), st, tc, q.err,
+        [&](uint32_t i) {
             const float4 o4 = q.sh_o[i], d4 = q.sh_d[i];
-            float tmax = o4.w;
-            int dummy = -1;
-            const bool occ = trace_ray<true>(sc, to, xyz(o4), xyz(d4), tmax, dummy, st, nullptr);
-            if (st.overflow) { atomicOr(q.err, 1u); st.overflow = false; }
-            out[i] = occ ? 1 : 0;
-        }
-    }
+            return RayIn{xyz(o4), xyz(d4), o4.w};
+        },
+        [&](uint32_t i, float, int prim) { out[i] = prim >= 0 ? 1 : 0; });
+}
+
+__global__ void k_shadow_flags(DevScene sc, TraceOpts to, Queues q, uint32_t n, int lds_depth, uint8_t* out) {
+    LaneStack st = make_stack(q, lds_depth);
+    if (to.accel == 0) flags_impl<2>(sc, to, q, n, st, out);
+    else flags_impl<4>(sc, to, q, n, st, out);
 }
 
 }  // namespace ctd

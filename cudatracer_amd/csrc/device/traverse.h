@@ -8,37 +8,29 @@
 // Empty quad slots (count 0, always "hit" in the reference) are not pushed:
 // popping them is a no-op, so the visit order of real nodes is unchanged.
 //
-// The traversal stack is per lane: the first `lds_depth` entries live in LDS
-// laid out [depth][thread] (consecutive lanes hit consecutive banks), deeper
-// entries spill to a per-thread global area.  The builder's depth limit (64)
-// bounds the stack at 97 entries; kMaxStack guards user-supplied trees.
+// Execution shape (what changes is only how a wave64 interleaves its lanes;
+// every lane performs the reference's sequence of pops, node tests, pushes
+// and triangle tests for its own ray):
+//   - persistent waves fetch 64 rays per atomic once all their lanes are
+//     idle, from the queue range of their XCD first (see Fetcher); mixing
+//     fresh rays into a busy wave measured slower (lost coherence);
+//   - one step per loop iteration and lane: either pop + node test, or one
+//     triangle of the current leaf, so long leaves do not stall node lanes;
+//   - the per-lane stack lives in LDS, laid out [depth][thread]; pushes are
+//     branch-free stores at the stack top (a store for a culled child is
+//     overwritten by the next one); entries beyond the LDS depth spill to a
+//     per-thread global area.  The builder's depth limit (64) bounds the
+//     stack at 97 entries; kMaxStack guards user-supplied trees.
 #pragma once
 
 #include "dev_common.h"
 
+extern __shared__ uint32_t ct_lds_stack[];
+
 namespace ctd {
 
-struct LaneStack {
-    uint32_t* lds;    // &lds_base[threadIdx.x]; entry k at lds[k * stride]
-    uint32_t* spill;  // this thread's overflow area (kMaxStack - lds_depth entries)
-    int stride;
-    int depth;
-    int sp;
-    bool overflow;
-
-    CT_DEV void push(uint32_t v) {
-        if (sp < depth) lds[sp * stride] = v;
-        else if (sp < kMaxStack) spill[sp - depth] = v;
-        else { overflow = true; return; }
-        ++sp;
-    }
-    CT_DEV uint32_t pop() {
-        --sp;
-        return sp < depth ? lds[sp * stride] : spill[sp - depth];
-    }
-};
-
 CT_DEV uint32_t fbits(float f) { return __float_as_uint(f); }
+CT_DEV uint32_t lane_id() { return __lane_id(); }
 
 // Optional per-ray work counters (node tests, triangle tests).
 struct TravCount {
@@ -66,87 +58,108 @@ CT_DEV bool tri_test(f3 o, f3 d, float& t, f3 p0, f3 edge1, f3 edge2, float eps)
     return false;
 }
 
-// Walk one leaf run; returns true on the first hit for any-hit queries.
-template <bool AnyHit>
-CT_DEV bool leaf_run(const float4* __restrict__ tris, uint32_t k, f3 o, f3 d, float& t, int& prim, float eps,
-                     TravCount* cnt) {
-    while (true) {
-        if (cnt) ++cnt->tris;
-        const float4 a = tris[3 * (size_t)k];
-        const float4 b = tris[3 * (size_t)k + 1];
-        const float4 c = tris[3 * (size_t)k + 2];
-        const uint32_t w = fbits(a.w);
-        if (tri_test(o, d, t, xyz(a), xyz(b), xyz(c), eps)) {
-            if (AnyHit) return true;
-            prim = (int)(w & ~kLastInLeaf);
-        }
-        if (w & kLastInLeaf) break;
-        ++k;
-    }
-    return false;
-}
-
 CT_DEV float tag(float v, uint32_t slot) { return __uint_as_float((fbits(v) & 0xFFFFFFFCu) | slot); }
 
-// Exact quad-tree traversal (MBVHNode.cuh:126-205).
-template <bool AnyHit>
-CT_DEV bool trace_bvh4_exact(const DevScene& sc, f3 o, f3 d, float& t, int& prim, float eps, LaneStack& st,
-                             TravCount* cnt) {
-    const f3 inv = mk3(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
-    st.sp = 0;
-    st.push(0u);
-    while (st.sp > 0) {
-        const uint32_t e = st.pop();
-        if (e & kLeafBit) {
-            if (leaf_run<AnyHit>(sc.tris, e & ~kLeafBit, o, d, t, prim, eps, cnt)) return true;
-            continue;
-        }
-        const float4* n = sc.nodes4 + 8 * (size_t)e;
-        const float4 bminx = n[0], bmaxx = n[1], bminy = n[2], bmaxy = n[3], bminz = n[4], bmaxz = n[5];
-        const float4 ch = n[6], cn = n[7];
-        if (cnt) ++cnt->nodes;
-        float tmin[4], tmax[4];
-        const float mnx[4] = {bminx.x, bminx.y, bminx.z, bminx.w}, mxx[4] = {bmaxx.x, bmaxx.y, bmaxx.z, bmaxx.w};
-        const float mny[4] = {bminy.x, bminy.y, bminy.z, bminy.w}, mxy[4] = {bmaxy.x, bmaxy.y, bmaxy.z, bmaxy.w};
-        const float mnz[4] = {bminz.x, bminz.y, bminz.z, bminz.w}, mxz[4] = {bmaxz.x, bmaxz.y, bmaxz.z, bmaxz.w};
-#pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            float t1 = (mnx[i] - o.x) * inv.x, t2 = (mxx[i] - o.x) * inv.x;
-            float mn = gmin(t1, t2), mx = gmax(t1, t2);
-            t1 = (mny[i] - o.y) * inv.y;
-            t2 = (mxy[i] - o.y) * inv.y;
-            mn = gmax(mn, gmin(t1, t2));
-            mx = gmin(mx, gmax(t1, t2));
-            t1 = (mnz[i] - o.z) * inv.z;
-            t2 = (mxz[i] - o.z) * inv.z;
-            mn = gmax(mn, gmin(t1, t2));
-            mx = gmin(mx, gmax(t1, t2));
-            tmin[i] = tag(mn, (uint32_t)i);
-            tmax[i] = mx;
-        }
-        bool res[4];
-#pragma unroll
-        for (int i = 0; i < 4; ++i) res[i] = tmax[i] > 0.0f && tmin[i] <= tmax[i] && tmin[i] < t;
-        float s0 = tmin[0], s1 = tmin[1], s2 = tmin[2], s3 = tmin[3], tmp;
-        if (s0 > s1) { tmp = s0; s0 = s1; s1 = tmp; }
-        if (s2 > s3) { tmp = s2; s2 = s3; s3 = tmp; }
-        if (s0 > s2) { tmp = s0; s0 = s2; s2 = tmp; }
-        if (s1 > s3) { tmp = s1; s1 = s3; s3 = tmp; }
-        if (s2 > s3) { tmp = s2; s2 = s3; s3 = tmp; }
-        const float sorted[4] = {s0, s1, s2, s3};
-        const uint32_t chi[4] = {fbits(ch.x), fbits(ch.y), fbits(ch.z), fbits(ch.w)};
-        const int cntv[4] = {(int)fbits(cn.x), (int)fbits(cn.y), (int)fbits(cn.z), (int)fbits(cn.w)};
-#pragma unroll
-        for (int i = 3; i >= 0; --i) {
-            const uint32_t idx = fbits(sorted[i]) & 3u;
-            const bool r = idx == 0 ? res[0] : (idx == 1 ? res[1] : (idx == 2 ? res[2] : res[3]));
-            const int c = idx == 0 ? cntv[0] : (idx == 1 ? cntv[1] : (idx == 2 ? cntv[2] : cntv[3]));
-            const uint32_t child = idx == 0 ? chi[0] : (idx == 1 ? chi[1] : (idx == 2 ? chi[2] : chi[3]));
-            if (r && c != 0) st.push(c == -1 ? child : (kLeafBit | child));
-        }
-        if (st.overflow) return false;
+// Per-lane stack: LDS entries [0, depth), global spill [depth, kMaxStack).
+// The spill area is addressed through a raw buffer resource (vector buffer
+// loads/stores, bounds-checked by the hardware), which also keeps the
+// compiler from folding LDS and spill reads into one flat access.
+struct LaneStack {
+    uint32_t base;      // threadIdx.x
+    uint32_t stride;    // blockDim.x
+    int depth;          // LDS entries per lane
+    __amdgpu_buffer_rsrc_t spill;  // whole spill area
+    uint32_t spill_off; // byte offset of this thread's spill slots
+    int sp;
+    bool overflow;
+
+    CT_DEV void put(int pos, uint32_t v) {
+        if (pos < depth) ct_lds_stack[(uint32_t)pos * stride + base] = v;
+        else if (pos < kMaxStack)
+            __builtin_amdgcn_raw_buffer_store_b32(v, spill, spill_off + 4u * (uint32_t)(pos - depth), 0, 0);
+        else overflow = true;
     }
-    return false;
+    CT_DEV uint32_t top_pop() {
+        --sp;
+        // LDS read at a clamped index (always in bounds), buffer read only for
+        // spilled entries
+        const int li = sp < depth ? sp : depth - 1;
+        uint32_t v = ct_lds_stack[(uint32_t)li * stride + base];
+        if (sp >= depth) v = __builtin_amdgcn_raw_buffer_load_b32(spill, spill_off + 4u * (uint32_t)(sp - depth), 0, 0);
+        return v;
+    }
+    // Push v0..v3 where k0..k3 (in that order); branch-free when all four
+    // candidate positions are in LDS.
+    CT_DEV void push4(uint32_t v0, bool k0, uint32_t v1, bool k1, uint32_t v2, bool k2, uint32_t v3, bool k3) {
+        if (sp + 4 <= depth) {
+            uint32_t p = (uint32_t)sp * stride + base;
+            ct_lds_stack[p] = v0;
+            p += k0 ? stride : 0u;
+            ct_lds_stack[p] = v1;
+            p += k1 ? stride : 0u;
+            ct_lds_stack[p] = v2;
+            p += k2 ? stride : 0u;
+            ct_lds_stack[p] = v3;
+            sp += (int)k0 + (int)k1 + (int)k2 + (int)k3;
+        } else {
+            if (k0) put(sp++, v0);
+            if (k1) put(sp++, v1);
+            if (k2) put(sp++, v2);
+            if (k3) put(sp++, v3);
+            if (sp > kMaxStack) sp = kMaxStack;
+        }
+    }
+};
+
+// Quad node test (MBVHNode::intersect): four slabs in the reference's float
+// order, tmin tagged with the slot, the 5-comparator network, then pushes in
+// the order of traverseMBVH (sorted position 3 first).
+CT_DEV void node4_step(const float4* __restrict__ nodes4, uint32_t e, f3 o, f3 inv, float t, LaneStack& st) {
+    const float4* n = nodes4 + 8 * (size_t)e;
+    const float4 bminx = n[0], bmaxx = n[1], bminy = n[2], bmaxy = n[3], bminz = n[4], bmaxz = n[5];
+    const float4 ch = n[6], cn = n[7];
+    float m0, m1, m2, m3, x0, x1, x2, x3;
+#define CT_SLAB(MN, MX, lo, hi)                                                 \
+    {                                                                           \
+        float t1 = (bminx.lo - o.x) * inv.x, t2 = (bmaxx.lo - o.x) * inv.x;     \
+        float mn = gmin(t1, t2), mx = gmax(t1, t2);                             \
+        t1 = (bminy.lo - o.y) * inv.y;                                          \
+        t2 = (bmaxy.lo - o.y) * inv.y;                                          \
+        mn = gmax(mn, gmin(t1, t2));                                            \
+        mx = gmin(mx, gmax(t1, t2));                                            \
+        t1 = (bminz.lo - o.z) * inv.z;                                          \
+        t2 = (bmaxz.lo - o.z) * inv.z;                                          \
+        mn = gmax(mn, gmin(t1, t2));                                            \
+        mx = gmin(mx, gmax(t1, t2));                                            \
+        MN = tag(mn, hi);                                                       \
+        MX = mx;                                                                \
+    }
+    CT_SLAB(m0, x0, x, 0u)
+    CT_SLAB(m1, x1, y, 1u)
+    CT_SLAB(m2, x2, z, 2u)
+    CT_SLAB(m3, x3, w, 3u)
+#undef CT_SLAB
+    const int c0 = (int)fbits(cn.x), c1 = (int)fbits(cn.y), c2 = (int)fbits(cn.z), c3 = (int)fbits(cn.w);
+    // hit mask on the tagged values; empty slots (count 0) never pushed
+    const uint32_t hm = ((x0 > 0.0f && m0 <= x0 && m0 < t && c0 != 0) ? 1u : 0u) |
+                        ((x1 > 0.0f && m1 <= x1 && m1 < t && c1 != 0) ? 2u : 0u) |
+                        ((x2 > 0.0f && m2 <= x2 && m2 < t && c2 != 0) ? 4u : 0u) |
+                        ((x3 > 0.0f && m3 <= x3 && m3 < t && c3 != 0) ? 8u : 0u);
+    if (hm == 0u) return;
+    float s0 = m0, s1 = m1, s2 = m2, s3 = m3, tmp;
+    if (s0 > s1) { tmp = s0; s0 = s1; s1 = tmp; }
+    if (s2 > s3) { tmp = s2; s2 = s3; s3 = tmp; }
+    if (s0 > s2) { tmp = s0; s0 = s2; s2 = tmp; }
+    if (s1 > s3) { tmp = s1; s1 = s3; s3 = tmp; }
+    if (s2 > s3) { tmp = s2; s2 = s3; s3 = tmp; }
+    const uint32_t e0 = c0 == -1 ? fbits(ch.x) : (kLeafBit | fbits(ch.x));
+    const uint32_t e1 = c1 == -1 ? fbits(ch.y) : (kLeafBit | fbits(ch.y));
+    const uint32_t e2 = c2 == -1 ? fbits(ch.z) : (kLeafBit | fbits(ch.z));
+    const uint32_t e3 = c3 == -1 ? fbits(ch.w) : (kLeafBit | fbits(ch.w));
+    auto entry = [&](uint32_t slot) { return slot == 0 ? e0 : (slot == 1 ? e1 : (slot == 2 ? e2 : e3)); };
+    const uint32_t q3 = fbits(s3) & 3u, q2 = fbits(s2) & 3u, q1 = fbits(s1) & 3u, q0 = fbits(s0) & 3u;
+    st.push4(entry(q3), (hm >> q3) & 1u, entry(q2), (hm >> q2) & 1u, entry(q1), (hm >> q1) & 1u, entry(q0),
+             (hm >> q0) & 1u);
 }
 
 // BVHNode::Intersect (BVHNode.cuh:50-62): no culling against the current t.
@@ -160,59 +173,157 @@ CT_DEV bool box2(float4 lo, float4 hi, f3 o, f3 inv, float& tmin) {
     return tmax >= 0.0f && tmin < tmax;
 }
 
-// Exact binary traversal (BVHNode.cuh:82-204): both children tested, the
-// strictly nearer pushed first (so popped last), leaves over
-// prim_indices[leftFirst, +count).
-template <bool AnyHit>
-CT_DEV bool trace_bvh2_exact(const DevScene& sc, f3 o, f3 d, float& t, int& prim, float eps, LaneStack& st,
-                             TravCount* cnt) {
-    const f3 inv = mk3(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
-    st.sp = 0;
-    st.push(0u);
-    while (st.sp > 0) {
-        const uint32_t ni = st.pop();
-        const float4 lo = sc.nodes2[2 * (size_t)ni];
-        const float4 hi = sc.nodes2[2 * (size_t)ni + 1];
-        const int count = (int)fbits(hi.w);
-        const uint32_t lf = fbits(lo.w);
-        if (count > -1) {
-            for (int i = 0; i < count; ++i) {
-                const uint32_t k = lf + (uint32_t)i;
-                if (cnt) ++cnt->tris;
-                const float4 a = sc.tris[3 * (size_t)k];
-                const float4 b = sc.tris[3 * (size_t)k + 1];
-                const float4 c = sc.tris[3 * (size_t)k + 2];
-                if (tri_test(o, d, t, xyz(a), xyz(b), xyz(c), eps)) {
-                    if (AnyHit) return true;
-                    prim = (int)(fbits(a.w) & ~kLastInLeaf);
-                }
-            }
-            continue;
-        }
-        const float4 l0 = sc.nodes2[2 * (size_t)lf], l1 = sc.nodes2[2 * (size_t)lf + 1];
-        const float4 r0 = sc.nodes2[2 * (size_t)(lf + 1)], r1 = sc.nodes2[2 * (size_t)(lf + 1) + 1];
-        if (cnt) cnt->nodes += 2;
-        float n1, n2;
-        const bool hl = box2(l0, l1, o, inv, n1);
-        const bool hr = box2(r0, r1, o, inv, n2);
-        if (hl && hr) {
-            if (n1 < n2) { st.push(lf); st.push(lf + 1); }
-            else { st.push(lf + 1); st.push(lf); }
-        } else if (hl) {
-            st.push(lf);
-        } else if (hr) {
-            st.push(lf + 1);
-        }
-        if (st.overflow) return false;
-    }
-    return false;
+// Binary interior step (traverseBVH, BVHNode.cuh:107-139): both children
+// tested; if both hit, the strictly nearer is pushed first.
+CT_DEV void node2_step(const float4* __restrict__ nodes2, uint32_t lf, f3 o, f3 inv, LaneStack& st) {
+    const float4 l0 = nodes2[2 * (size_t)lf], l1 = nodes2[2 * (size_t)lf + 1];
+    const float4 r0 = nodes2[2 * (size_t)(lf + 1)], r1 = nodes2[2 * (size_t)(lf + 1) + 1];
+    float n1, n2;
+    const bool hl = box2(l0, l1, o, inv, n1);
+    const bool hr = box2(r0, r1, o, inv, n2);
+    const bool left_first = !(hl && hr) || (n1 < n2);
+    const uint32_t a = left_first ? lf : lf + 1, b = left_first ? lf + 1 : lf;
+    const bool ka = left_first ? hl : hr, kb = left_first ? hr : hl;
+    st.push4(a, ka, b, kb, 0u, false, 0u, false);
 }
 
-template <bool AnyHit>
-CT_DEV bool trace_ray(const DevScene& sc, const TraceOpts& to, f3 o, f3 d, float& t, int& prim, LaneStack& st,
-                      TravCount* cnt) {
-    if (to.accel == 0) return trace_bvh2_exact<AnyHit>(sc, o, d, t, prim, to.teps, st, cnt);
-    return trace_bvh4_exact<AnyHit>(sc, o, d, t, prim, to.teps, st, cnt);
+// ----------------------------------------------------------------------
+// Persistent traversal driver.  Source: how ray i is loaded; Sink: what is
+// done with the result.  ACCEL 4 = quad tree, 2 = binary tree.
+struct RayIn {
+    f3 o, d;
+    float tmax;
+};
+
+constexpr uint32_t kNoLeaf = 0xFFFFFFFFu;
+
+#ifndef CT_FETCH_CHUNK
+#define CT_FETCH_CHUNK 64
+#endif
+#ifndef CT_FETCH_STEAL
+#define CT_FETCH_STEAL 1
+#endif
+
+// The queue [0, n) is cut into kFetchRanges contiguous ranges.  A wave claims
+// CT_FETCH_CHUNK rays at a time with one atomic on the counter of its range
+// (workgroups are dispatched to the XCDs round-robin by block id, so each
+// XCD's L2 serves spatially coherent rays) and hands them to its lanes 64 at
+// a time.  All state is wave-uniform.
+struct Fetcher {
+    uint32_t* base;
+    uint32_t n;
+    int range;
+    uint32_t next, end;
+
+    CT_DEV bool claim() {
+        for (int tries = 0; tries < kFetchRanges; ++tries) {
+            const uint32_t lo = (uint32_t)(((uint64_t)n * (uint32_t)range) / kFetchRanges);
+            const uint32_t hi = (uint32_t)(((uint64_t)n * (uint32_t)(range + 1)) / kFetchRanges);
+            uint32_t got = 0;
+            if (lane_id() == 0) got = atomicAdd(base + range * kFetchSlot, (uint32_t)CT_FETCH_CHUNK);
+            got = __shfl(got, 0);
+            if (got < hi - lo) {
+                next = lo + got;
+                end = lo + min(got + (uint32_t)CT_FETCH_CHUNK, hi - lo);
+                return true;
+            }
+            if (!CT_FETCH_STEAL) return false;
+            range = (range + 1) % kFetchRanges;
+        }
+        return false;
+    }
+    CT_DEV bool get(uint32_t& first, uint32_t& count) {
+        if (next >= end && !claim()) return false;
+        first = next;
+        count = min(64u, end - next);
+        next += count;
+        return true;
+    }
+};
+
+template <int ACCEL, bool AnyHit, bool COUNT, class Source, class Sink>
+CT_DEV void trace_persistent(const DevScene& sc, float eps, uint32_t n, uint32_t* fetch, LaneStack& st,
+                             TravCount& cnt, uint32_t* err, Source src, Sink sink) {
+    bool alive = false, exhausted = false;
+    uint32_t ray = 0, k = kNoLeaf;
+    f3 o = mk3(0, 0, 0), d = mk3(0, 0, 0), inv = mk3(0, 0, 0);
+    float t = 0.0f;
+    int prim = -1;
+    Fetcher fe{fetch, n, (int)(blockIdx.x % kFetchRanges), 0u, 0u};
+    while (true) {
+        // ---- refill the wave once every lane is idle (wave-uniform)
+        const uint64_t idle = __ballot(!alive);
+        if (!exhausted && idle == __ballot(true)) {
+            uint32_t first = 0, count = 0;
+            if (!fe.get(first, count)) exhausted = true;
+            if (!alive && !exhausted) {
+                const uint32_t idx = first + lane_id();
+                if (lane_id() < count) {
+                    const RayIn r = src(idx);
+                    ray = idx;
+                    o = r.o;
+                    d = r.d;
+                    t = r.tmax;
+                    prim = -1;
+                    inv = mk3(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
+                    st.sp = 0;
+                    st.overflow = false;
+                    st.push4(0u, true, 0u, false, 0u, false, 0u, false);
+                    k = kNoLeaf;
+                    alive = true;
+                }
+            }
+        }
+        if (__ballot(alive) == 0) {
+            if (exhausted) break;
+            continue;
+        }
+        if (!alive) continue;
+        // ---- one step: pop (+ node test) and/or one triangle
+        if (k == kNoLeaf) {
+            const uint32_t e = st.top_pop();
+            if (ACCEL == 4) {
+                if (e & kLeafBit) {
+                    k = e & ~kLeafBit;
+                } else {
+                    if (COUNT) ++cnt.nodes;
+                    node4_step(sc.nodes4, e, o, inv, t, st);
+                }
+            } else {
+                const float4 lo = sc.nodes2[2 * (size_t)e];
+                const float4 hi = sc.nodes2[2 * (size_t)e + 1];
+                const int count = (int)fbits(hi.w);
+                if (count > -1) {
+                    k = fbits(lo.w);  // leaf: triangles until the end-of-leaf flag
+                } else {
+                    if (COUNT) cnt.nodes += 2;
+                    node2_step(sc.nodes2, fbits(lo.w), o, inv, st);
+                }
+            }
+        }
+        bool done = false;
+        if (k != kNoLeaf) {
+            if (COUNT) ++cnt.tris;
+            const float4 a = sc.tris[3 * (size_t)k];
+            const float4 b = sc.tris[3 * (size_t)k + 1];
+            const float4 c = sc.tris[3 * (size_t)k + 2];
+            const uint32_t w = fbits(a.w);
+            if (tri_test(o, d, t, xyz(a), xyz(b), xyz(c), eps)) {
+                prim = (int)(w & ~kLastInLeaf);
+                if (AnyHit) done = true;
+            }
+            k = (w & kLastInLeaf) ? kNoLeaf : k + 1;
+        }
+        if (st.overflow) {
+            atomicOr(err, 1u);
+            done = true;
+            prim = -2;
+        }
+        if (done || (k == kNoLeaf && st.sp == 0)) {
+            sink(ray, t, prim);
+            alive = false;
+        }
+    }
 }
 
 }  // namespace ctd
