@@ -126,6 +126,10 @@ CT_DEV void extend_impl(const DevScene& sc, const TraceOpts& to, const Queues& q
 // GFX9 buffer resource word 3 (raw 32-bit access).
 constexpr int kBufferWord3 = 0x00020000;
 
+// Tree the traversal kernels walk: 2 binary, 4 quad (exact), 5 compressed
+// quad (fast mode; the binary tree has no compressed form).
+CT_DEV int tree_kind(const TraceOpts& to) { return to.accel == 0 ? 2 : (to.traversal == 1 ? 5 : 4); }
+
 CT_DEV LaneStack make_stack(const Queues& q, int lds_depth) {
     LaneStack st;
     st.base = threadIdx.x;
@@ -144,11 +148,14 @@ __global__ void k_extend(DevScene sc, TraceOpts to, Queues q, int sel, int bounc
                          unsigned long long* wstats) {
     LaneStack st = make_stack(q, lds_depth);
     TravCount tc{0u, 0u};
+    const int tree = tree_kind(to);
     if (wstats) {
-        if (to.accel == 0) extend_impl<2, true>(sc, to, q, sel, bounce, st, tc);
+        if (tree == 2) extend_impl<2, true>(sc, to, q, sel, bounce, st, tc);
+        else if (tree == 5) extend_impl<5, true>(sc, to, q, sel, bounce, st, tc);
         else extend_impl<4, true>(sc, to, q, sel, bounce, st, tc);
     } else {
-        if (to.accel == 0) extend_impl<2, false>(sc, to, q, sel, bounce, st, tc);
+        if (tree == 2) extend_impl<2, false>(sc, to, q, sel, bounce, st, tc);
+        else if (tree == 5) extend_impl<5, false>(sc, to, q, sel, bounce, st, tc);
         else extend_impl<4, false>(sc, to, q, sel, bounce, st, tc);
     }
     if (wstats) {
@@ -186,11 +193,14 @@ __global__ void k_connect(DevScene sc, TraceOpts to, Queues q, int bounce, int l
                           unsigned long long* wstats) {
     LaneStack st = make_stack(q, lds_depth);
     TravCount tc{0u, 0u};
+    const int tree = tree_kind(to);
     if (wstats) {
-        if (to.accel == 0) connect_impl<2, true>(sc, to, q, bounce, st, tc);
+        if (tree == 2) connect_impl<2, true>(sc, to, q, bounce, st, tc);
+        else if (tree == 5) connect_impl<5, true>(sc, to, q, bounce, st, tc);
         else connect_impl<4, true>(sc, to, q, bounce, st, tc);
     } else {
-        if (to.accel == 0) connect_impl<2, false>(sc, to, q, bounce, st, tc);
+        if (tree == 2) connect_impl<2, false>(sc, to, q, bounce, st, tc);
+        else if (tree == 5) connect_impl<5, false>(sc, to, q, bounce, st, tc);
         else connect_impl<4, false>(sc, to, q, bounce, st, tc);
     }
     if (wstats) {
@@ -485,7 +495,9 @@ CT_DEV void flags_impl(const DevScene& sc, const TraceOpts& to, const Queues& q,
 
 __global__ void k_shadow_flags(DevScene sc, TraceOpts to, Queues q, uint32_t n, int lds_depth, uint8_t* out) {
     LaneStack st = make_stack(q, lds_depth);
-    if (to.accel == 0) flags_impl<2>(sc, to, q, n, st, out);
+    const int tree = tree_kind(to);
+    if (tree == 2) flags_impl<2>(sc, to, q, n, st, out);
+    else if (tree == 5) flags_impl<5>(sc, to, q, n, st, out);
     else flags_impl<4>(sc, to, q, n, st, out);
 }
 

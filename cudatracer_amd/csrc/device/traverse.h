@@ -111,9 +111,34 @@ struct LaneStack {
     }
 };
 
+// Push the hit children of a quad node: sorted by the tagged tmin (nearest
+// popped first) with the reference's 5-comparator network, or in slot order.
+template <bool SORT>
+CT_DEV void push_children(uint32_t hm, float m0, float m1, float m2, float m3, uint32_t e0, uint32_t e1,
+                          uint32_t e2, uint32_t e3, LaneStack& st) {
+    if (hm == 0u) return;
+    if (!SORT) {
+        st.push4(e0, hm & 1u, e1, (hm >> 1) & 1u, e2, (hm >> 2) & 1u, e3, (hm >> 3) & 1u);
+        return;
+    }
+    float s0 = m0, s1 = m1, s2 = m2, s3 = m3, tmp;
+    if (s0 > s1) { tmp = s0; s0 = s1; s1 = tmp; }
+    if (s2 > s3) { tmp = s2; s2 = s3; s3 = tmp; }
+    if (s0 > s2) { tmp = s0; s0 = s2; s2 = tmp; }
+    if (s1 > s3) { tmp = s1; s1 = s3; s3 = tmp; }
+    if (s2 > s3) { tmp = s2; s2 = s3; s3 = tmp; }
+    auto entry = [&](uint32_t slot) { return slot == 0 ? e0 : (slot == 1 ? e1 : (slot == 2 ? e2 : e3)); };
+    const uint32_t q3 = fbits(s3) & 3u, q2 = fbits(s2) & 3u, q1 = fbits(s1) & 3u, q0 = fbits(s0) & 3u;
+    st.push4(entry(q3), (hm >> q3) & 1u, entry(q2), (hm >> q2) & 1u, entry(q1), (hm >> q1) & 1u, entry(q0),
+             (hm >> q0) & 1u);
+}
+
 // Quad node test (MBVHNode::intersect): four slabs in the reference's float
 // order, tmin tagged with the slot, the 5-comparator network, then pushes in
-// the order of traverseMBVH (sorted position 3 first).
+// the order of traverseMBVH (sorted position 3 first).  SORT = false skips
+// the network: an any-hit query's result does not depend on the visit order
+// (its t never shrinks, so every culling decision is the same).
+template <bool SORT>
 CT_DEV void node4_step(const float4* __restrict__ nodes4, uint32_t e, f3 o, f3 inv, float t, LaneStack& st) {
     const float4* n = nodes4 + 8 * (size_t)e;
     const float4 bminx = n[0], bmaxx = n[1], bminy = n[2], bmaxy = n[3], bminz = n[4], bmaxz = n[5];
@@ -145,21 +170,51 @@ CT_DEV void node4_step(const float4* __restrict__ nodes4, uint32_t e, f3 o, f3 i
                         ((x1 > 0.0f && m1 <= x1 && m1 < t && c1 != 0) ? 2u : 0u) |
                         ((x2 > 0.0f && m2 <= x2 && m2 < t && c2 != 0) ? 4u : 0u) |
                         ((x3 > 0.0f && m3 <= x3 && m3 < t && c3 != 0) ? 8u : 0u);
-    if (hm == 0u) return;
-    float s0 = m0, s1 = m1, s2 = m2, s3 = m3, tmp;
-    if (s0 > s1) { tmp = s0; s0 = s1; s1 = tmp; }
-    if (s2 > s3) { tmp = s2; s2 = s3; s3 = tmp; }
-    if (s0 > s2) { tmp = s0; s0 = s2; s2 = tmp; }
-    if (s1 > s3) { tmp = s1; s1 = s3; s3 = tmp; }
-    if (s2 > s3) { tmp = s2; s2 = s3; s3 = tmp; }
     const uint32_t e0 = c0 == -1 ? fbits(ch.x) : (kLeafBit | fbits(ch.x));
     const uint32_t e1 = c1 == -1 ? fbits(ch.y) : (kLeafBit | fbits(ch.y));
     const uint32_t e2 = c2 == -1 ? fbits(ch.z) : (kLeafBit | fbits(ch.z));
     const uint32_t e3 = c3 == -1 ? fbits(ch.w) : (kLeafBit | fbits(ch.w));
-    auto entry = [&](uint32_t slot) { return slot == 0 ? e0 : (slot == 1 ? e1 : (slot == 2 ? e2 : e3)); };
-    const uint32_t q3 = fbits(s3) & 3u, q2 = fbits(s2) & 3u, q1 = fbits(s1) & 3u, q0 = fbits(s0) & 3u;
-    st.push4(entry(q3), (hm >> q3) & 1u, entry(q2), (hm >> q2) & 1u, entry(q1), (hm >> q1) & 1u, entry(q0),
-             (hm >> q0) & 1u);
+    push_children<SORT>(hm, m0, m1, m2, m3, e0, e1, e2, e3, st);
+}
+
+// Compressed quad node (64 B, layout in host/cnode.h): child planes decoded
+// to world space as fmaf(q, 2^k, p), then the same slab arithmetic, tagging
+// and push order as node4_step.  The decoded boxes contain the exact ones.
+template <bool SORT>
+CT_DEV void fnode4_step(const uint4* __restrict__ fnodes, uint32_t e, f3 o, f3 inv, float t, LaneStack& st) {
+    const uint4* n = fnodes + 4 * (size_t)e;
+    const uint4 h = n[0], qx = n[1], qy = n[2], ce = n[3];
+    const float px = __uint_as_float(h.x), py = __uint_as_float(h.y), pz = __uint_as_float(h.z);
+    const float sx = __uint_as_float((h.w & 0xFFu) << 23), sy = __uint_as_float(((h.w >> 8) & 0xFFu) << 23),
+                sz = __uint_as_float(((h.w >> 16) & 0xFFu) << 23);
+    const uint32_t valid = h.w >> 24;
+    float m0, m1, m2, m3, x0, x1, x2, x3;
+#define CT_QB(w, i) ((float)(((w) >> (8 * (i))) & 0xFFu))
+#define CT_FSLAB(MN, MX, i)                                                                        \
+    {                                                                                              \
+        float t1 = (fmaf(CT_QB(qx.x, i), sx, px) - o.x) * inv.x;                                    \
+        float t2 = (fmaf(CT_QB(qx.y, i), sx, px) - o.x) * inv.x;                                    \
+        float mn = gmin(t1, t2), mx = gmax(t1, t2);                                                \
+        t1 = (fmaf(CT_QB(qx.z, i), sy, py) - o.y) * inv.y;                                          \
+        t2 = (fmaf(CT_QB(qx.w, i), sy, py) - o.y) * inv.y;                                          \
+        mn = gmax(mn, gmin(t1, t2));                                                               \
+        mx = gmin(mx, gmax(t1, t2));                                                               \
+        t1 = (fmaf(CT_QB(qy.x, i), sz, pz) - o.z) * inv.z;                                          \
+        t2 = (fmaf(CT_QB(qy.y, i), sz, pz) - o.z) * inv.z;                                          \
+        mn = gmax(mn, gmin(t1, t2));                                                               \
+        mx = gmin(mx, gmax(t1, t2));                                                               \
+        MN = tag(mn, i);                                                                           \
+        MX = mx;                                                                                   \
+    }
+    CT_FSLAB(m0, x0, 0u)
+    CT_FSLAB(m1, x1, 1u)
+    CT_FSLAB(m2, x2, 2u)
+    CT_FSLAB(m3, x3, 3u)
+#undef CT_FSLAB
+#undef CT_QB
+    const uint32_t hm = ((x0 > 0.0f && m0 <= x0 && m0 < t) ? 1u : 0u) | ((x1 > 0.0f && m1 <= x1 && m1 < t) ? 2u : 0u) |
+                        ((x2 > 0.0f && m2 <= x2 && m2 < t) ? 4u : 0u) | ((x3 > 0.0f && m3 <= x3 && m3 < t) ? 8u : 0u);
+    push_children<SORT>(hm & valid, m0, m1, m2, m3, qy.z, qy.w, ce.x, ce.y, st);
 }
 
 // BVHNode::Intersect (BVHNode.cuh:50-62): no culling against the current t.
@@ -189,7 +244,8 @@ CT_DEV void node2_step(const float4* __restrict__ nodes2, uint32_t lf, f3 o, f3 
 
 // ----------------------------------------------------------------------
 // Persistent traversal driver.  Source: how ray i is loaded; Sink: what is
-// done with the result.  ACCEL 4 = quad tree, 2 = binary tree.
+// done with the result.  ACCEL 4 = quad tree, 5 = compressed quad tree,
+// 2 = binary tree.
 struct RayIn {
     f3 o, d;
     float tmax;
@@ -282,12 +338,13 @@ CT_DEV void trace_persistent(const DevScene& sc, float eps, uint32_t n, uint32_t
         // ---- one step: pop (+ node test) and/or one triangle
         if (k == kNoLeaf) {
             const uint32_t e = st.top_pop();
-            if (ACCEL == 4) {
+            if (ACCEL == 4 || ACCEL == 5) {
                 if (e & kLeafBit) {
                     k = e & ~kLeafBit;
                 } else {
                     if (COUNT) ++cnt.nodes;
-                    node4_step(sc.nodes4, e, o, inv, t, st);
+                    if (ACCEL == 4) node4_step<!AnyHit>(sc.nodes4, e, o, inv, t, st);
+                    else fnode4_step<!AnyHit>(sc.fnodes, e, o, inv, t, st);
                 }
             } else {
                 const float4 lo = sc.nodes2[2 * (size_t)e];

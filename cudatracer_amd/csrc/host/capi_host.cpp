@@ -7,6 +7,7 @@
 
 #include "../../../include/cudatracer.h"
 #include "bvh.h"
+#include "cnode.h"
 #include "scene.h"
 #include "scenegen.h"
 
@@ -20,6 +21,11 @@ struct ct_host_scene {
 };
 
 extern "C" {
+
+int ct_compress_quad_nodes(const ct_mbvh_node* nodes, uint32_t n, uint32_t* out_words) {
+    if ((!nodes || !out_words) && n) return CT_ERR_INVALID_ARG;
+    return ct::compress_quad_nodes(nodes, n, out_words) ? CT_ERR_INVALID_SCENE : CT_OK;
+}
 
 int ct_host_scene_create(ct_host_scene** out) {
     if (!out) return CT_ERR_INVALID_ARG;

@@ -7,6 +7,7 @@
 #include <cstring>
 #include <sstream>
 
+#include "../host/cnode.h"
 #include "runtime.h"
 
 using ctd::Segment;
@@ -227,6 +228,9 @@ int ct_scene_upload(ct_ctx* c, const ct_scene_view* v) {
     for (uint32_t l = 0; l < v->n_lights; ++l) light_rows(v, v->light_indices[l], v->materials, &lights[5 * (size_t)l]);
     std::vector<ct_mbvh_node> n4(v->mbvh_nodes, v->mbvh_nodes + v->n_mbvh_nodes);
     std::vector<ct_bvh_node> n2(v->bvh_nodes, v->bvh_nodes + v->n_bvh_nodes);
+    std::vector<uint32_t> fn((size_t)ct::kCompressedNodeWords * n4.size());
+    if (!n4.empty() && ct::compress_quad_nodes(n4.data(), (uint32_t)n4.size(), fn.data()))
+        return fail(c, CT_ERR_INVALID_SCENE, "quad node bounds are not finite");
 
     HIPCHK(hipStreamSynchronize(c->stream), "scene upload");
     free_scene(c);
@@ -237,13 +241,18 @@ int ct_scene_upload(ct_ctx* c, const ct_scene_view* v) {
     if (r == CT_OK) r = upload_vec(c, mats, &sc.mats, "materials");
     if (r == CT_OK) r = upload_vec(c, lights, &sc.lights, "lights");
     if (r == CT_OK && !n4.empty()) r = upload_vec(c, n4, &sc.nodes4, "quad nodes");
+    if (r == CT_OK && !n4.empty()) {
+        const float4* fp = nullptr;
+        r = upload_vec(c, fn, &fp, "compressed quad nodes");
+        sc.fnodes = reinterpret_cast<const uint4*>(fp);
+    }
     if (r == CT_OK && !n2.empty()) r = upload_vec(c, n2, &sc.nodes2, "binary nodes");
     if (r != CT_OK) { free_scene(c); return r; }
     HIPCHK(hipStreamSynchronize(c->stream), "scene upload");
     sc.n_tris = N;
     sc.n_nodes4 = v->n_mbvh_nodes;
     sc.n_nodes2 = v->n_bvh_nodes;
-    sc.n_fnodes = 0;
+    sc.n_fnodes = v->n_mbvh_nodes;
     sc.n_mats = v->n_materials;
     sc.n_lights = v->n_lights;
     sc.sky_enabled = v->sky_enabled ? 1 : 0;

@@ -347,6 +347,14 @@ int ct_host_scene_stats(const ct_host_scene* s, uint64_t out[8]);
 int ct_trace_frame_counted(ct_ctx* ctx, const ct_camera* cam, const ct_frame_opts* opts,
                            ct_frame_stats* stats, uint64_t work[4]);
 
+/* The 64-byte compressed quad nodes of CT_TRAV_FAST, 16 words per node,
+ * built by ct_scene_upload from the quad tree (no counterpart in the
+ * reference; exposed so the quantisation can be checked on the host).
+ * Words: p.xyz (float), biased exponents of the per-axis steps | valid-child
+ * mask << 24, per axis the low and high 8-bit child offsets, four child
+ * entries (node index, or 0x80000000 | first primitive-list position). */
+int ct_compress_quad_nodes(const ct_mbvh_node* nodes, uint32_t n, uint32_t* out_words);
+
 #ifdef __cplusplus
 }
 #endif

@@ -55,6 +55,55 @@ def test_closest_hit_matches_brute_force_c1(ct, orc, gpu_ctx, c1):
     assert ties.size <= 4
 
 
+@pytest.mark.parametrize("config", [1, 2])
+def test_fast_traversal_closest_hit(ct, orc, gpu_ctx, config):
+    """Compressed 64-byte quad nodes (CT_TRAV_FAST) against the oracle's exact
+    traversal: the same triangle records are tested with the same arithmetic,
+    so t is bit-identical wherever both find the same nearest surface; ids
+    differ only on exact t ties.  The few rays whose t differs are ones where
+    the exact traversal's tagged tmin (the reference's low-bit slot tag) culls
+    a box within ulps of the current hit; there the difference is a few ulps."""
+    from conftest import get_scene
+    scene, arrays, osc = get_scene(ct, orc, config)
+    _upload(gpu_ctx, scene)
+    o, d = ray_set(scene, arrays, N_RAYS, seed=200 + config)
+    opts, _, _ = scene.frame_opts()
+    opts.accel = ct.ACCEL_BVH4
+    opts.traversal = ct.TRAV_FAST
+    t_gpu, p_gpu = gpu_ctx.trace_rays(o, d, opts)
+    t_ref, p_ref = osc.trace(o, d, 1, 32, opts.triangle_epsilon, threads=16)
+    t_ref32 = t_ref.astype(np.float32)
+    same_t = t_gpu.view(np.uint32) == t_ref32.view(np.uint32)
+    diff_t = np.flatnonzero(~same_t)
+    ties = np.flatnonzero(same_t & (p_gpu != p_ref))
+    print(f"config {config}: rays {N_RAYS}, t differs {diff_t.size}, id ties {ties.size}")
+    assert diff_t.size <= N_RAYS // 100000
+    if diff_t.size:
+        rel = np.abs(t_gpu[diff_t] - t_ref32[diff_t]) / t_ref32[diff_t]
+        assert np.all(rel <= 1e-6), rel.max()
+    assert ties.size <= N_RAYS // 10000
+
+
+@pytest.mark.parametrize("config", [1, 2])
+def test_fast_traversal_any_hit(ct, orc, gpu_ctx, config):
+    """Any-hit over compressed nodes equals the exact any-hit (order-free)."""
+    from conftest import get_scene
+    scene, arrays, osc = get_scene(ct, orc, config)
+    _upload(gpu_ctx, scene)
+    n = 1 << 18
+    o, d = ray_set(scene, arrays, n, seed=300 + config)
+    rng = np.random.default_rng(config)
+    tmax = rng.uniform(0.0, 60.0, n).astype(np.float32)
+    opts, _, _ = scene.frame_opts()
+    opts.accel = ct.ACCEL_BVH4
+    opts.traversal = ct.TRAV_FAST
+    occ_gpu = gpu_ctx.trace_shadow_rays(o, d, tmax, opts)
+    occ_ref = osc.trace_shadow(o, d, tmax, 1, 32, opts.triangle_epsilon, threads=16)
+    bad = int(np.sum(occ_gpu != occ_ref))
+    print(f"config {config}: occluded {int(occ_ref.sum())} of {n}, differ {bad}")
+    assert bad <= n // 100000
+
+
 @pytest.mark.parametrize("config,accel", [(1, 0), (1, 1), (2, 1)])
 def test_shadow_any_hit(ct, orc, gpu_ctx, config, accel):
     from conftest import get_scene

@@ -158,6 +158,51 @@ def test_builder_invariants(ct, orc, config):
     assert max_depth <= 63
 
 
+def _mbvh_records(raw: np.ndarray):
+    return np.frombuffer(raw.tobytes(), dtype=[("lo", "<f4", (3, 4)), ("hi", "<f4", (3, 4)), ("ch", "<i4", 4),
+                                               ("cnt", "<i4", 4)])
+
+
+def _interleave_mbvh(rec):
+    """(n, 3 axes, 4 children) lo/hi from bminx, bmaxx, bminy, bmaxy, bminz, bmaxz."""
+    f = np.frombuffer(rec.tobytes(), "<f4").reshape(-1, 32)[:, :24].reshape(-1, 6, 4)
+    return f[:, 0::2, :], f[:, 1::2, :]
+
+
+@pytest.mark.parametrize("config", [1, 2])
+def test_compressed_nodes_contain_exact_boxes(ct, orc, config):
+    """Every decoded child box of the fast mode's 64-byte nodes, computed as
+    the kernel does (fmaf(q, 2^k, p), one rounding), contains the exact child
+    box; padding is at most a few grid steps; child entries and the valid mask
+    match the quad tree."""
+    scene, arrays, _ = get_scene(ct, orc, config)
+    rec = _mbvh_records(arrays["mbvh"])
+    lo, hi = _interleave_mbvh(rec)
+    w = ct.compress_quad_nodes(arrays["mbvh"])
+    assert w.shape == (len(rec), 16)
+    valid = (w[:, 3] >> 24) & 0xF
+    cnt = rec["cnt"]
+    assert np.array_equal(valid, ((cnt != 0) * (1 << np.arange(4))).sum(1))
+    p = w[:, 0:3].view(np.float32).astype(np.float64)
+    k = np.stack([((w[:, 3] >> (8 * a)) & 0xFF).astype(np.int64) - 127 for a in range(3)], 1)
+    step = np.ldexp(1.0, k)
+    for a in range(3):
+        ql = np.stack([(w[:, 4 + 2 * a] >> (8 * c)) & 0xFF for c in range(4)], 1).astype(np.float64)
+        qh = np.stack([(w[:, 5 + 2 * a] >> (8 * c)) & 0xFF for c in range(4)], 1).astype(np.float64)
+        # q * 2^k + p is exact in float64 here; one rounding to float32 = fmaf
+        dlo = (ql * step[:, a:a + 1] + p[:, a:a + 1]).astype(np.float32)
+        dhi = (qh * step[:, a:a + 1] + p[:, a:a + 1]).astype(np.float32)
+        m = cnt != 0
+        assert np.all(dlo[m] <= lo[:, a, :][m]) and np.all(dhi[m] >= hi[:, a, :][m])
+        pad = np.maximum(lo[:, a, :] - dlo, dhi - hi[:, a, :])
+        assert np.all(pad[m] <= 4 * np.broadcast_to(step[:, a:a + 1], pad.shape)[m])
+    ent = w[:, 10:14]
+    leaf = cnt > 0
+    inner = cnt == -1
+    assert np.array_equal(ent[inner], rec["ch"][inner].astype(np.uint32))
+    assert np.array_equal(ent[leaf], rec["ch"][leaf].astype(np.uint32) | np.uint32(0x80000000))
+
+
 # -------------------------------------------------------------- traversal --
 @pytest.mark.parametrize("config,accel", [(1, 0), (1, 1), (2, 1)])
 def test_traversal_equals_brute_force(ct, orc, config, accel):
