@@ -14,6 +14,19 @@ frames).
 
 cpu_baseline: the CPU oracle (FP32 restatement of the reference loop, OpenMP
 over pixels) on whole frames of the same workload, about 10 s of CPU work.
+
+served (unless --no-serve): render requests on the resident atrium arriving
+as a Poisson stream (resolution 256x256 / 512x512 / 1920x1080 with
+probability .5 / .35 / .15, spp uniform in {1, 4, 16}, depth 8, seeded
+cameras).  p50 / p99 of request latency (completion - arrival) for the render
+server (continuous batching) and for a plain loop that renders requests one
+at a time with ct_trace_frame; images/s of both with all requests queued at
+once.  Every request renders the same path samples either way (checked
+bit-exact on a subset), so the comparison is at equal rel-MSE.
+
+batched: 64 views of the box-and-sphere scene (sides log-uniform in
+[128, 1024], aspect 1:1 or 4:3, spp in {1, 2, 4, 8, 16}, depth 8) in one
+ct_render_batch call vs a plain per-view loop; images/s of both.
 """
 from __future__ import annotations
 
@@ -148,11 +161,127 @@ def main() -> int:
         "roofline": roofline,
     }
 
+    if not args.no_serve:
+        out["served"] = serve_leg(ct, ctx, scene, opts, args.seed)
+        out["batched"] = batched_leg(ct, ctx, args.seed)
+    ctx.close()
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(scene, cam, opts, W, H, args.cpu_seconds)
-    ctx.close()
     print(json.dumps(out))
     return 0
+
+
+def _pct(x, q):
+    import numpy as np
+    return round(float(np.percentile(np.asarray(x), q)), 3)
+
+
+def _fresh(ctx, w: int, h: int) -> None:
+    """Empty accumulator of w x h (reallocated only when the size changes)."""
+    if (ctx.width, ctx.height) != (w, h):
+        ctx.resize(w, h)
+    else:
+        ctx.reset()
+
+
+def serve_requests(ct, scene, base_opts, seed: int, n: int):
+    """The served workload: n requests (camera, w, h, spp, opts) on the atrium."""
+    import numpy as np
+    rng = np.random.default_rng(seed)
+    sizes = [(256, 256), (512, 512), (1920, 1080)]
+    reqs = []
+    for _ in range(n):
+        w, h = sizes[rng.choice(3, p=[0.5, 0.35, 0.15])]
+        spp = int(rng.choice([1, 4, 16]))
+        pos = np.array([-85.0, 14.0, -80.0]) + rng.uniform(-10, 10, 3)
+        pos[1] = min(max(pos[1], 3.0), 27.0)
+        tgt = np.array([10.0, 9.0, 15.0]) + rng.uniform(-20, 20, 3)
+        o = base_opts.copy()
+        o.spp = spp
+        reqs.append((ct.look_at(pos, tgt, 70.0), w, h, spp, o))
+    return reqs
+
+
+def serve_leg(ct, ctx, scene, opts, seed: int, n: int = 120, rate: float = 40.0) -> dict:
+    import numpy as np
+    reqs = serve_requests(ct, scene, opts, seed + 1, n)
+    arrivals = np.cumsum(np.random.default_rng(seed + 2).exponential(1.0 / rate, n))
+    # plain loop: one request at a time (resize, reset, trace spp samples, read back)
+    service, plain_out = [], {}
+    for i, (cam, w, h, spp, o) in enumerate(reqs):
+        t0 = time.perf_counter()
+        _fresh(ctx, w, h)
+        ctx.trace_frame(cam, o, stats=False)
+        acc, _ = ctx.read_accum()
+        service.append(time.perf_counter() - t0)
+        if w == 256 and len(plain_out) < 8:
+            plain_out[i] = acc
+    finish, plain_lat = 0.0, []
+    for a, sv in zip(arrivals, service):  # FIFO single server on the measured service times
+        finish = max(finish, a) + sv
+        plain_lat.append((finish - a) * 1e3)
+    plain_ips = n / sum(service)
+    srv = ct.Server(ctx)
+    # Poisson stream
+    ids = []
+    t0 = time.perf_counter()
+    for (cam, w, h, spp, o), a in zip(reqs, arrivals):
+        dt = a - (time.perf_counter() - t0)
+        if dt > 0:
+            time.sleep(dt)
+        ids.append(srv.submit(cam, w, h, spp, o))
+    lat, same = [], 0
+    for i, rid in enumerate(ids):
+        res, out = srv.wait(rid)
+        lat.append(res.done_ms - res.submit_ms)
+        if i in plain_out:
+            same += int(np.array_equal(out.view(np.uint32), plain_out[i].view(np.uint32)))
+    # saturated: everything queued at once
+    c0 = srv.clock_ms()
+    ids = [srv.submit(cam, w, h, spp, o) for (cam, w, h, spp, o) in reqs]
+    last = max(srv.wait(rid)[0].done_ms for rid in ids)
+    srv.close()
+    served_ips = n / ((last - c0) * 1e-3)
+    return {"requests": n, "rate_per_s": rate, "p50_ms": _pct(lat, 50), "p99_ms": _pct(lat, 99),
+            "plain_p50_ms": _pct(plain_lat, 50), "plain_p99_ms": _pct(plain_lat, 99),
+            "images_per_s": round(served_ips, 2), "plain_images_per_s": round(plain_ips, 2),
+            "speedup": round(served_ips / plain_ips, 3), "bit_exact_vs_plain": f"{same}/{len(plain_out)}"}
+
+
+def batched_leg(ct, ctx, seed: int, n_views: int = 64) -> dict:
+    import numpy as np
+    scene = ct.HostScene.generate(ct.SCENE_C1, seed)
+    opts, _, _ = scene.frame_opts()
+    opts.max_depth = 8
+    rng = np.random.default_rng(seed + 3)
+    views = []
+    for _ in range(n_views):
+        w = int(round(np.exp(rng.uniform(np.log(128), np.log(1024)))))
+        h = w if rng.random() < 0.5 else (w * 3) // 4
+        spp = int(rng.choice([1, 2, 4, 8, 16]))
+        d = rng.normal(size=3)
+        d /= np.linalg.norm(d)
+        pos = d * 42.0
+        views.append((ct.look_at(pos, rng.uniform(-5, 5, 3), 70.0), w, h, spp, 0))
+    ctx.upload(scene)
+    ctx.render_batch(views[:4], opts)  # warm (graph capture, buffers)
+    t0 = time.perf_counter()
+    outs, st = ctx.render_batch(views, opts)
+    t_batch = time.perf_counter() - t0
+    t_plain, same = 0.0, 0
+    for i, (cam, w, h, spp, fb) in enumerate(views):
+        o = opts.copy()
+        o.spp = spp
+        t0 = time.perf_counter()
+        _fresh(ctx, w, h)
+        ctx.trace_frame(cam, o, stats=False)
+        acc, _ = ctx.read_accum()
+        t_plain += time.perf_counter() - t0
+        if i < 8:
+            same += int(np.array_equal(acc.view(np.uint32), outs[i].view(np.uint32)))
+    return {"views": n_views, "path_samples": int(sum(w * h * spp for (_, w, h, spp, _) in views)),
+            "images_per_s": round(n_views / t_batch, 2), "plain_images_per_s": round(n_views / t_plain, 2),
+            "speedup": round(t_plain / t_batch, 3), "chunks": int(st.chunks), "bit_exact_vs_plain": f"{same}/8"}
 
 
 def summarize(paths: list) -> int:

@@ -118,6 +118,9 @@ void ct_server::loop() {
     std::vector<ViewDesc> views;
     std::vector<Run> runs;
     std::vector<uint64_t> ids;
+    // accumulators to zero before the chunk: (float4 offset, pixels), copied
+    // out under the lock (submit() may insert into reqs concurrently)
+    std::vector<std::pair<uint64_t, uint64_t>> fresh;
     while (true) {
         {
             std::unique_lock<std::mutex> lk(mu);
@@ -165,6 +168,7 @@ void ct_server::loop() {
             views.clear();
             runs.clear();
             ids.clear();
+            fresh.clear();
             uint64_t room = chunk;
             for (uint64_t id : active) {
                 if (room == 0) break;
@@ -174,19 +178,16 @@ void ct_server::loop() {
                 views.push_back(make_view(q.r.camera, q.r.width, q.r.height, q.r.spp, q.r.frame_base, q.acc_off));
                 runs.push_back({(uint32_t)(views.size() - 1), (uint32_t)q.next, (uint32_t)take});
                 ids.push_back(id);
+                if (q.next == 0) fresh.emplace_back(q.acc_off, (uint64_t)q.r.width * q.r.height);
                 q.next += take;
                 room -= take;
             }
         }
         // zero accumulators of requests starting in this chunk, then trace
         int rc = CT_OK;
-        for (size_t k = 0; k < ids.size() && rc == CT_OK; ++k)
-            if (runs[k].first_path == 0) {
-                const Req& q = reqs[ids[k]];
-                if (hipMemsetAsync(arena_ptr + q.acc_off, 0, (size_t)q.r.width * q.r.height * sizeof(float4),
-                                   ctx->stream) != hipSuccess)
-                    rc = CT_ERR_DEVICE;
-            }
+        for (const auto& f : fresh)
+            if (rc == CT_OK && hipMemsetAsync(arena_ptr + f.first, 0, f.second * sizeof(float4), ctx->stream) != hipSuccess)
+                rc = CT_ERR_DEVICE;
         ChunkStats st;
         if (rc == CT_OK)
             rc = run_paths(ctx, views, runs, chunk, arena_ptr, fopts, false, false, true, &st);
@@ -215,9 +216,12 @@ void ct_server::loop() {
             if (q.next >= q.total) {
                 const size_t n = (size_t)q.r.width * q.r.height * 4;
                 q.result.resize(n);
+                const uint64_t off = q.acc_off;
+                float* dst = q.result.data();  // map nodes stay put while unlocked
                 lk.unlock();
-                const hipError_t e = hipMemcpy(q.result.data(), arena_ptr + q.acc_off, n * sizeof(float),
-                                               hipMemcpyDeviceToHost);
+                hipError_t e = hipMemcpyAsync(dst, arena_ptr + off, n * sizeof(float), hipMemcpyDeviceToHost,
+                                              ctx->stream);
+                if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
                 lk.lock();
                 q.status = e == hipSuccess ? CT_OK : CT_ERR_DEVICE;
                 q.state = DONE;
