@@ -41,6 +41,12 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
 METRIC = "Mrays/s at 1080p depth 8; served p50/p99 ms/request at equal rel-MSE"
+WORKLOADS = {
+    2: "config 2: procedural atrium (%d triangles), 1920x1080, 1 spp, depth 8, MIS+NEE, quad BVH, %s traversal, "
+       "typed BSDF (Lambert/GGX/mirror/dielectric) + 4 quad lights + sky",
+    3: "config 3: displaced icosphere + noisy ground (%d triangles), 3840x2160, 1 spp, depth 16, Russian roulette "
+       "from bounce 3, quad BVH, %s traversal, Beckmann/GGX microfacets + 1 area light",
+}
 HBM_PEAK_GBS = 8000.0
 
 
@@ -51,6 +57,8 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--seed", type=int, default=7)
     ap.add_argument("--traversal", default="exact", choices=["exact", "fast"])
+    ap.add_argument("--config", type=int, default=2, choices=[2, 3],
+                    help="2: atrium 1080p depth 8 (the headline workload); 3: displaced sphere 4K depth 16, RR")
     ap.add_argument("--cpu-seconds", type=float, default=10.0, help="minimum CPU-baseline work")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
@@ -72,7 +80,7 @@ def main() -> int:
     import numpy as np
     import cudatracer_amd as ct
 
-    scene = ct.HostScene.generate(ct.SCENE_C2, args.seed)
+    scene = ct.HostScene.generate(ct.SCENE_C2 if args.config == 2 else ct.SCENE_C3, args.seed)
     opts, W, H = scene.frame_opts()
     opts.traversal = ct.TRAV_FAST if args.traversal == "fast" else ct.TRAV_EXACT
     cam = scene.camera(W, H)
@@ -146,11 +154,9 @@ def main() -> int:
         "dtype": "f32",
         "data": "synthetic",
         "config": {
-            "workload": "config 2: procedural atrium (%d triangles), 1920x1080, 1 spp, depth 8, MIS+NEE, "
-                        "quad BVH, %s traversal, typed BSDF (Lambert/GGX/mirror/dielectric) + 4 quad lights + sky"
-                        % (st_scene["triangles"], args.traversal),
+            "workload": WORKLOADS[args.config] % (st_scene["triangles"], args.traversal),
             "seed": args.seed,
-            "width": W, "height": H, "spp": 1, "depth": opts.max_depth,
+            "width": W, "height": H, "spp": 1, "depth": opts.max_depth, "rr_start": opts.rr_start,
             "device": ctx.name(),
         },
         "rays_per_step": {"primary": prim // args.steps, "extension": ext // args.steps, "shadow": sh // args.steps},
@@ -161,12 +167,13 @@ def main() -> int:
         "roofline": roofline,
     }
 
-    if not args.no_serve:
+    if not args.no_serve and args.config == 2:
         out["served"] = serve_leg(ct, ctx, scene, opts, args.seed)
         out["batched"] = batched_leg(ct, ctx, args.seed)
     ctx.close()
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(scene, cam, opts, W, H, args.cpu_seconds)
+        out["cpu_baseline"] = cpu_baseline(scene, cam, opts, W, H, args.cpu_seconds,
+                                           strip=1 if args.config == 2 else 16)
     print(json.dumps(out))
     return 0
 
@@ -323,25 +330,29 @@ def copy_probe(mib: int) -> int:
     return 0
 
 
-def cpu_baseline(scene, cam, opts, W, H, min_seconds: float, max_frames: int = 64) -> dict:
-    """Oracle FP32 on whole benchmark frames (frame indices 0, 1, ...) until
-    at least min_seconds of CPU work, at most max_frames frames."""
+def cpu_baseline(scene, cam, opts, W, H, min_seconds: float, max_frames: int = 64, strip: int = 1) -> dict:
+    """Oracle FP32 on the benchmark frames (frame indices 0, 1, ...) until at
+    least min_seconds of CPU work, at most max_frames frames.  strip > 1
+    renders the first H / strip rows of each frame only (a bounded sample of
+    the 4K workload)."""
     import numpy as np
     from oracle import oracle as orc
     threads = min(16, os.cpu_count() or 1)
     osc = orc.Scene(scene.arrays())
+    rows = H // strip
     acc = np.zeros((H, W, 4), np.float32)
     rays = frames = 0
     t0 = time.perf_counter()
     while frames < max_frames:
-        _, st = osc.render(cam, opts, W, H, frames, 1, threads=threads, accum=acc)
+        _, st = osc.render(cam, opts, W, H, frames, 1, rows=(0, rows), threads=threads, accum=acc)
         rays += st["primary_rays"] + st["extension_rays"] + st["shadow_rays"]
         frames += 1
         if time.perf_counter() - t0 >= min_seconds:
             break
     dt = time.perf_counter() - t0
+    what = f"full {W}x{H} frames" if strip == 1 else f"{W}x{rows} strips (rows 0..{rows - 1}) of {W}x{H} frames"
     return {"value": round(rays / dt / 1e6, 3), "unit": "Mrays/s", "cores": threads, "kind": "port",
-            "sample": f"{frames} full 1920x1080 frames at 1 spp (frame indices 0..{frames - 1}), {rays} rays, "
+            "sample": f"{frames} {what} at 1 spp (frame indices 0..{frames - 1}), {rays} rays, "
                       f"{dt:.1f} s on {threads} OpenMP threads; oracle built -O3 -march=x86-64-v3 without fast-math"}
 
 

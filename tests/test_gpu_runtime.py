@@ -1,0 +1,190 @@
+"""GPU tests of the runtime around the kernels: the render server, checkpoint
+and resume of the accumulator, material edits, the autotuner, the tonemap
+read-out and error returns.  Results are compared with ct_trace_frame of the
+same samples (bit-exact: the path samples do not depend on how they are
+batched) or with the CPU oracle.
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _bits(a):
+    return np.ascontiguousarray(a).view(np.uint32)
+
+
+def _plain(ctx, cam, w, h, opts, spp, frame_base=0):
+    """ct_trace_frame of spp samples from frame index frame_base."""
+    ctx.resize(w, h)
+    if frame_base:
+        ctx.write_accum(np.zeros((h, w, 4), np.float32), frame_base)
+    o = opts.copy()
+    o.spp = spp
+    ctx.trace_frame(cam, o)
+    acc, _ = ctx.read_accum()
+    return acc
+
+
+def test_server_requests_equal_plain_frames(ct, gpu_ctx, c2):
+    """Requests of mixed size and spp submitted together (so they share
+    wavefront chunks) give the accumulators of separate plain frames."""
+    scene, _, _ = c2
+    gpu_ctx.upload(scene)
+    opts, _, _ = scene.frame_opts()
+    reqs = [(ct.look_at((-85 + 3 * i, 14, -80), (10, 9 - i, 15), 70.0), w, h, spp)
+            for i, (w, h, spp) in enumerate([(64, 48, 4), (160, 90, 1), (96, 96, 16), (33, 17, 3), (128, 128, 2)])]
+    expect = [_plain(gpu_ctx, cam, w, h, opts, spp) for (cam, w, h, spp) in reqs]
+    srv = ct.Server(gpu_ctx, chunk_paths=1 << 14)
+    try:
+        ids = []
+        for cam, w, h, spp in reqs:
+            o = opts.copy()
+            o.spp = spp
+            ids.append(srv.submit(cam, w, h, spp, o))
+        for rid, e, (_, w, h, spp) in zip(ids, expect, reqs):
+            res, out = srv.wait(rid)
+            assert res.status == 0 and (res.width, res.height, res.spp) == (w, h, spp)
+            assert res.done_ms >= res.admit_ms >= res.submit_ms
+            assert np.array_equal(_bits(out), _bits(e)), (w, h, spp)
+    finally:
+        srv.close()
+
+
+def test_server_cancel_and_admission(ct, gpu_ctx, c1):
+    scene, _, _ = c1
+    gpu_ctx.upload(scene)
+    opts, w, h = scene.frame_opts()
+    cam = scene.camera(w, h)
+    srv = ct.Server(gpu_ctx, mem_budget=64 << 20, chunk_paths=1 << 12)
+    try:
+        with pytest.raises(ct.CTError) as e:  # 4096^2 float4 accumulator > 64 MiB
+            srv.submit(cam, 4096, 4096, 1, opts)
+        assert e.value.code == ct.CT_ERR_ADMISSION
+        big = srv.submit(cam, 1024, 1024, 16, opts)
+        small = srv.submit(cam, 64, 64, 1, opts)
+        srv.cancel(big)
+        res, _ = srv.wait(big)
+        assert res.status in (0, ct.CT_ERR_CANCELLED)
+        res, out = srv.wait(small)
+        assert res.status == 0
+        assert np.array_equal(_bits(out), _bits(_plain(gpu_ctx, cam, 64, 64, opts, 1)))
+    finally:
+        srv.close()
+
+
+def test_checkpoint_resume(ct, gpu_ctx, c1):
+    """4 frames, read back, restore into a fresh context, 4 more frames ==
+    8 frames in one go (frame indices continue from the sample count)."""
+    scene, _, _ = c1
+    gpu_ctx.upload(scene)
+    opts, w, h = scene.frame_opts()
+    cam = scene.camera(w, h)
+    gpu_ctx.resize(w, h)
+    for _ in range(8):
+        gpu_ctx.trace_frame(cam, opts)
+    full, n = gpu_ctx.read_accum()
+    assert n == 8
+    gpu_ctx.reset()
+    for _ in range(4):
+        gpu_ctx.trace_frame(cam, opts)
+    half, n = gpu_ctx.read_accum()
+    other = ct.Context(0)
+    try:
+        other.upload(scene)
+        other.resize(w, h)
+        other.write_accum(half, n)
+        for _ in range(4):
+            other.trace_frame(cam, opts)
+        resumed, n2 = other.read_accum()
+    finally:
+        other.close()
+    assert n2 == 8
+    assert np.array_equal(_bits(resumed), _bits(full))
+
+
+def test_material_update_matches_oracle(ct, orc, gpu_ctx):
+    """Editing a material on the device (Application::updateMaterials) gives
+    the oracle's render of the edited scene."""
+    from oracle import oracle as orm
+    scene = ct.HostScene.generate(ct.SCENE_C1, 7)
+    opts, _, _ = scene.frame_opts()
+    w = h = 64
+    cam = scene.camera(w, h)
+    gpu_ctx.upload(scene)
+    gpu_ctx.resize(w, h)
+    arrays = scene.arrays()
+    mats = arrays["materials"]
+    idx = int(arrays["material_idx"][-1])  # the sphere's material
+    m = ct.Material()
+    C_ = __import__("ctypes")
+    C_.memmove(C_.byref(m), C_.byref(mats[idx]), C_.sizeof(m))
+    m.albedo[0], m.albedo[1], m.albedo[2] = 0.9, 0.1, 0.2
+    gpu_ctx.material_update(idx, m)
+    gpu_ctx.trace_frame(cam, opts)
+    acc, _ = gpu_ctx.read_accum()
+    C_.memmove(C_.byref(mats[idx]), C_.byref(m), C_.sizeof(m))
+    osc = orm.Scene(arrays)
+    ref, _ = osc.render(cam, opts, w, h, 0, 1, threads=16)
+    assert np.array_equal(_bits(acc), _bits(ref))
+
+
+def test_autotune_keeps_results(ct, gpu_ctx, c2):
+    scene, _, _ = c2
+    gpu_ctx.upload(scene)
+    opts, _, _ = scene.frame_opts()
+    w, h = 320, 180
+    cam = scene.camera(w, h)
+    before = _plain(gpu_ctx, cam, w, h, opts, 1)
+    r = gpu_ctx.autotune(cam, opts, 256, 144)
+    print("autotune:", r.block_size, r.stack_lds, r.chunk_paths, r.best_ms, r.trials)
+    assert r.trials > 1 and r.best_ms > 0
+    assert r.block_size in (64, 128, 256, 512, 1024) and r.stack_lds > 0
+    after = _plain(gpu_ctx, cam, w, h, opts, 1)
+    assert np.array_equal(_bits(before), _bits(after))
+
+
+def test_read_image_tonemap(ct, gpu_ctx, c1):
+    """draw_framebuffer (Kernel.cu:258-270): rgb / samples, then ^(1/2.2)."""
+    scene, _, _ = c1
+    gpu_ctx.upload(scene)
+    opts, w, h = scene.frame_opts()
+    cam = scene.camera(w, h)
+    gpu_ctx.resize(w, h)
+    for _ in range(3):
+        gpu_ctx.trace_frame(cam, opts)
+    acc, n = gpu_ctx.read_accum()
+    img = gpu_ctx.read_image()
+    ref = np.power(acc[..., :3].astype(np.float64) / n, 1.0 / 2.2)
+    assert np.allclose(img[..., :3], ref, rtol=2e-6, atol=1e-7)
+
+
+def test_error_returns(ct, c1):
+    ctx = ct.Context(0)
+    try:
+        scene, _, _ = c1
+        opts, w, h = scene.frame_opts()
+        cam = scene.camera(w, h)
+        ctx.resize(w, h)
+        with pytest.raises(ct.CTError) as e:
+            ctx.trace_frame(cam, opts)
+        assert e.value.code == ct.CT_ERR_NO_SCENE
+        ctx.upload(scene)
+        bad = opts.copy()
+        bad.max_depth = 0
+        with pytest.raises(ct.CTError) as e:
+            ctx.trace_frame(cam, bad)
+        assert e.value.code == ct.CT_ERR_INVALID_ARG
+        bad = opts.copy()
+        bad.accel = 7
+        with pytest.raises(ct.CTError) as e:
+            ctx.trace_frame(cam, bad)
+        assert e.value.code == ct.CT_ERR_INVALID_ARG
+        with pytest.raises(ct.CTError):
+            ctx.resize(0, 10)
+        ctx.resize(w, h)
+        ctx.trace_frame(cam, opts)  # still usable after the errors
+    finally:
+        ctx.close()
