@@ -130,10 +130,7 @@ CT_DEV uint32_t* cnt_shadow(uint32_t* c, int b) { return c + (kMaxBounce + 1) + 
 // 384-byte slot so ranges never share a cache line (same-line atomics from
 // different XCDs serialise).
 constexpr int kXcds = 8;
-#ifndef CT_FETCH_RANGES
-#define CT_FETCH_RANGES 1
-#endif
-constexpr int kFetchRanges = CT_FETCH_RANGES;
+constexpr int kFetchRanges = kXcds;
 constexpr int kFetchSlot = 96;
 constexpr int kFetchBase = 192;
 CT_DEV uint32_t* cnt_fetch_ext(uint32_t* c, int b) { return c + kFetchBase + b; }

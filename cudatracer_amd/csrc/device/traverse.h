@@ -12,8 +12,8 @@
 // every lane performs the reference's sequence of pops, node tests, pushes
 // and triangle tests for its own ray):
 //   - persistent waves fetch 64 rays per atomic once all their lanes are
-//     idle, from the queue range of their XCD first (see Fetcher); mixing
-//     fresh rays into a busy wave measured slower (lost coherence);
+//     idle, from the queue range of their XCD (see Fetcher); mixing fresh
+//     rays into a busy wave measured slower (lost coherence);
 //   - one step per loop iteration and lane: either pop + node test, or one
 //     triangle of the current leaf, so long leaves do not stall node lanes;
 //   - the per-lane stack lives in LDS, laid out [depth][thread]; pushes are
@@ -253,47 +253,39 @@ struct RayIn {
 
 constexpr uint32_t kNoLeaf = 0xFFFFFFFFu;
 
-#ifndef CT_FETCH_CHUNK
-#define CT_FETCH_CHUNK 64
-#endif
-#ifndef CT_FETCH_STEAL
-#define CT_FETCH_STEAL 1
-#endif
+// The queue [0, n) is cut into kFetchRanges contiguous ranges, one per XCD:
+// workgroups are dispatched to the XCDs round-robin by block id, so blocks
+// b = x (mod 8) share XCD x's L2, and their rays are neighbours in the queue.
+// A wave claims 64 rays with one atomic on its range's counter (each counter
+// on its own cache line; sharing one line serialised the atomics of all
+// XCDs and halved the throughput).  A range is drained only by its own
+// blocks: moving waves to other ranges at the end measured slower.  Every
+// range has blocks as long as the grid has at least kFetchRanges of them;
+// smaller grids move on to the next ranges.
+constexpr uint32_t kFetchChunk = 64;
 
-// The queue [0, n) is cut into kFetchRanges contiguous ranges.  A wave claims
-// CT_FETCH_CHUNK rays at a time with one atomic on the counter of its range
-// (workgroups are dispatched to the XCDs round-robin by block id, so each
-// XCD's L2 serves spatially coherent rays) and hands them to its lanes 64 at
-// a time.  All state is wave-uniform.
 struct Fetcher {
-    uint32_t* base;
+    uint32_t* base;  // counter of range 0 (range r at base + r * kFetchSlot)
     uint32_t n;
     int range;
-    uint32_t next, end;
+    bool steal;
 
-    CT_DEV bool claim() {
+    CT_DEV bool get(uint32_t& first, uint32_t& count) {
         for (int tries = 0; tries < kFetchRanges; ++tries) {
             const uint32_t lo = (uint32_t)(((uint64_t)n * (uint32_t)range) / kFetchRanges);
             const uint32_t hi = (uint32_t)(((uint64_t)n * (uint32_t)(range + 1)) / kFetchRanges);
             uint32_t got = 0;
-            if (lane_id() == 0) got = atomicAdd(base + range * kFetchSlot, (uint32_t)CT_FETCH_CHUNK);
+            if (lane_id() == 0) got = atomicAdd(base + range * kFetchSlot, kFetchChunk);
             got = __shfl(got, 0);
             if (got < hi - lo) {
-                next = lo + got;
-                end = lo + min(got + (uint32_t)CT_FETCH_CHUNK, hi - lo);
+                first = lo + got;
+                count = min(kFetchChunk, hi - lo - got);
                 return true;
             }
-            if (!CT_FETCH_STEAL) return false;
+            if (!steal) return false;
             range = (range + 1) % kFetchRanges;
         }
         return false;
-    }
-    CT_DEV bool get(uint32_t& first, uint32_t& count) {
-        if (next >= end && !claim()) return false;
-        first = next;
-        count = min(64u, end - next);
-        next += count;
-        return true;
     }
 };
 
@@ -305,7 +297,7 @@ CT_DEV void trace_persistent(const DevScene& sc, float eps, uint32_t n, uint32_t
     f3 o = mk3(0, 0, 0), d = mk3(0, 0, 0), inv = mk3(0, 0, 0);
     float t = 0.0f;
     int prim = -1;
-    Fetcher fe{fetch, n, (int)(blockIdx.x % kFetchRanges), 0u, 0u};
+    Fetcher fe{fetch, n, (int)(blockIdx.x % kFetchRanges), gridDim.x < (uint32_t)kFetchRanges};
     while (true) {
         // ---- refill the wave once every lane is idle (wave-uniform)
         const uint64_t idle = __ballot(!alive);
