@@ -19,14 +19,16 @@ served (unless --no-serve): render requests on the resident atrium arriving
 as a Poisson stream (resolution 256x256 / 512x512 / 1920x1080 with
 probability .5 / .35 / .15, spp uniform in {1, 4, 16}, depth 8, seeded
 cameras).  p50 / p99 of request latency (completion - arrival) for the render
-server (continuous batching) and for a plain loop that renders requests one
-at a time with ct_trace_frame; images/s of both with all requests queued at
-once.  Every request renders the same path samples either way (checked
-bit-exact on a subset), so the comparison is at equal rel-MSE.
+server (continuous batching) and for the plain per-view loop of the reference
+(one request at a time, one ct_trace_frame call per 1-spp frame, like
+launchKernels); images/s of both with all requests queued at once, and of the
+plain loop with all spp of a request in one call.  Every request renders the
+same path samples either way (checked bit-exact on a subset), so the
+comparison is at equal rel-MSE.
 
 batched: 64 views of the box-and-sphere scene (sides log-uniform in
 [128, 1024], aspect 1:1 or 4:3, spp in {1, 2, 4, 8, 16}, depth 8) in one
-ct_render_batch call vs a plain per-view loop; images/s of both.
+ct_render_batch call after ct_autotune vs the plain per-view loop; images/s.
 """
 from __future__ import annotations
 
@@ -67,6 +69,9 @@ def main() -> int:
     ap.add_argument("--summarize", nargs="+", metavar="DB", help="reduce rocprofv3 result databases and exit")
     ap.add_argument("--block", type=int, default=0, help="threads per block of the traversal kernels (0 = default)")
     ap.add_argument("--stack-lds", type=int, default=0, help="per-lane LDS stack entries (0 = default)")
+    ap.add_argument("--serve-chunk", type=int, default=0, help="paths per server chunk (0 = server default)")
+    ap.add_argument("--sort", type=int, default=-1, choices=[-1, 0, 1],
+                    help="block-local material sort before shading (-1 = the library default)")
     args = ap.parse_args()
     if args.summarize:
         return summarize(args.summarize)
@@ -83,6 +88,8 @@ def main() -> int:
     scene = ct.HostScene.generate(ct.SCENE_C2 if args.config == 2 else ct.SCENE_C3, args.seed)
     opts, W, H = scene.frame_opts()
     opts.traversal = ct.TRAV_FAST if args.traversal == "fast" else ct.TRAV_EXACT
+    if args.sort >= 0:
+        opts.sort_by_material = args.sort
     cam = scene.camera(W, H)
     st_scene = scene.stats()
     ctx = ct.Context(0, block_size=args.block, stack_lds=args.stack_lds)
@@ -157,6 +164,7 @@ def main() -> int:
             "workload": WORKLOADS[args.config] % (st_scene["triangles"], args.traversal),
             "seed": args.seed,
             "width": W, "height": H, "spp": 1, "depth": opts.max_depth, "rr_start": opts.rr_start,
+            "sort_by_material": int(opts.sort_by_material),
             "device": ctx.name(),
         },
         "rays_per_step": {"primary": prim // args.steps, "extension": ext // args.steps, "shadow": sh // args.steps},
@@ -168,7 +176,7 @@ def main() -> int:
     }
 
     if not args.no_serve and args.config == 2:
-        out["served"] = serve_leg(ct, ctx, scene, opts, args.seed)
+        out["served"] = serve_leg(ct, ctx, scene, opts, args.seed, chunk=args.serve_chunk)
         out["batched"] = batched_leg(ct, ctx, args.seed)
     ctx.close()
     if not args.no_cpu:
@@ -181,6 +189,15 @@ def main() -> int:
 def _pct(x, q):
     import numpy as np
     return round(float(np.percentile(np.asarray(x), q)), 3)
+
+
+def _plain_view(ctx, cam, w: int, h: int, spp: int, opts) -> None:
+    """The reference's per-view loop: spp frames of 1 sample, one call each."""
+    _fresh(ctx, w, h)
+    o = opts.copy()
+    o.spp = 1
+    for _ in range(spp):
+        ctx.trace_frame(cam, o, stats=False)
 
 
 def _fresh(ctx, w: int, h: int) -> None:
@@ -209,26 +226,33 @@ def serve_requests(ct, scene, base_opts, seed: int, n: int):
     return reqs
 
 
-def serve_leg(ct, ctx, scene, opts, seed: int, n: int = 120, rate: float = 40.0) -> dict:
+def serve_leg(ct, ctx, scene, opts, seed: int, n: int = 120, rate: float = 40.0, chunk: int = 0) -> dict:
     import numpy as np
     reqs = serve_requests(ct, scene, opts, seed + 1, n)
     arrivals = np.cumsum(np.random.default_rng(seed + 2).exponential(1.0 / rate, n))
-    # plain loop: one request at a time (resize, reset, trace spp samples, read back)
+    # plain per-view loop of the reference (launchKernels: one frame of 1 spp
+    # per call, Kernel.cu:355-372): one request at a time, spp calls, read back
     service, plain_out = [], {}
     for i, (cam, w, h, spp, o) in enumerate(reqs):
         t0 = time.perf_counter()
-        _fresh(ctx, w, h)
-        ctx.trace_frame(cam, o, stats=False)
+        _plain_view(ctx, cam, w, h, spp, o)
         acc, _ = ctx.read_accum()
         service.append(time.perf_counter() - t0)
         if w == 256 and len(plain_out) < 8:
             plain_out[i] = acc
+    # the same with all spp samples of a request in one call
+    t0 = time.perf_counter()
+    for cam, w, h, spp, o in reqs:
+        _fresh(ctx, w, h)
+        ctx.trace_frame(cam, o, stats=False)
+        ctx.read_accum()
+    multi_ips = n / (time.perf_counter() - t0)
     finish, plain_lat = 0.0, []
     for a, sv in zip(arrivals, service):  # FIFO single server on the measured service times
         finish = max(finish, a) + sv
         plain_lat.append((finish - a) * 1e3)
     plain_ips = n / sum(service)
-    srv = ct.Server(ctx)
+    srv = ct.Server(ctx, chunk_paths=chunk)
     # Poisson stream
     ids = []
     t0 = time.perf_counter()
@@ -249,10 +273,12 @@ def serve_leg(ct, ctx, scene, opts, seed: int, n: int = 120, rate: float = 40.0)
     last = max(srv.wait(rid)[0].done_ms for rid in ids)
     srv.close()
     served_ips = n / ((last - c0) * 1e-3)
-    return {"requests": n, "rate_per_s": rate, "p50_ms": _pct(lat, 50), "p99_ms": _pct(lat, 99),
+    return {"requests": n, "rate_per_s": rate, "chunk_paths": chunk or "server default",
+            "p50_ms": _pct(lat, 50), "p99_ms": _pct(lat, 99),
             "plain_p50_ms": _pct(plain_lat, 50), "plain_p99_ms": _pct(plain_lat, 99),
             "images_per_s": round(served_ips, 2), "plain_images_per_s": round(plain_ips, 2),
-            "speedup": round(served_ips / plain_ips, 3), "bit_exact_vs_plain": f"{same}/{len(plain_out)}"}
+            "speedup": round(served_ips / plain_ips, 3), "plain_multi_spp_images_per_s": round(multi_ips, 2),
+            "bit_exact_vs_plain": f"{same}/{len(plain_out)}"}
 
 
 def batched_leg(ct, ctx, seed: int, n_views: int = 64) -> dict:
@@ -271,24 +297,34 @@ def batched_leg(ct, ctx, seed: int, n_views: int = 64) -> dict:
         pos = d * 42.0
         views.append((ct.look_at(pos, rng.uniform(-5, 5, 3), 70.0), w, h, spp, 0))
     ctx.upload(scene)
+    tune = ctx.autotune(views[0][0], opts, 256, 256)
     ctx.render_batch(views[:4], opts)  # warm (graph capture, buffers)
     t0 = time.perf_counter()
     outs, st = ctx.render_batch(views, opts)
     t_batch = time.perf_counter() - t0
-    t_plain, same = 0.0, 0
+    # plain per-view loop of the reference (1 spp per call), then the same
+    # with all spp of a view in one call
+    t_plain, t_multi, same = 0.0, 0.0, 0
     for i, (cam, w, h, spp, fb) in enumerate(views):
+        t0 = time.perf_counter()
+        _plain_view(ctx, cam, w, h, spp, opts)
+        acc, _ = ctx.read_accum()
+        t_plain += time.perf_counter() - t0
+        if i < 8:
+            same += int(np.array_equal(acc.view(np.uint32), outs[i].view(np.uint32)))
         o = opts.copy()
         o.spp = spp
         t0 = time.perf_counter()
         _fresh(ctx, w, h)
         ctx.trace_frame(cam, o, stats=False)
-        acc, _ = ctx.read_accum()
-        t_plain += time.perf_counter() - t0
-        if i < 8:
-            same += int(np.array_equal(acc.view(np.uint32), outs[i].view(np.uint32)))
+        ctx.read_accum()
+        t_multi += time.perf_counter() - t0
     return {"views": n_views, "path_samples": int(sum(w * h * spp for (_, w, h, spp, _) in views)),
             "images_per_s": round(n_views / t_batch, 2), "plain_images_per_s": round(n_views / t_plain, 2),
-            "speedup": round(t_plain / t_batch, 3), "chunks": int(st.chunks), "bit_exact_vs_plain": f"{same}/8"}
+            "speedup": round(t_plain / t_batch, 3), "plain_multi_spp_images_per_s": round(n_views / t_multi, 2),
+            "chunks": int(st.chunks), "autotuned": {"block": tune.block_size, "stack_lds": tune.stack_lds,
+                                                     "chunk_paths": tune.chunk_paths},
+            "bit_exact_vs_plain": f"{same}/8"}
 
 
 def summarize(paths: list) -> int:

@@ -100,6 +100,7 @@ struct ShadeOpts {
     int max_depth, rr_start, bsdf_mode, shadow, indirect;
     float neps, deps, teps;
     int light_pdf;
+    int sort;  // block-local material sort before shading
 };
 
 // Light-selection factor of the MIS light pdf (x count like Kernel.cu:128,184,

@@ -210,17 +210,65 @@ __global__ void k_connect(DevScene sc, TraceOpts to, Queues q, int bounce, int l
 }
 
 // ---------------------------------------------------------------- shade --
+constexpr int kShadeBlock = 256;
+constexpr uint32_t kSortKeys = 16;
+
+// Material key of queue entry i for the shading sort: 0 miss, 1 + material
+// type for hits, kSortKeys - 1 past the end of the queue.
+CT_DEV uint32_t material_key(const DevScene& sc, const Queues& q, uint32_t i, uint32_t n) {
+    if (i >= n) return kSortKeys - 1;
+    const float2 h = q.hit[i];
+    if (!(h.x < 1e34f)) return 0;
+    const uint32_t mi = __float_as_uint(sc.sh_pos[3 * (size_t)__float_as_int(h.y)].w);
+    const int mtype = __float_as_int(sc.mats[2 * (size_t)mi + 1].x);
+    return 1u + (uint32_t)min(max(mtype, 0), (int)kSortKeys - 3);
+}
+
+// Block-local counting sort of the block's kShadeBlock queue entries by
+// material key (LDS histogram, one-wave scan, scatter of indices), so each
+// wave shades mostly one BSDF.  Returns the entry this thread shades.  The
+// order inside a key is arbitrary: a path's radiance and the rays it emits do
+// not depend on which lane shades it.
+CT_DEV uint32_t sorted_entry(const DevScene& sc, const Queues& q, uint32_t first, uint32_t n, uint32_t* hist,
+                             uint32_t* perm) {
+    __syncthreads();  // previous iteration's readers are done
+    if (threadIdx.x < kSortKeys) hist[threadIdx.x] = 0u;
+    __syncthreads();
+    const uint32_t mine = first + threadIdx.x;
+    const uint32_t key = material_key(sc, q, mine, n);
+    const uint32_t rank = atomicAdd(&hist[key], 1u);
+    __syncthreads();
+    if (threadIdx.x < 64) {
+        const uint32_t c = threadIdx.x < kSortKeys ? hist[threadIdx.x] : 0u;
+        uint32_t v = c;
+        for (int d = 1; d < (int)kSortKeys; d <<= 1) {
+            const uint32_t u = __shfl_up(v, d);
+            if ((int)lane_id() >= d) v += u;
+        }
+        if (threadIdx.x < kSortKeys) hist[threadIdx.x] = v - c;
+    }
+    __syncthreads();
+    perm[hist[key] + rank] = mine;
+    __syncthreads();
+    return perm[threadIdx.x];
+}
+
 // shade (Kernel.cu:88-206) for queue entries [0, active[bounce]); see the
-// oracle's path_sample for the same steps on the CPU.
-__global__ void k_shade(DevScene sc, ShadeOpts so, const ViewDesc* __restrict__ views,
-                        const Segment* __restrict__ segs, const ChunkHdr* hdr, Queues q, int sel, int bounce) {
+// oracle's path_sample for the same steps on the CPU.  SORT: shade in
+// block-local material order (ct_frame_opts.sort_by_material).
+template <bool SORT>
+__global__ void __launch_bounds__(kShadeBlock) k_shade(DevScene sc, ShadeOpts so, const ViewDesc* __restrict__ views,
+                                                      const Segment* __restrict__ segs, const ChunkHdr* hdr, Queues q,
+                                                      int sel, int bounce) {
+    __shared__ uint32_t s_hist[kSortKeys];
+    __shared__ uint32_t s_perm[SORT ? kShadeBlock : 1];
     const uint32_t n = *cnt_active(q.counters, bounce);
     const int nseg = (int)hdr->nseg;
     const int nsel = sel ^ 1;
-    const uint32_t stride = gridDim.x * blockDim.x;
-    // wave-uniform loop: all lanes of a wave iterate together
-    for (uint32_t base = blockIdx.x * blockDim.x + (threadIdx.x & ~63u); base < n; base += stride) {
-        const uint32_t i = base + lane_id();
+    const uint32_t stride = gridDim.x * kShadeBlock;
+    // block-uniform loop: all threads of a block iterate together
+    for (uint32_t first = blockIdx.x * kShadeBlock; first < n; first += stride) {
+        const uint32_t i = SORT ? sorted_entry(sc, q, first, n, s_hist, s_perm) : first + threadIdx.x;
         const bool active = i < n;
         bool push_ext = false, push_sh = false;
         f3 e_o = mk3(0, 0, 0), e_d = mk3(0, 0, 0), e_thr = mk3(0, 0, 0);
@@ -535,7 +583,12 @@ hipError_t launch_connect(const LaunchShape& ls, const DevScene& sc, const Trace
 hipError_t launch_shade(const LaunchShape& ls, const DevScene& sc, const ShadeOpts& so, const ViewDesc* views,
                         const Segment* segs, const ChunkHdr* hdr, const Queues& q, int sel, int bounce,
                         hipStream_t s) {
-    hipLaunchKernelGGL(k_shade, dim3(ls.shade_blocks), dim3(256), 0, s, sc, so, views, segs, hdr, q, sel, bounce);
+    if (so.sort)
+        hipLaunchKernelGGL(k_shade<true>, dim3(ls.shade_blocks), dim3(kShadeBlock), 0, s, sc, so, views, segs, hdr, q,
+                           sel, bounce);
+    else
+        hipLaunchKernelGGL(k_shade<false>, dim3(ls.shade_blocks), dim3(kShadeBlock), 0, s, sc, so, views, segs, hdr,
+                           q, sel, bounce);
     return hipGetLastError();
 }
 hipError_t launch_resolve(const LaunchShape& ls, const ViewDesc* views, const Segment* segs, const ChunkHdr* hdr,

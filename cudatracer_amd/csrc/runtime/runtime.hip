@@ -228,6 +228,7 @@ static ctd::ShadeOpts shade_opts(const ct_frame_opts& o) {
     s.deps = o.dist_epsilon;
     s.teps = o.triangle_epsilon;
     s.light_pdf = o.light_pdf;
+    s.sort = o.sort_by_material ? 1 : 0;
     return s;
 }
 
@@ -278,7 +279,7 @@ static std::string graph_key(const ct_ctx* c, const float4* accum, const ct_fram
     std::ostringstream os;
     os << (const void*)accum << ':' << c->cap << ':' << o.max_depth << ':' << o.rr_start << ':' << o.accel << ':'
        << o.bsdf_mode << ':' << o.traversal << ':' << o.shadow << ':' << o.indirect << ':' << o.normal_epsilon << ':'
-       << o.dist_epsilon << ':' << o.triangle_epsilon << ':' << o.light_pdf << ':' << c->shape.block << ':' << c->shape.lds_depth << ':'
+       << o.dist_epsilon << ':' << o.triangle_epsilon << ':' << o.light_pdf << ':' << o.sort_by_material << ':' << c->shape.block << ':' << c->shape.lds_depth << ':'
        << c->shape.trace_blocks << ':' << c->shape.shade_blocks << ':' << (const void*)c->d_views << ':'
        << (const void*)c->d_segs << ':' << (const void*)c->sc.tris << ':' << (const void*)c->sc.mats;
     return os.str();

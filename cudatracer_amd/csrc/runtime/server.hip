@@ -9,11 +9,15 @@
 //     (accumulator + the path-state queue slots they can occupy) fits the
 //     memory budget, so requests that arrive while others are on the GPU join
 //     the next chunk;
-//   - every chunk packs path samples of all admitted requests (oldest first)
-//     into one wavefront: one generate, depth x (extend, shade, connect),
-//     one resolve — replayed from a captured hipGraph;
-//   - a request is finished when its last path sample has been resolved; its
-//     accumulator is copied out and its memory released.
+//   - every chunk packs path samples of the admitted requests into one
+//     wavefront (one generate, depth x (extend, shade, connect), one resolve,
+//     replayed from a captured hipGraph), smallest remaining work first with
+//     aging (remaining paths / (1 + wait / 50 ms)), so a small request that
+//     arrives behind a large one is done in the next chunk and a large one is
+//     not starved;
+//   - a request is finished when its last path sample has been resolved; a
+//     copier thread moves its accumulator to host memory on its own stream
+//     while the next chunk runs, then releases its memory.
 // A request's result does not depend on what it was batched with: the RNG is
 // keyed by (pixel, frame_base + sample, bounce) and samples are summed in
 // sample order.
@@ -34,8 +38,11 @@ using namespace ctr;
 namespace {
 
 constexpr uint64_t kBytesPerPath = 176;  // path-state queue bytes per in-flight path sample
+// Default paths per chunk: small enough that a request arriving while a
+// chunk runs starts within a few milliseconds, large enough to fill the GPU.
+constexpr uint32_t kServerChunk = 1u << 21;
 
-enum ReqState { PENDING = 0, ACTIVE = 1, DONE = 2 };
+enum ReqState { PENDING = 0, ACTIVE = 1, COPYING = 2, DONE = 3 };
 
 struct Req {
     uint64_t id = 0;
@@ -92,8 +99,10 @@ struct ct_server {
     bool have_fopts = false;
     std::mutex mu;
     std::condition_variable cv;
-    std::thread worker;
-    bool stop = false;
+    std::thread worker, copier;
+    bool stop = false, worker_done = false;
+    std::deque<uint64_t> to_copy;  // finished requests awaiting copy-out
+    hipStream_t copy_stream = nullptr;
     uint64_t next_id = 1;
     std::map<uint64_t, Req> reqs;
     std::deque<uint64_t> pending;
@@ -111,7 +120,43 @@ struct ct_server {
         return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     }
     void loop();
+    void copy_loop();
+    void finish(Req& q, int status);
 };
+
+// Caller holds mu.
+void ct_server::finish(Req& q, int status) {
+    arena.release(q.acc_off, (uint64_t)q.r.width * q.r.height);
+    in_use -= q.footprint;
+    q.status = status;
+    q.state = DONE;
+    q.done_ms = now();
+    cv.notify_all();
+}
+
+void ct_server::copy_loop() {
+    (void)hipSetDevice(ctx->device);
+    std::unique_lock<std::mutex> lk(mu);
+    while (true) {
+        if (to_copy.empty()) {
+            if (worker_done) return;
+            cv.wait(lk);
+            continue;
+        }
+        Req& q = reqs[to_copy.front()];
+        to_copy.pop_front();
+        const size_t n = (size_t)q.r.width * q.r.height * 4;
+        q.result.resize(n);
+        const uint64_t off = q.acc_off;
+        float* dst = q.result.data();  // map nodes stay put while unlocked
+        lk.unlock();
+        // the chunk that finished this request was synchronised by the worker
+        hipError_t e = hipMemcpyAsync(dst, arena_ptr + off, n * sizeof(float), hipMemcpyDeviceToHost, copy_stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(copy_stream);
+        lk.lock();
+        finish(q, e == hipSuccess ? CT_OK : CT_ERR_DEVICE);
+    }
+}
 
 void ct_server::loop() {
     (void)hipSetDevice(ctx->device);
@@ -121,6 +166,7 @@ void ct_server::loop() {
     // accumulators to zero before the chunk: (float4 offset, pixels), copied
     // out under the lock (submit() may insert into reqs concurrently)
     std::vector<std::pair<uint64_t, uint64_t>> fresh;
+    std::vector<std::pair<double, uint64_t>> order;  // (priority, id)
     while (true) {
         {
             std::unique_lock<std::mutex> lk(mu);
@@ -128,11 +174,7 @@ void ct_server::loop() {
             for (auto it = active.begin(); it != active.end();) {
                 Req& q = reqs[*it];
                 if (q.cancel) {
-                    arena.release(q.acc_off, (uint64_t)q.r.width * q.r.height);
-                    in_use -= q.footprint;
-                    q.state = DONE;
-                    q.status = CT_ERR_CANCELLED;
-                    q.done_ms = now();
+                    finish(q, CT_ERR_CANCELLED);
                     it = active.erase(it);
                 } else {
                     ++it;
@@ -160,17 +202,30 @@ void ct_server::loop() {
                 cv.notify_all();
             }
             if (active.empty()) {
-                if (stop) return;
+                if (stop) {
+                    worker_done = true;
+                    cv.notify_all();
+                    return;
+                }
                 cv.wait(lk);
                 continue;
             }
-            // pack one chunk, oldest request first
+            // pack one chunk: smallest remaining work first, with aging
             views.clear();
             runs.clear();
             ids.clear();
             fresh.clear();
-            uint64_t room = chunk;
+            order.clear();
+            const double t_now = now();
             for (uint64_t id : active) {
+                const Req& q = reqs[id];
+                const double wait = t_now - q.submit_ms;
+                order.emplace_back((double)(q.total - q.next) / (1.0 + wait / 50.0), id);
+            }
+            std::sort(order.begin(), order.end());
+            uint64_t room = chunk;
+            for (const auto& oi : order) {
+                const uint64_t id = oi.second;
                 if (room == 0) break;
                 Req& q = reqs[id];
                 const uint64_t take = std::min<uint64_t>(room, q.total - q.next);
@@ -195,14 +250,8 @@ void ct_server::loop() {
         if (rc != CT_OK) {
             fatal = rc;
             fatal_msg = ctx->err;
-            for (uint64_t id : active) {
-                Req& q = reqs[id];
-                q.state = DONE;
-                q.status = rc;
-                q.done_ms = now();
-            }
+            for (uint64_t id : active) finish(reqs[id], rc);
             active.clear();
-            cv.notify_all();
             continue;
         }
         // rays are attributed to requests in proportion to their path samples
@@ -214,20 +263,8 @@ void ct_server::loop() {
         for (auto it = active.begin(); it != active.end();) {
             Req& q = reqs[*it];
             if (q.next >= q.total) {
-                const size_t n = (size_t)q.r.width * q.r.height * 4;
-                q.result.resize(n);
-                const uint64_t off = q.acc_off;
-                float* dst = q.result.data();  // map nodes stay put while unlocked
-                lk.unlock();
-                hipError_t e = hipMemcpyAsync(dst, arena_ptr + off, n * sizeof(float), hipMemcpyDeviceToHost,
-                                              ctx->stream);
-                if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
-                lk.lock();
-                q.status = e == hipSuccess ? CT_OK : CT_ERR_DEVICE;
-                q.state = DONE;
-                q.done_ms = now();
-                arena.release(q.acc_off, (uint64_t)q.r.width * q.r.height);
-                in_use -= q.footprint;
+                q.state = COPYING;
+                to_copy.push_back(q.id);
                 it = active.erase(it);
                 cv.notify_all();
             } else {
@@ -247,7 +284,7 @@ int ct_server_create(ct_ctx* ctx, const ct_server_opts* opts, ct_server** out) {
     s->ctx = ctx;
     if (opts) s->opts = *opts;
     if (s->opts.mem_budget == 0) s->opts.mem_budget = 8ull << 30;
-    s->chunk = s->opts.chunk_paths > 0 ? (uint32_t)s->opts.chunk_paths : ctx->chunk_default;
+    s->chunk = s->opts.chunk_paths > 0 ? (uint32_t)s->opts.chunk_paths : kServerChunk;
     (void)hipSetDevice(ctx->device);
     int r = ensure_paths(ctx, s->chunk);
     if (r == CT_OK) {
@@ -258,9 +295,17 @@ int ct_server_create(ct_ctx* ctx, const ct_server_opts* opts, ct_server** out) {
         delete s;
         return r;
     }
+    if (r == CT_OK && hipStreamCreate(&s->copy_stream) != hipSuccess)
+        r = fail(ctx, CT_ERR_DEVICE, "copy stream");
+    if (r != CT_OK) {
+        dfree(ctx, s->arena_ptr);
+        delete s;
+        return r;
+    }
     s->arena = Arena(s->arena_len);
     s->t0 = std::chrono::steady_clock::now();
     s->worker = std::thread([s] { s->loop(); });
+    s->copier = std::thread([s] { s->copy_loop(); });
     *out = s;
     return CT_OK;
 }
@@ -362,7 +407,9 @@ void ct_server_destroy(ct_server* s) {
         s->cv.notify_all();
     }
     if (s->worker.joinable()) s->worker.join();
+    if (s->copier.joinable()) s->copier.join();
     (void)hipStreamSynchronize(s->ctx->stream);
+    if (s->copy_stream) (void)hipStreamDestroy(s->copy_stream);
     dfree(s->ctx, s->arena_ptr);
     delete s;
 }

@@ -234,3 +234,25 @@ def test_batch_chunked_graph_replay(ct, orc, gpu_ctx, c1):
     assert np.array_equal(outs_small[0], outs_big[0])
     ref, _ = osc.render(cam, opts, 128, 128, 0, 8, threads=16)
     assert np.array_equal(outs_big[0], ref)
+
+
+@pytest.mark.parametrize("config", [1, 2])
+def test_material_sort_same_image(ct, orc, gpu_ctx, config):
+    """Shading in block-local material order (sort_by_material) changes which
+    lane shades a path, not the path: accumulators are bit-identical."""
+    from conftest import get_scene
+    scene, arrays, osc = get_scene(ct, orc, config)
+    _upload(gpu_ctx, scene)
+    opts, w, h = scene.frame_opts()
+    w, h = min(w, 480), min(h, 270)
+    cam = scene.camera(w, h)
+    gpu_ctx.resize(w, h)
+    out = []
+    for sort in (0, 1):
+        o = opts.copy()
+        o.sort_by_material = sort
+        o.spp = 2
+        gpu_ctx.reset()
+        gpu_ctx.trace_frame(cam, o)
+        out.append(gpu_ctx.read_accum()[0])
+    assert np.array_equal(out[0].view(np.uint32), out[1].view(np.uint32))
