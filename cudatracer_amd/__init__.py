@@ -185,7 +185,8 @@ _sig = {
     "ct_host_scene_frame_opts": (C.c_int, [_P, C.POINTER(FrameOpts), C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
     "ct_host_scene_stats": (C.c_int, [_P, C.POINTER(C.c_uint64)]),
     "ct_camera_look_at": (None, [C.POINTER(C.c_float), C.POINTER(C.c_float), C.c_float, C.POINTER(Camera)]),
-    "ct_compress_quad_nodes": (C.c_int, [_P, C.c_uint32, C.POINTER(C.c_uint32)]),
+    "ct_build_wide_nodes": (C.c_int, [C.POINTER(SceneView), C.POINTER(C.c_uint32), C.c_uint32, C.POINTER(C.c_uint32),
+                                      C.POINTER(C.c_uint32)]),
 }
 for _name, (_res, _args) in _sig.items():
     _f = getattr(lib, _name)
@@ -195,15 +196,16 @@ for _name, (_res, _args) in _sig.items():
 EXPORTED = tuple(_sig)
 
 
-def compress_quad_nodes(mbvh: np.ndarray) -> np.ndarray:
-    """The fast traversal's 64-byte nodes for a quad tree given as raw
-    ct_mbvh_node bytes; returns uint32[n, 16]."""
-    raw = np.ascontiguousarray(mbvh, np.uint8)
-    n = raw.size // 128
-    out = np.zeros((n, 16), np.uint32)
-    _check(lib.ct_compress_quad_nodes(raw.ctypes.data_as(C.c_void_p), n, out.ctypes.data_as(C.POINTER(C.c_uint32))),
-           None)
-    return out
+def build_wide_nodes(scene: "HostScene"):
+    """The fast traversal's wide tree for a host scene: (uint32[n, 32] node
+    words, worst-case stack entries)."""
+    v = scene.view()
+    n, bound = C.c_uint32(), C.c_uint32()
+    _check(lib.ct_build_wide_nodes(C.byref(v), None, 0, C.byref(n), C.byref(bound)), None)
+    out = np.zeros((n.value, 32), np.uint32)
+    _check(lib.ct_build_wide_nodes(C.byref(v), out.ctypes.data_as(C.POINTER(C.c_uint32)), n.value, C.byref(n),
+                                   C.byref(bound)), None)
+    return out, bound.value
 
 
 def _fp(a: np.ndarray, ct=C.c_float):

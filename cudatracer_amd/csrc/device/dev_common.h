@@ -53,7 +53,7 @@ struct DevScene {
     const float4* tris;     // 3 per triangle (leaf order): {p0, id}, {p1-p0}, {p2-p0}
     const float4* nodes4;   // 8 per quad node: bminx,bmaxx,bminy,bmaxy,bminz,bmaxz,childs,counts
     const float4* nodes2;   // 2 per binary node: {min, leftFirst}, {max, count}
-    const uint4* fnodes;    // 4 per compressed quad node (64 B), layout in host/cnode.h
+    const uint4* fnodes;    // 8 per wide compressed node (128 B), layout in host/cnode.h
     const float4* sh_pos;   // 3 per primitive: v0 (w = material index bits), v1, v2
     const float4* sh_nrm;   // 4 per primitive: n0, n1, n2, centre normal
     const float4* mats;     // 2 per material: {albedo, roughness}, {type bits, ior, alpha, 0}

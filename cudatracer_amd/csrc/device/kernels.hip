@@ -126,10 +126,6 @@ CT_DEV void extend_impl(const DevScene& sc, const TraceOpts& to, const Queues& q
 // GFX9 buffer resource word 3 (raw 32-bit access).
 constexpr int kBufferWord3 = 0x00020000;
 
-// Tree the traversal kernels walk: 2 binary, 4 quad (exact), 5 compressed
-// quad (fast mode; the binary tree has no compressed form).
-CT_DEV int tree_kind(const TraceOpts& to) { return to.accel == 0 ? 2 : (to.traversal == 1 ? 5 : 4); }
-
 CT_DEV LaneStack make_stack(const Queues& q, int lds_depth) {
     LaneStack st;
     st.base = threadIdx.x;
@@ -144,21 +140,15 @@ CT_DEV LaneStack make_stack(const Queues& q, int lds_depth) {
     return st;
 }
 
+// One kernel per tree and counting mode, so each gets its own register
+// budget (the wide tree's step needs more than the exact ones).
+template <int TREE, bool COUNT>
 __global__ void k_extend(DevScene sc, TraceOpts to, Queues q, int sel, int bounce, int lds_depth,
                          unsigned long long* wstats) {
     LaneStack st = make_stack(q, lds_depth);
     TravCount tc{0u, 0u};
-    const int tree = tree_kind(to);
-    if (wstats) {
-        if (tree == 2) extend_impl<2, true>(sc, to, q, sel, bounce, st, tc);
-        else if (tree == 5) extend_impl<5, true>(sc, to, q, sel, bounce, st, tc);
-        else extend_impl<4, true>(sc, to, q, sel, bounce, st, tc);
-    } else {
-        if (tree == 2) extend_impl<2, false>(sc, to, q, sel, bounce, st, tc);
-        else if (tree == 5) extend_impl<5, false>(sc, to, q, sel, bounce, st, tc);
-        else extend_impl<4, false>(sc, to, q, sel, bounce, st, tc);
-    }
-    if (wstats) {
+    extend_impl<TREE, COUNT>(sc, to, q, sel, bounce, st, tc);
+    if (COUNT) {
         atomicAdd(&wstats[0], (unsigned long long)tc.nodes);
         atomicAdd(&wstats[1], (unsigned long long)tc.tris);
     }
@@ -189,21 +179,13 @@ CT_DEV void connect_impl(const DevScene& sc, const TraceOpts& to, const Queues& 
         });
 }
 
+template <int TREE, bool COUNT>
 __global__ void k_connect(DevScene sc, TraceOpts to, Queues q, int bounce, int lds_depth,
                           unsigned long long* wstats) {
     LaneStack st = make_stack(q, lds_depth);
     TravCount tc{0u, 0u};
-    const int tree = tree_kind(to);
-    if (wstats) {
-        if (tree == 2) connect_impl<2, true>(sc, to, q, bounce, st, tc);
-        else if (tree == 5) connect_impl<5, true>(sc, to, q, bounce, st, tc);
-        else connect_impl<4, true>(sc, to, q, bounce, st, tc);
-    } else {
-        if (tree == 2) connect_impl<2, false>(sc, to, q, bounce, st, tc);
-        else if (tree == 5) connect_impl<5, false>(sc, to, q, bounce, st, tc);
-        else connect_impl<4, false>(sc, to, q, bounce, st, tc);
-    }
-    if (wstats) {
+    connect_impl<TREE, COUNT>(sc, to, q, bounce, st, tc);
+    if (COUNT) {
         atomicAdd(&wstats[2], (unsigned long long)tc.nodes);
         atomicAdd(&wstats[3], (unsigned long long)tc.tris);
     }
@@ -541,12 +523,10 @@ CT_DEV void flags_impl(const DevScene& sc, const TraceOpts& to, const Queues& q,
         [&](uint32_t i, float, int prim) { out[i] = prim >= 0 ? 1 : 0; });
 }
 
+template <int TREE>
 __global__ void k_shadow_flags(DevScene sc, TraceOpts to, Queues q, uint32_t n, int lds_depth, uint8_t* out) {
     LaneStack st = make_stack(q, lds_depth);
-    const int tree = tree_kind(to);
-    if (tree == 2) flags_impl<2>(sc, to, q, n, st, out);
-    else if (tree == 5) flags_impl<5>(sc, to, q, n, st, out);
-    else flags_impl<4>(sc, to, q, n, st, out);
+    flags_impl<TREE>(sc, to, q, n, st, out);
 }
 
 }  // namespace ctd
@@ -566,18 +546,46 @@ hipError_t launch_generate(const LaunchShape& ls, const ViewDesc* views, const S
     hipLaunchKernelGGL(k_generate, dim3(blocks), dim3(256), 0, s, views, segs, hdr, q);
     return hipGetLastError();
 }
+int tree_of(const TraceOpts& to) { return to.traversal == CT_TRAV_FAST ? 8 : (to.accel == CT_ACCEL_BVH2 ? 2 : 4); }
+int grid_of(const LaunchShape& ls, int tree) { return tree == 2 ? ls.grid2 : (tree == 4 ? ls.grid4 : ls.grid8); }
+
+template <int TREE>
+void extend_t(const LaunchShape& ls, const DevScene& sc, const TraceOpts& to, const Queues& q, int sel, int bounce,
+              unsigned long long* wstats, hipStream_t s) {
+    const size_t lds = (size_t)ls.lds_depth * ls.block * 4;
+    const dim3 grid(grid_of(ls, TREE)), block(ls.block);
+    if (wstats)
+        hipLaunchKernelGGL((k_extend<TREE, true>), grid, block, lds, s, sc, to, q, sel, bounce, ls.lds_depth, wstats);
+    else
+        hipLaunchKernelGGL((k_extend<TREE, false>), grid, block, lds, s, sc, to, q, sel, bounce, ls.lds_depth, wstats);
+}
+template <int TREE>
+void connect_t(const LaunchShape& ls, const DevScene& sc, const TraceOpts& to, const Queues& q, int bounce,
+               unsigned long long* wstats, hipStream_t s) {
+    const size_t lds = (size_t)ls.lds_depth * ls.block * 4;
+    const dim3 grid(grid_of(ls, TREE)), block(ls.block);
+    if (wstats)
+        hipLaunchKernelGGL((k_connect<TREE, true>), grid, block, lds, s, sc, to, q, bounce, ls.lds_depth, wstats);
+    else
+        hipLaunchKernelGGL((k_connect<TREE, false>), grid, block, lds, s, sc, to, q, bounce, ls.lds_depth, wstats);
+}
+
 hipError_t launch_extend(const LaunchShape& ls, const DevScene& sc, const TraceOpts& to, const Queues& q, int sel,
                          int bounce, unsigned long long* wstats, hipStream_t s) {
-    const size_t lds = (size_t)ls.lds_depth * ls.block * 4;
-    hipLaunchKernelGGL(k_extend, dim3(ls.trace_blocks), dim3(ls.block), lds, s, sc, to, q, sel, bounce,
-                       ls.lds_depth, wstats);
+    switch (tree_of(to)) {
+        case 2: extend_t<2>(ls, sc, to, q, sel, bounce, wstats, s); break;
+        case 4: extend_t<4>(ls, sc, to, q, sel, bounce, wstats, s); break;
+        default: extend_t<8>(ls, sc, to, q, sel, bounce, wstats, s); break;
+    }
     return hipGetLastError();
 }
 hipError_t launch_connect(const LaunchShape& ls, const DevScene& sc, const TraceOpts& to, const Queues& q,
                           int bounce, unsigned long long* wstats, hipStream_t s) {
-    const size_t lds = (size_t)ls.lds_depth * ls.block * 4;
-    hipLaunchKernelGGL(k_connect, dim3(ls.trace_blocks), dim3(ls.block), lds, s, sc, to, q, bounce, ls.lds_depth,
-                       wstats);
+    switch (tree_of(to)) {
+        case 2: connect_t<2>(ls, sc, to, q, bounce, wstats, s); break;
+        case 4: connect_t<4>(ls, sc, to, q, bounce, wstats, s); break;
+        default: connect_t<8>(ls, sc, to, q, bounce, wstats, s); break;
+    }
     return hipGetLastError();
 }
 hipError_t launch_shade(const LaunchShape& ls, const DevScene& sc, const ShadeOpts& so, const ViewDesc* views,
@@ -607,25 +615,42 @@ hipError_t launch_tonemap(const float4* accum, uint32_t n, float samples, float4
 hipError_t launch_shadow_flags(const LaunchShape& ls, const DevScene& sc, const TraceOpts& to, const Queues& q,
                                uint32_t n, uint8_t* out, hipStream_t s) {
     const size_t lds = (size_t)ls.lds_depth * ls.block * 4;
-    hipLaunchKernelGGL(k_shadow_flags, dim3(ls.trace_blocks), dim3(ls.block), lds, s, sc, to, q, n, ls.lds_depth,
-                       out);
+    const int tree = tree_of(to);
+    const dim3 grid(grid_of(ls, tree)), block(ls.block);
+    if (tree == 2) hipLaunchKernelGGL(k_shadow_flags<2>, grid, block, lds, s, sc, to, q, n, ls.lds_depth, out);
+    else if (tree == 4) hipLaunchKernelGGL(k_shadow_flags<4>, grid, block, lds, s, sc, to, q, n, ls.lds_depth, out);
+    else hipLaunchKernelGGL(k_shadow_flags<8>, grid, block, lds, s, sc, to, q, n, ls.lds_depth, out);
     return hipGetLastError();
 }
 
 }  // namespace ctk
 
 namespace ctk {
-// Resident blocks per CU of the persistent traversal kernels for a launch shape.
-int trace_blocks_per_cu(int block, int lds_depth) {
-    int nb = 0;
-    const size_t lds = (size_t)lds_depth * block * 4;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, reinterpret_cast<const void*>(&ctd::k_extend), block,
-                                                     lds) != hipSuccess)
+// Resident blocks per CU of the persistent traversal kernels of one tree
+// (extend and connect, the smaller of the two) for a block size and LDS depth.
+template <int TREE>
+static int per_cu(int block, size_t lds) {
+    int nb = 0, nc = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, reinterpret_cast<const void*>(&ctd::k_extend<TREE, false>),
+                                                     block, lds) != hipSuccess)
         return 0;
-    int nc = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nc, reinterpret_cast<const void*>(&ctd::k_connect), block,
-                                                     lds) != hipSuccess)
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nc, reinterpret_cast<const void*>(&ctd::k_connect<TREE, false>),
+                                                     block, lds) != hipSuccess)
         return 0;
     return nb < nc ? nb : nc;
+}
+
+bool set_trace_grids(LaunchShape& ls, int num_cus) {
+    const size_t lds = (size_t)ls.lds_depth * ls.block * 4;
+    const int p2 = per_cu<2>(ls.block, lds), p4 = per_cu<4>(ls.block, lds), p8 = per_cu<8>(ls.block, lds);
+    if (p2 <= 0 || p4 <= 0 || p8 <= 0) return false;
+    // at least one block per fetch range (the persistent kernels drain ranges
+    // by block id)
+    const int lo = ctd::kFetchRanges;
+    ls.grid2 = std::max(lo, num_cus * p2);
+    ls.grid4 = std::max(lo, num_cus * p4);
+    ls.grid8 = std::max(lo, num_cus * p8);
+    ls.trace_blocks = std::max(ls.grid2, std::max(ls.grid4, ls.grid8));
+    return true;
 }
 }  // namespace ctk

@@ -12,13 +12,16 @@ namespace ctk {
 struct LaunchShape {
     int block = 256;          // threads per block for extend / connect
     int lds_depth = 16;       // per-lane LDS stack entries
-    int trace_blocks = 1024;  // persistent grid of extend / connect
+    int trace_blocks = 1024;  // largest persistent grid (sizes the stack spill area)
+    int grid2 = 1024, grid4 = 1024, grid8 = 1024;  // persistent grids per tree
     int shade_blocks = 1024;  // grid-stride grid of shade
     int max_blocks = 65536;   // cap for the simple grid-stride kernels
 };
 
-// Resident blocks per CU of extend/connect (0 on error).
-int trace_blocks_per_cu(int block, int lds_depth);
+// Persistent grids of the traversal kernels of each tree for ls.block and
+// ls.lds_depth (resident blocks per CU x CUs); false if a kernel cannot run
+// with that shape.
+bool set_trace_grids(LaunchShape& ls, int num_cus);
 
 hipError_t launch_setup(uint32_t* counters, const ctd::ChunkHdr* hdr, hipStream_t s);
 hipError_t launch_generate(const LaunchShape& ls, const ctd::ViewDesc* views, const ctd::Segment* segs,

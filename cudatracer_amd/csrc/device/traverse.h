@@ -177,44 +177,78 @@ CT_DEV void node4_step(const float4* __restrict__ nodes4, uint32_t e, f3 o, f3 i
     push_children<SORT>(hm, m0, m1, m2, m3, e0, e1, e2, e3, st);
 }
 
-// Compressed quad node (64 B, layout in host/cnode.h): child planes decoded
-// to world space as fmaf(q, 2^k, p), then the same slab arithmetic, tagging
-// and push order as node4_step.  The decoded boxes contain the exact ones.
+// Wide compressed node (128 B, up to 8 children, layout in host/cnode.h):
+// child planes decoded to world space as fmaf(q, 2^k, p), then the slab
+// arithmetic of node4_step.  Keys are tmin with the slot in the low 3 bits
+// (misses get a huge key); the optimal 19-comparator network sorts them and
+// the hit children are pushed farthest first.  SORT = false pushes in slot
+// order (any-hit).
+CT_DEV float wide_key(bool hit, float tmin, uint32_t slot) {
+    return hit ? __uint_as_float((fbits(tmin) & ~7u) | slot) : __uint_as_float(0x7F7FFFF8u | slot);
+}
+CT_DEV void cswap(float& a, float& b) {
+    const float lo = gmin(a, b), hi = gmax(a, b);
+    a = lo;
+    b = hi;
+}
+
 template <bool SORT>
-CT_DEV void fnode4_step(const uint4* __restrict__ fnodes, uint32_t e, f3 o, f3 inv, float t, LaneStack& st) {
-    const uint4* n = fnodes + 4 * (size_t)e;
-    const uint4 h = n[0], qx = n[1], qy = n[2], ce = n[3];
+CT_DEV void fnode8_step(const uint4* __restrict__ fnodes, uint32_t e, f3 o, f3 inv, float t, LaneStack& st) {
+    const uint4* n = fnodes + 8 * (size_t)e;
+    const uint4 h = n[0], qx = n[1], qy = n[2], qz = n[3], ca = n[4], cb = n[5];
     const float px = __uint_as_float(h.x), py = __uint_as_float(h.y), pz = __uint_as_float(h.z);
     const float sx = __uint_as_float((h.w & 0xFFu) << 23), sy = __uint_as_float(((h.w >> 8) & 0xFFu) << 23),
                 sz = __uint_as_float(((h.w >> 16) & 0xFFu) << 23);
     const uint32_t valid = h.w >> 24;
-    float m0, m1, m2, m3, x0, x1, x2, x3;
-#define CT_QB(w, i) ((float)(((w) >> (8 * (i))) & 0xFFu))
-#define CT_FSLAB(MN, MX, i)                                                                        \
-    {                                                                                              \
-        float t1 = (fmaf(CT_QB(qx.x, i), sx, px) - o.x) * inv.x;                                    \
-        float t2 = (fmaf(CT_QB(qx.y, i), sx, px) - o.x) * inv.x;                                    \
-        float mn = gmin(t1, t2), mx = gmax(t1, t2);                                                \
-        t1 = (fmaf(CT_QB(qx.z, i), sy, py) - o.y) * inv.y;                                          \
-        t2 = (fmaf(CT_QB(qx.w, i), sy, py) - o.y) * inv.y;                                          \
-        mn = gmax(mn, gmin(t1, t2));                                                               \
-        mx = gmin(mx, gmax(t1, t2));                                                               \
-        t1 = (fmaf(CT_QB(qy.x, i), sz, pz) - o.z) * inv.z;                                          \
-        t2 = (fmaf(CT_QB(qy.y, i), sz, pz) - o.z) * inv.z;                                          \
-        mn = gmax(mn, gmin(t1, t2));                                                               \
-        mx = gmin(mx, gmax(t1, t2));                                                               \
-        MN = tag(mn, i);                                                                           \
-        MX = mx;                                                                                   \
+    float key[8];
+    uint32_t hm = 0;
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+        const uint32_t sh = 8u * (uint32_t)(i & 3);
+        const uint32_t lxw = i < 4 ? qx.x : qx.y, hxw = i < 4 ? qx.z : qx.w;
+        const uint32_t lyw = i < 4 ? qy.x : qy.y, hyw = i < 4 ? qy.z : qy.w;
+        const uint32_t lzw = i < 4 ? qz.x : qz.y, hzw = i < 4 ? qz.z : qz.w;
+        float t1 = (fmaf((float)((lxw >> sh) & 0xFFu), sx, px) - o.x) * inv.x;
+        float t2 = (fmaf((float)((hxw >> sh) & 0xFFu), sx, px) - o.x) * inv.x;
+        float mn = gmin(t1, t2), mx = gmax(t1, t2);
+        t1 = (fmaf((float)((lyw >> sh) & 0xFFu), sy, py) - o.y) * inv.y;
+        t2 = (fmaf((float)((hyw >> sh) & 0xFFu), sy, py) - o.y) * inv.y;
+        mn = gmax(mn, gmin(t1, t2));
+        mx = gmin(mx, gmax(t1, t2));
+        t1 = (fmaf((float)((lzw >> sh) & 0xFFu), sz, pz) - o.z) * inv.z;
+        t2 = (fmaf((float)((hzw >> sh) & 0xFFu), sz, pz) - o.z) * inv.z;
+        mn = gmax(mn, gmin(t1, t2));
+        mx = gmin(mx, gmax(t1, t2));
+        const bool hit = ((valid >> i) & 1u) && mx > 0.0f && mn <= mx && mn < t;
+        hm |= hit ? (1u << i) : 0u;
+        key[i] = wide_key(hit, mn, (uint32_t)i);
     }
-    CT_FSLAB(m0, x0, 0u)
-    CT_FSLAB(m1, x1, 1u)
-    CT_FSLAB(m2, x2, 2u)
-    CT_FSLAB(m3, x3, 3u)
-#undef CT_FSLAB
-#undef CT_QB
-    const uint32_t hm = ((x0 > 0.0f && m0 <= x0 && m0 < t) ? 1u : 0u) | ((x1 > 0.0f && m1 <= x1 && m1 < t) ? 2u : 0u) |
-                        ((x2 > 0.0f && m2 <= x2 && m2 < t) ? 4u : 0u) | ((x3 > 0.0f && m3 <= x3 && m3 < t) ? 8u : 0u);
-    push_children<SORT>(hm & valid, m0, m1, m2, m3, qy.z, qy.w, ce.x, ce.y, st);
+    if (hm == 0u) return;
+    const uint32_t ent[8] = {ca.x, ca.y, ca.z, ca.w, cb.x, cb.y, cb.z, cb.w};
+    auto entry = [&](uint32_t slot) {
+        const uint32_t a0 = (slot & 1u) ? ent[1] : ent[0], a1 = (slot & 1u) ? ent[3] : ent[2];
+        const uint32_t a2 = (slot & 1u) ? ent[5] : ent[4], a3 = (slot & 1u) ? ent[7] : ent[6];
+        const uint32_t b0 = (slot & 2u) ? a1 : a0, b1 = (slot & 2u) ? a3 : a2;
+        return (slot & 4u) ? b1 : b0;
+    };
+    if (!SORT) {
+        st.push4(ent[0], hm & 1u, ent[1], (hm >> 1) & 1u, ent[2], (hm >> 2) & 1u, ent[3], (hm >> 3) & 1u);
+        st.push4(ent[4], (hm >> 4) & 1u, ent[5], (hm >> 5) & 1u, ent[6], (hm >> 6) & 1u, ent[7], (hm >> 7) & 1u);
+        return;
+    }
+    cswap(key[0], key[2]); cswap(key[1], key[3]); cswap(key[4], key[6]); cswap(key[5], key[7]);
+    cswap(key[0], key[4]); cswap(key[1], key[5]); cswap(key[2], key[6]); cswap(key[3], key[7]);
+    cswap(key[0], key[1]); cswap(key[2], key[3]); cswap(key[4], key[5]); cswap(key[6], key[7]);
+    cswap(key[2], key[4]); cswap(key[3], key[5]);
+    cswap(key[1], key[4]); cswap(key[3], key[6]);
+    cswap(key[1], key[2]); cswap(key[3], key[4]); cswap(key[5], key[6]);
+    // hits sort before misses: push sorted positions h-1 .. 0 (farthest first)
+    const int nh = __popc(hm);
+    auto slot_of = [&](int j) { return fbits(key[j]) & 7u; };
+    st.push4(entry(slot_of(7)), nh > 7, entry(slot_of(6)), nh > 6, entry(slot_of(5)), nh > 5, entry(slot_of(4)),
+             nh > 4);
+    st.push4(entry(slot_of(3)), nh > 3, entry(slot_of(2)), nh > 2, entry(slot_of(1)), nh > 1, entry(slot_of(0)),
+             nh > 0);
 }
 
 // BVHNode::Intersect (BVHNode.cuh:50-62): no culling against the current t.
@@ -244,7 +278,7 @@ CT_DEV void node2_step(const float4* __restrict__ nodes2, uint32_t lf, f3 o, f3 
 
 // ----------------------------------------------------------------------
 // Persistent traversal driver.  Source: how ray i is loaded; Sink: what is
-// done with the result.  ACCEL 4 = quad tree, 5 = compressed quad tree,
+// done with the result.  ACCEL 4 = quad tree, 8 = wide compressed tree,
 // 2 = binary tree.
 struct RayIn {
     f3 o, d;
@@ -330,13 +364,13 @@ CT_DEV void trace_persistent(const DevScene& sc, float eps, uint32_t n, uint32_t
         // ---- one step: pop (+ node test) and/or one triangle
         if (k == kNoLeaf) {
             const uint32_t e = st.top_pop();
-            if (ACCEL == 4 || ACCEL == 5) {
+            if (ACCEL == 4 || ACCEL == 8) {
                 if (e & kLeafBit) {
                     k = e & ~kLeafBit;
                 } else {
                     if (COUNT) ++cnt.nodes;
                     if (ACCEL == 4) node4_step<!AnyHit>(sc.nodes4, e, o, inv, t, st);
-                    else fnode4_step<!AnyHit>(sc.fnodes, e, o, inv, t, st);
+                    else fnode8_step<!AnyHit>(sc.fnodes, e, o, inv, t, st);
                 }
             } else {
                 const float4 lo = sc.nodes2[2 * (size_t)e];

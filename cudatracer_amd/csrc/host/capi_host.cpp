@@ -22,9 +22,19 @@ struct ct_host_scene {
 
 extern "C" {
 
-int ct_compress_quad_nodes(const ct_mbvh_node* nodes, uint32_t n, uint32_t* out_words) {
-    if ((!nodes || !out_words) && n) return CT_ERR_INVALID_ARG;
-    return ct::compress_quad_nodes(nodes, n, out_words) ? CT_ERR_INVALID_SCENE : CT_OK;
+int ct_build_wide_nodes(const ct_scene_view* v, uint32_t* out_words, uint32_t capacity_nodes, uint32_t* n_nodes,
+                        uint32_t* stack_bound) {
+    if (!v || !n_nodes) return CT_ERR_INVALID_ARG;
+    std::vector<uint32_t> w;
+    uint32_t bound = 0;
+    if (ct::build_wide_nodes(*v, w, bound)) return CT_ERR_INVALID_SCENE;
+    const uint32_t nn = (uint32_t)(w.size() / ct::kWideNodeWords);
+    *n_nodes = nn;
+    if (stack_bound) *stack_bound = bound;
+    if (!out_words) return CT_OK;
+    if (capacity_nodes < nn) return CT_ERR_INVALID_ARG;
+    std::memcpy(out_words, w.data(), w.size() * 4);
+    return CT_OK;
 }
 
 int ct_host_scene_create(ct_host_scene** out) {

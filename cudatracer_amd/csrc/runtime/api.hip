@@ -228,9 +228,9 @@ int ct_scene_upload(ct_ctx* c, const ct_scene_view* v) {
     for (uint32_t l = 0; l < v->n_lights; ++l) light_rows(v, v->light_indices[l], v->materials, &lights[5 * (size_t)l]);
     std::vector<ct_mbvh_node> n4(v->mbvh_nodes, v->mbvh_nodes + v->n_mbvh_nodes);
     std::vector<ct_bvh_node> n2(v->bvh_nodes, v->bvh_nodes + v->n_bvh_nodes);
-    std::vector<uint32_t> fn((size_t)ct::kCompressedNodeWords * n4.size());
-    if (!n4.empty() && ct::compress_quad_nodes(n4.data(), (uint32_t)n4.size(), fn.data()))
-        return fail(c, CT_ERR_INVALID_SCENE, "quad node bounds are not finite");
+    std::vector<uint32_t> fn;
+    uint32_t wide_bound = 0;
+    if (ct::build_wide_nodes(*v, fn, wide_bound)) return fail(c, CT_ERR_INVALID_SCENE, "tree bounds are not finite");
 
     HIPCHK(hipStreamSynchronize(c->stream), "scene upload");
     free_scene(c);
@@ -241,9 +241,9 @@ int ct_scene_upload(ct_ctx* c, const ct_scene_view* v) {
     if (r == CT_OK) r = upload_vec(c, mats, &sc.mats, "materials");
     if (r == CT_OK) r = upload_vec(c, lights, &sc.lights, "lights");
     if (r == CT_OK && !n4.empty()) r = upload_vec(c, n4, &sc.nodes4, "quad nodes");
-    if (r == CT_OK && !n4.empty()) {
+    if (r == CT_OK) {
         const float4* fp = nullptr;
-        r = upload_vec(c, fn, &fp, "compressed quad nodes");
+        r = upload_vec(c, fn, &fp, "wide nodes");
         sc.fnodes = reinterpret_cast<const uint4*>(fp);
     }
     if (r == CT_OK && !n2.empty()) r = upload_vec(c, n2, &sc.nodes2, "binary nodes");
@@ -252,7 +252,7 @@ int ct_scene_upload(ct_ctx* c, const ct_scene_view* v) {
     sc.n_tris = N;
     sc.n_nodes4 = v->n_mbvh_nodes;
     sc.n_nodes2 = v->n_bvh_nodes;
-    sc.n_fnodes = v->n_mbvh_nodes;
+    sc.n_fnodes = (uint32_t)(fn.size() / ct::kWideNodeWords);
     sc.n_mats = v->n_materials;
     sc.n_lights = v->n_lights;
     sc.sky_enabled = v->sky_enabled ? 1 : 0;
@@ -262,6 +262,7 @@ int ct_scene_upload(ct_ctx* c, const ct_scene_view* v) {
     c->sc = sc;
     c->has_bvh4 = v->n_mbvh_nodes > 0;
     c->has_bvh2 = v->n_bvh_nodes > 0;
+    c->wide_stack_bound = wide_bound;
     c->has_scene = true;
     c->h_mats.assign(v->materials, v->materials + v->n_materials);
     c->h_mat_idx.assign(v->material_idx, v->material_idx + N);
@@ -636,11 +637,11 @@ int ct_autotune(ct_ctx* c, const ct_camera* cam, const ct_frame_opts* opts, int3
     const int ldss[] = {8, 16, 24, 32};
     for (int b : blocks)
         for (int l : ldss) {
-            c->shape.block = b;
-            c->shape.lds_depth = l;
-            int per_cu = ctk::trace_blocks_per_cu(b, l);
-            if (per_cu <= 0) continue;
-            c->shape.trace_blocks = std::max(1, c->num_cus * per_cu);
+            ctk::LaunchShape trial = c->shape;
+            trial.block = b;
+            trial.lds_depth = l;
+            if (!ctk::set_trace_grids(trial, c->num_cus)) continue;
+            c->shape = trial;
             double t = 0;
             bool ok = true;
             for (int rep = 0; rep < 3 && ok; ++rep) {

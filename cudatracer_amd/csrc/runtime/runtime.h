@@ -34,6 +34,7 @@ struct ct_ctx {
     // scene
     bool has_scene = false;
     bool has_bvh2 = false, has_bvh4 = false;
+    uint32_t wide_stack_bound = 0;  // worst-case stack of the wide tree (fast mode)
     ctd::DevScene sc{};
     std::vector<void*> scene_bufs;
     std::vector<ct_material> h_mats;

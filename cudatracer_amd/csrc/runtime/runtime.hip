@@ -91,8 +91,8 @@ int validate_opts(ct_ctx* c, const ct_frame_opts& o) {
     if (!c->has_scene) return fail(c, CT_ERR_NO_SCENE, "no scene uploaded");
     if (o.accel == CT_ACCEL_BVH2 && !c->has_bvh2) return fail(c, CT_ERR_INVALID_ARG, "scene has no binary tree");
     if (o.accel == CT_ACCEL_BVH4 && !c->has_bvh4) return fail(c, CT_ERR_INVALID_ARG, "scene has no quad tree");
-    if (o.traversal == CT_TRAV_FAST && c->sc.n_fnodes == 0 && o.accel == CT_ACCEL_BVH4)
-        return fail(c, CT_ERR_INVALID_ARG, "fast traversal not available for this scene");
+    if (o.traversal == CT_TRAV_FAST && (c->sc.n_fnodes == 0 || c->wide_stack_bound > (uint32_t)ctd::kMaxStack))
+        return fail(c, CT_ERR_INVALID_ARG, "fast traversal not available for this scene (wide tree too deep)");
     return CT_OK;
 }
 
@@ -280,7 +280,7 @@ static std::string graph_key(const ct_ctx* c, const float4* accum, const ct_fram
     os << (const void*)accum << ':' << c->cap << ':' << o.max_depth << ':' << o.rr_start << ':' << o.accel << ':'
        << o.bsdf_mode << ':' << o.traversal << ':' << o.shadow << ':' << o.indirect << ':' << o.normal_epsilon << ':'
        << o.dist_epsilon << ':' << o.triangle_epsilon << ':' << o.light_pdf << ':' << o.sort_by_material << ':' << c->shape.block << ':' << c->shape.lds_depth << ':'
-       << c->shape.trace_blocks << ':' << c->shape.shade_blocks << ':' << (const void*)c->d_views << ':'
+       << c->shape.grid2 << ':' << c->shape.grid4 << ':' << c->shape.grid8 << ':' << c->shape.shade_blocks << ':' << (const void*)c->d_views << ':'
        << (const void*)c->d_segs << ':' << (const void*)c->sc.tris << ':' << (const void*)c->sc.mats;
     return os.str();
 }
@@ -435,9 +435,9 @@ const char* ct_last_error(const ct_ctx* c) { return c ? c->err.c_str() : "null c
 void ctr_apply_shape(ct_ctx* c, int block, int lds_depth) {
     c->shape.block = block;
     c->shape.lds_depth = lds_depth;
-    int per_cu = ctk::trace_blocks_per_cu(block, lds_depth);
-    if (per_cu <= 0) per_cu = 4;
-    c->shape.trace_blocks = std::max(1, c->num_cus * per_cu);
+    if (!ctk::set_trace_grids(c->shape, c->num_cus)) {
+        c->shape.grid2 = c->shape.grid4 = c->shape.grid8 = c->shape.trace_blocks = std::max(8, c->num_cus * 4);
+    }
     c->shape.shade_blocks = std::max(1, c->num_cus * 8);
 }
 

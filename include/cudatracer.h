@@ -134,7 +134,8 @@ enum { CT_ACCEL_BVH2 = 0, CT_ACCEL_BVH4 = 1 };
  * dielectric, GGX and Beckmann microfacet reflection). */
 enum { CT_BSDF_REFERENCE = 0, CT_BSDF_TYPED = 1 };
 /* CT_TRAV_EXACT: reference node order and culling, bit-exact hit ids.
- * CT_TRAV_FAST: compressed 64-byte nodes; ids equal except exact t-ties. */
+ * CT_TRAV_FAST: wide compressed tree (8 children, 128-byte nodes) for either
+ * accel setting; hit distances equal, ids equal except exact t-ties. */
 enum { CT_TRAV_EXACT = 0, CT_TRAV_FAST = 1 };
 /* Light-selection pdf in the MIS weights.  CT_LIGHTPDF_REFERENCE multiplies
  * the per-triangle solid-angle pdf by the light count as the reference does
@@ -347,13 +348,17 @@ int ct_host_scene_stats(const ct_host_scene* s, uint64_t out[8]);
 int ct_trace_frame_counted(ct_ctx* ctx, const ct_camera* cam, const ct_frame_opts* opts,
                            ct_frame_stats* stats, uint64_t work[4]);
 
-/* The 64-byte compressed quad nodes of CT_TRAV_FAST, 16 words per node,
- * built by ct_scene_upload from the quad tree (no counterpart in the
- * reference; exposed so the quantisation can be checked on the host).
- * Words: p.xyz (float), biased exponents of the per-axis steps | valid-child
- * mask << 24, per axis the low and high 8-bit child offsets, four child
- * entries (node index, or 0x80000000 | first primitive-list position). */
-int ct_compress_quad_nodes(const ct_mbvh_node* nodes, uint32_t n, uint32_t* out_words);
+/* The wide compressed tree of CT_TRAV_FAST, which ct_scene_upload builds
+ * from the view's binary tree (or quad tree): up to 8 children per node,
+ * 32 words per node (no counterpart in the reference; exposed so the
+ * quantisation can be checked on the host).  Words: p.xyz (float), biased
+ * exponents of the per-axis steps | valid-child mask << 24, per axis the low
+ * and high 8-bit child offsets (8 bytes each), eight child entries (node
+ * index, or 0x80000000 | first primitive-list position), 8 zero words.
+ * Call with out_words == NULL for the node count; stack_bound (may be NULL)
+ * receives the deepest traversal stack the tree can need. */
+int ct_build_wide_nodes(const ct_scene_view* v, uint32_t* out_words, uint32_t capacity_nodes, uint32_t* n_nodes,
+                        uint32_t* stack_bound);
 
 #ifdef __cplusplus
 }

@@ -57,7 +57,7 @@ def test_closest_hit_matches_brute_force_c1(ct, orc, gpu_ctx, c1):
 
 @pytest.mark.parametrize("config", [1, 2])
 def test_fast_traversal_closest_hit(ct, orc, gpu_ctx, config):
-    """Compressed 64-byte quad nodes (CT_TRAV_FAST) against the oracle's exact
+    """The wide compressed tree (CT_TRAV_FAST) against the oracle's exact
     traversal: the same triangle records are tested with the same arithmetic,
     so t is bit-identical wherever both find the same nearest surface; ids
     differ only on exact t ties.  The few rays whose t differs are ones where
@@ -86,7 +86,7 @@ def test_fast_traversal_closest_hit(ct, orc, gpu_ctx, config):
 
 @pytest.mark.parametrize("config", [1, 2])
 def test_fast_traversal_any_hit(ct, orc, gpu_ctx, config):
-    """Any-hit over compressed nodes equals the exact any-hit (order-free)."""
+    """Any-hit over the wide tree equals the exact any-hit (order-free)."""
     from conftest import get_scene
     scene, arrays, osc = get_scene(ct, orc, config)
     _upload(gpu_ctx, scene)

@@ -158,49 +158,64 @@ def test_builder_invariants(ct, orc, config):
     assert max_depth <= 63
 
 
-def _mbvh_records(raw: np.ndarray):
-    return np.frombuffer(raw.tobytes(), dtype=[("lo", "<f4", (3, 4)), ("hi", "<f4", (3, 4)), ("ch", "<i4", 4),
-                                               ("cnt", "<i4", 4)])
-
-
-def _interleave_mbvh(rec):
-    """(n, 3 axes, 4 children) lo/hi from bminx, bmaxx, bminy, bmaxy, bminz, bmaxz."""
-    f = np.frombuffer(rec.tobytes(), "<f4").reshape(-1, 32)[:, :24].reshape(-1, 6, 4)
-    return f[:, 0::2, :], f[:, 1::2, :]
-
-
-@pytest.mark.parametrize("config", [1, 2])
-def test_compressed_nodes_contain_exact_boxes(ct, orc, config):
-    """Every decoded child box of the fast mode's 64-byte nodes, computed as
-    the kernel does (fmaf(q, 2^k, p), one rounding), contains the exact child
-    box; padding is at most a few grid steps; child entries and the valid mask
-    match the quad tree."""
-    scene, arrays, _ = get_scene(ct, orc, config)
-    rec = _mbvh_records(arrays["mbvh"])
-    lo, hi = _interleave_mbvh(rec)
-    w = ct.compress_quad_nodes(arrays["mbvh"])
-    assert w.shape == (len(rec), 16)
-    valid = (w[:, 3] >> 24) & 0xF
-    cnt = rec["cnt"]
-    assert np.array_equal(valid, ((cnt != 0) * (1 << np.arange(4))).sum(1))
+def _decode_wide(w):
+    """Decoded child boxes (n, 8, 3) lo / hi of wide nodes, as the kernel
+    computes them: fmaf(q, 2^k, p) rounded once to float32."""
     p = w[:, 0:3].view(np.float32).astype(np.float64)
     k = np.stack([((w[:, 3] >> (8 * a)) & 0xFF).astype(np.int64) - 127 for a in range(3)], 1)
     step = np.ldexp(1.0, k)
+    lo = np.zeros((len(w), 8, 3), np.float32)
+    hi = np.zeros((len(w), 8, 3), np.float32)
     for a in range(3):
-        ql = np.stack([(w[:, 4 + 2 * a] >> (8 * c)) & 0xFF for c in range(4)], 1).astype(np.float64)
-        qh = np.stack([(w[:, 5 + 2 * a] >> (8 * c)) & 0xFF for c in range(4)], 1).astype(np.float64)
-        # q * 2^k + p is exact in float64 here; one rounding to float32 = fmaf
-        dlo = (ql * step[:, a:a + 1] + p[:, a:a + 1]).astype(np.float32)
-        dhi = (qh * step[:, a:a + 1] + p[:, a:a + 1]).astype(np.float32)
-        m = cnt != 0
-        assert np.all(dlo[m] <= lo[:, a, :][m]) and np.all(dhi[m] >= hi[:, a, :][m])
-        pad = np.maximum(lo[:, a, :] - dlo, dhi - hi[:, a, :])
-        assert np.all(pad[m] <= 4 * np.broadcast_to(step[:, a:a + 1], pad.shape)[m])
-    ent = w[:, 10:14]
-    leaf = cnt > 0
-    inner = cnt == -1
-    assert np.array_equal(ent[inner], rec["ch"][inner].astype(np.uint32))
-    assert np.array_equal(ent[leaf], rec["ch"][leaf].astype(np.uint32) | np.uint32(0x80000000))
+        for c in range(8):
+            ql = (w[:, 4 + 4 * a + c // 4] >> (8 * (c % 4))) & 0xFF
+            qh = (w[:, 6 + 4 * a + c // 4] >> (8 * (c % 4))) & 0xFF
+            # q * 2^k + p is exact in float64 for these magnitudes; one rounding = fmaf
+            lo[:, c, a] = (ql * step[:, a] + p[:, a]).astype(np.float32)
+            hi[:, c, a] = (qh * step[:, a] + p[:, a]).astype(np.float32)
+    return lo, hi
+
+
+@pytest.mark.parametrize("config", [1, 2])
+def test_wide_nodes_contain_their_geometry(ct, orc, config):
+    """The fast mode's wide tree: every decoded child box (as the kernel
+    decodes it) contains all vertices beneath it, every primitive is reached
+    through exactly one leaf, leaves are the binary tree's leaves, arity <= 8,
+    and the worst-case stack fits the traversal's 128 entries."""
+    scene, arrays, _ = get_scene(ct, orc, config)
+    w, bound = ct.build_wide_nodes(scene)
+    assert 0 < bound <= 128
+    n = len(w)
+    valid = (w[:, 3] >> 24) & 0xFF
+    arity = np.array([bin(int(v)).count("1") for v in valid])
+    assert np.all(arity >= 1) and np.all(arity <= 8)
+    assert np.all(valid == (1 << arity) - 1)  # children packed from slot 0
+    bvh = _bvh_records(arrays["bvh"])
+    leaf_count = {int(r["lf"]): int(r["cnt"]) for r in bvh if r["cnt"] >= 0}
+    V = arrays["vertices"]
+    I = arrays["indices"]
+    prims = arrays["prim_indices"]
+    lo, hi = _decode_wide(w)
+    # vertex bounds beneath each node, children before parents (higher index)
+    vmin = np.full((n, 3), np.inf)
+    vmax = np.full((n, 3), -np.inf)
+    seen = np.zeros(len(prims), np.int32)
+    for i in range(n - 1, -1, -1):
+        for c in range(int(arity[i])):
+            e = int(w[i, 16 + c])
+            if e & 0x80000000:
+                first = e & 0x7FFFFFFF
+                cnt = leaf_count[first]
+                seen[first:first + cnt] += 1
+                pts = V[I[prims[first:first + cnt]]].reshape(-1, 3)
+                bmin, bmax = pts.min(0), pts.max(0)
+            else:
+                assert i < e < n
+                bmin, bmax = vmin[e], vmax[e]
+            assert np.all(lo[i, c] <= bmin) and np.all(hi[i, c] >= bmax), (i, c)
+            vmin[i] = np.minimum(vmin[i], bmin)
+            vmax[i] = np.maximum(vmax[i], bmax)
+    assert np.all(seen == 1)
 
 
 # -------------------------------------------------------------- traversal --
