@@ -140,7 +140,7 @@ def main() -> int:
         "peak": HBM_PEAK_GBS,
         "unit": "GB/s",
         "frac": round(achieved / HBM_PEAK_GBS, 4),
-        "traffic": None,
+        "traffic": measured_traffic(args.config, args.traversal),
         "bytes_per_launch": int(bytes_per_launch),
         "mean_launch_ms": round(mean_launch_ms, 4),
         "node_tests_per_ray": round(work[0] / max(closest_rays, 1), 2),
@@ -184,6 +184,18 @@ def main() -> int:
                                            strip=1 if args.config == 2 else 16)
     print(json.dumps(out))
     return 0
+
+
+def measured_traffic(config: int, traversal: str):
+    """HBM bytes per extend launch of this workload from the committed
+    rocprofv3 summary (FETCH_SIZE and WRITE_SIZE per dispatch, each scaled by
+    what the counter reads for a plain copy of known size), or None."""
+    import re
+    f = ROOT / "profiles" / f"k_extend_c{config}.md"
+    if not f.exists():
+        return None
+    m = re.search(r"^traffic_bytes_per_launch %s: (\d+)" % traversal, f.read_text(), re.M)
+    return int(m.group(1)) if m else None
 
 
 def _pct(x, q):
@@ -342,9 +354,15 @@ def summarize(paths: list) -> int:
         for r in rows:
             print(f"| {r[0].split('(')[0]} | {r[1]} | {r[2] / 1e6:.3f} | {r[3] / 1e3:.1f} | {r[4] / 1e3:.1f} | "
                   f"{r[5] / 1e3:.1f} | {100 * r[2] / tot:.1f} |")
-        pmc = db.execute("select k.name, p.counter_name, avg(p.counter_value), count(*) from pmc_events p "
-                         "join kernels k on k.dispatch_id = p.dispatch_id group by k.name, p.counter_name "
-                         "order by k.name").fetchall()
+        # a counter can come as one row per hardware instance (XCD, SE):
+        # hardware counters are summed per dispatch (GRBM_GUI_ACTIVE, a clock
+        # every instance sees, takes the largest), derived ones averaged
+        pmc = db.execute(
+            "select k.name, d.counter_name, avg(d.v), count(*) from (select dispatch_id, counter_name, "
+            "case when counter_name like 'GRBM%' then max(counter_value) "
+            "when counter_name glob '[A-Z][A-Z]*_*' then sum(counter_value) else avg(counter_value) end v "
+            "from pmc_events group by dispatch_id, counter_name) d "
+            "join kernels k on k.dispatch_id = d.dispatch_id group by k.name, d.counter_name order by k.name").fetchall()
         if pmc:
             print("| kernel | counter | mean per dispatch | dispatches |")
             print("|---|---|---|---|")

@@ -4,7 +4,7 @@
 //   setGlobals          :75-86                  k_setup (all bounce counters zeroed once per chunk)
 //   generatePrimaryRays :208-234                k_generate
 //   extend              :236-245                k_extend   (persistent, wave-fetched)
-//   shade               :88-206                 k_shade    (wave64 ballot + prefix-sum queue append)
+//   shade               :88-206                 k_shade    (ballot + block prefix-sum queue append)
 //   connect             :247-256                k_connect  (persistent, wave-fetched)
 //   accumulation        :100,134,255            k_resolve  (per-path radiance summed in sample order)
 //   draw_framebuffer    :258-270                k_tonemap
@@ -19,21 +19,6 @@
 #include "traverse.h"
 
 namespace ctd {
-
-// ------------------------------------------------------------ wave tools --
-
-// Append for all lanes with pred set: one atomic per wave, offsets by a
-// prefix popcount of the ballot.  Must be reached by every lane of the wave.
-CT_DEV uint32_t wave_append(bool pred, uint32_t* counter) {
-    const uint64_t mask = __ballot(pred ? 1 : 0);
-    if (mask == 0) return 0;
-    const uint32_t lane = lane_id();
-    const int leader = __ffsll((unsigned long long)mask) - 1;
-    uint32_t base = 0;
-    if ((int)lane == leader) base = atomicAdd(counter, (uint32_t)__popcll(mask));
-    base = __shfl(base, leader);
-    return base + (uint32_t)__popcll(mask & ((1ull << lane) - 1ull));
-}
 
 CT_DEV int find_seg(const Segment* __restrict__ segs, int nseg, uint32_t slot) {
     int lo = 0, hi = nseg - 1;
@@ -235,6 +220,39 @@ CT_DEV uint32_t sorted_entry(const DevScene& sc, const Queues& q, uint32_t first
     return perm[threadIdx.x];
 }
 
+// Appends to the extension and shadow queues for a block of kShadeBlock
+// threads: LDS prefix over the block's 4 waves and one atomic per block and
+// queue.  Every thread of the block must call it (two barriers).  One atomic
+// per wave instead, all on the same two counters from every XCD, measured
+// 2.2x slower shading (the atomics serialise).  The LDS slots are safe to
+// reuse in the next call: a wave writes only its own count slot before the
+// first barrier, and thread 0 rewrites the shared ones only after it.
+CT_DEV void block_append2(bool pe, bool ps, uint32_t* ce, uint32_t* cs, uint32_t* lds, uint32_t& je, uint32_t& js) {
+    const uint64_t me = __ballot(pe ? 1 : 0), ms = __ballot(ps ? 1 : 0);
+    const uint32_t w = threadIdx.x >> 6, lane = lane_id();
+    const uint64_t lt = lane == 0 ? 0ull : (~0ull >> (64u - lane));
+    if (lane == 0) {
+        lds[w] = (uint32_t)__popcll(me);
+        lds[4 + w] = (uint32_t)__popcll(ms);
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t te = 0, ts = 0;
+        for (int k = 0; k < 4; ++k) {
+            const uint32_t a = lds[k], b = lds[4 + k];
+            lds[k] = te;
+            lds[4 + k] = ts;
+            te += a;
+            ts += b;
+        }
+        lds[8] = te ? atomicAdd(ce, te) : 0u;
+        lds[9] = ts ? atomicAdd(cs, ts) : 0u;
+    }
+    __syncthreads();
+    je = lds[8] + lds[w] + (uint32_t)__popcll(me & lt);
+    js = lds[9] + lds[4 + w] + (uint32_t)__popcll(ms & lt);
+}
+
 // shade (Kernel.cu:88-206) for queue entries [0, active[bounce]); see the
 // oracle's path_sample for the same steps on the CPU.  SORT: shade in
 // block-local material order (ct_frame_opts.sort_by_material).
@@ -243,6 +261,7 @@ __global__ void __launch_bounds__(kShadeBlock) k_shade(DevScene sc, ShadeOpts so
                                                       const Segment* __restrict__ segs, const ChunkHdr* hdr, Queues q,
                                                       int sel, int bounce) {
     __shared__ uint32_t s_hist[kSortKeys];
+    __shared__ uint32_t s_app[10];
     __shared__ uint32_t s_perm[SORT ? kShadeBlock : 1];
     const uint32_t n = *cnt_active(q.counters, bounce);
     const int nseg = (int)hdr->nseg;
@@ -450,15 +469,16 @@ __global__ void __launch_bounds__(kShadeBlock) k_shade(DevScene sc, ShadeOpts so
                 q.lpath[slot] = L;
             }
         }
-        // wave-aggregated queue appends (all lanes of the wave reach here)
-        const uint32_t je = wave_append(push_ext, cnt_active(q.counters, bounce + 1));
+        // block-aggregated queue appends (every thread of the block reaches here)
+        uint32_t je = 0, js = 0;
+        block_append2(push_ext, push_sh, cnt_active(q.counters, bounce + 1), cnt_shadow(q.counters, bounce), s_app, je,
+                      js);
         if (push_ext) {
             q.ray_o[nsel][je] = make_float4(e_o.x, e_o.y, e_o.z, 0.0f);
             q.ray_d[nsel][je] = make_float4(e_d.x, e_d.y, e_d.z, 0.0f);
             q.thr[nsel][je] = make_float4(e_thr.x, e_thr.y, e_thr.z, e_pdf);
             q.path[nsel][je] = slot | (e_type << 30);
         }
-        const uint32_t js = wave_append(push_sh, cnt_shadow(q.counters, bounce));
         if (push_sh) {
             q.sh_o[js] = make_float4(s_o.x, s_o.y, s_o.z, s_tmax);
             q.sh_d[js] = make_float4(s_d.x, s_d.y, s_d.z, __uint_as_float(slot));

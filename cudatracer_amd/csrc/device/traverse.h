@@ -177,78 +177,69 @@ CT_DEV void node4_step(const float4* __restrict__ nodes4, uint32_t e, f3 o, f3 i
     push_children<SORT>(hm, m0, m1, m2, m3, e0, e1, e2, e3, st);
 }
 
-// Wide compressed node (128 B, up to 8 children, layout in host/cnode.h):
-// child planes decoded to world space as fmaf(q, 2^k, p), then the slab
-// arithmetic of node4_step.  Keys are tmin with the slot in the low 3 bits
-// (misses get a huge key); the optimal 19-comparator network sorts them and
-// the hit children are pushed farthest first.  SORT = false pushes in slot
-// order (any-hit).
-CT_DEV float wide_key(bool hit, float tmin, uint32_t slot) {
-    return hit ? __uint_as_float((fbits(tmin) & ~7u) | slot) : __uint_as_float(0x7F7FFFF8u | slot);
-}
-CT_DEV void cswap(float& a, float& b) {
-    const float lo = gmin(a, b), hi = gmax(a, b);
-    a = lo;
-    b = hi;
-}
-
+// Wide compressed node (128 B, up to 8 children, layout in host/cnode.h).
+// The ray is moved into the node's grid once per node, t = q * (s / d) +
+// (p - o) / d, so each child plane costs one conversion and one fma (the
+// decoded boxes contain the exact ones by a grid step, which covers the
+// rounding of this form).  The nearest hit child is pushed last (popped
+// first), the others in slot order: a full sort of 8 measured slower.
+// SORT = false (any-hit) pushes in slot order.
 template <bool SORT>
 CT_DEV void fnode8_step(const uint4* __restrict__ fnodes, uint32_t e, f3 o, f3 inv, float t, LaneStack& st) {
     const uint4* n = fnodes + 8 * (size_t)e;
     const uint4 h = n[0], qx = n[1], qy = n[2], qz = n[3], ca = n[4], cb = n[5];
-    const float px = __uint_as_float(h.x), py = __uint_as_float(h.y), pz = __uint_as_float(h.z);
     const float sx = __uint_as_float((h.w & 0xFFu) << 23), sy = __uint_as_float(((h.w >> 8) & 0xFFu) << 23),
                 sz = __uint_as_float(((h.w >> 16) & 0xFFu) << 23);
+    // the ray in the node's grid: t = q * (s / d) + (p - o) / d
+    const float ax = sx * inv.x, ay = sy * inv.y, az = sz * inv.z;
+    const float bx = (__uint_as_float(h.x) - o.x) * inv.x, by = (__uint_as_float(h.y) - o.y) * inv.y,
+                bz = (__uint_as_float(h.z) - o.z) * inv.z;
     const uint32_t valid = h.w >> 24;
-    float key[8];
     uint32_t hm = 0;
+    float best = 3.0e38f;
+    uint32_t bslot = 0;
 #pragma unroll
     for (int i = 0; i < 8; ++i) {
         const uint32_t sh = 8u * (uint32_t)(i & 3);
         const uint32_t lxw = i < 4 ? qx.x : qx.y, hxw = i < 4 ? qx.z : qx.w;
         const uint32_t lyw = i < 4 ? qy.x : qy.y, hyw = i < 4 ? qy.z : qy.w;
         const uint32_t lzw = i < 4 ? qz.x : qz.y, hzw = i < 4 ? qz.z : qz.w;
-        float t1 = (fmaf((float)((lxw >> sh) & 0xFFu), sx, px) - o.x) * inv.x;
-        float t2 = (fmaf((float)((hxw >> sh) & 0xFFu), sx, px) - o.x) * inv.x;
+        float t1 = fmaf((float)((lxw >> sh) & 0xFFu), ax, bx), t2 = fmaf((float)((hxw >> sh) & 0xFFu), ax, bx);
         float mn = gmin(t1, t2), mx = gmax(t1, t2);
-        t1 = (fmaf((float)((lyw >> sh) & 0xFFu), sy, py) - o.y) * inv.y;
-        t2 = (fmaf((float)((hyw >> sh) & 0xFFu), sy, py) - o.y) * inv.y;
+        t1 = fmaf((float)((lyw >> sh) & 0xFFu), ay, by);
+        t2 = fmaf((float)((hyw >> sh) & 0xFFu), ay, by);
         mn = gmax(mn, gmin(t1, t2));
         mx = gmin(mx, gmax(t1, t2));
-        t1 = (fmaf((float)((lzw >> sh) & 0xFFu), sz, pz) - o.z) * inv.z;
-        t2 = (fmaf((float)((hzw >> sh) & 0xFFu), sz, pz) - o.z) * inv.z;
+        t1 = fmaf((float)((lzw >> sh) & 0xFFu), az, bz);
+        t2 = fmaf((float)((hzw >> sh) & 0xFFu), az, bz);
         mn = gmax(mn, gmin(t1, t2));
         mx = gmin(mx, gmax(t1, t2));
         const bool hit = ((valid >> i) & 1u) && mx > 0.0f && mn <= mx && mn < t;
         hm |= hit ? (1u << i) : 0u;
-        key[i] = wide_key(hit, mn, (uint32_t)i);
+        if (hit && mn < best) { best = mn; bslot = (uint32_t)i; }
     }
     if (hm == 0u) return;
     const uint32_t ent[8] = {ca.x, ca.y, ca.z, ca.w, cb.x, cb.y, cb.z, cb.w};
-    auto entry = [&](uint32_t slot) {
-        const uint32_t a0 = (slot & 1u) ? ent[1] : ent[0], a1 = (slot & 1u) ? ent[3] : ent[2];
-        const uint32_t a2 = (slot & 1u) ? ent[5] : ent[4], a3 = (slot & 1u) ? ent[7] : ent[6];
-        const uint32_t b0 = (slot & 2u) ? a1 : a0, b1 = (slot & 2u) ? a3 : a2;
-        return (slot & 4u) ? b1 : b0;
-    };
-    if (!SORT) {
-        st.push4(ent[0], hm & 1u, ent[1], (hm >> 1) & 1u, ent[2], (hm >> 2) & 1u, ent[3], (hm >> 3) & 1u);
-        st.push4(ent[4], (hm >> 4) & 1u, ent[5], (hm >> 5) & 1u, ent[6], (hm >> 6) & 1u, ent[7], (hm >> 7) & 1u);
+    if (SORT) {
+        // nearest hit child on top; slot 7 takes the nearest child's place
+        const uint32_t a0 = (bslot & 1u) ? ent[1] : ent[0], a1 = (bslot & 1u) ? ent[3] : ent[2];
+        const uint32_t a2 = (bslot & 1u) ? ent[5] : ent[4], a3 = (bslot & 1u) ? ent[7] : ent[6];
+        const uint32_t b0 = (bslot & 2u) ? a1 : a0, b1 = (bslot & 2u) ? a3 : a2;
+        const uint32_t near = (bslot & 4u) ? b1 : b0;
+        const uint32_t h7 = (hm >> 7) & 1u;
+        uint32_t x[7], k[7];
+#pragma unroll
+        for (int i = 0; i < 7; ++i) {
+            const bool at = (uint32_t)i == bslot;
+            x[i] = at ? ent[7] : ent[i];
+            k[i] = at ? h7 : ((hm >> i) & 1u);
+        }
+        st.push4(x[0], k[0], x[1], k[1], x[2], k[2], x[3], k[3]);
+        st.push4(x[4], k[4], x[5], k[5], x[6], k[6], near, true);
         return;
     }
-    cswap(key[0], key[2]); cswap(key[1], key[3]); cswap(key[4], key[6]); cswap(key[5], key[7]);
-    cswap(key[0], key[4]); cswap(key[1], key[5]); cswap(key[2], key[6]); cswap(key[3], key[7]);
-    cswap(key[0], key[1]); cswap(key[2], key[3]); cswap(key[4], key[5]); cswap(key[6], key[7]);
-    cswap(key[2], key[4]); cswap(key[3], key[5]);
-    cswap(key[1], key[4]); cswap(key[3], key[6]);
-    cswap(key[1], key[2]); cswap(key[3], key[4]); cswap(key[5], key[6]);
-    // hits sort before misses: push sorted positions h-1 .. 0 (farthest first)
-    const int nh = __popc(hm);
-    auto slot_of = [&](int j) { return fbits(key[j]) & 7u; };
-    st.push4(entry(slot_of(7)), nh > 7, entry(slot_of(6)), nh > 6, entry(slot_of(5)), nh > 5, entry(slot_of(4)),
-             nh > 4);
-    st.push4(entry(slot_of(3)), nh > 3, entry(slot_of(2)), nh > 2, entry(slot_of(1)), nh > 1, entry(slot_of(0)),
-             nh > 0);
+    st.push4(ent[0], hm & 1u, ent[1], (hm >> 1) & 1u, ent[2], (hm >> 2) & 1u, ent[3], (hm >> 3) & 1u);
+    st.push4(ent[4], (hm >> 4) & 1u, ent[5], (hm >> 5) & 1u, ent[6], (hm >> 6) & 1u, ent[7], (hm >> 7) & 1u);
 }
 
 // BVHNode::Intersect (BVHNode.cuh:50-62): no culling against the current t.

@@ -312,6 +312,7 @@ void MBVHTree::mergeNodes(unsigned int mIdx, const BVHNode& node) {
                 m_Tree[mIdx].SetBounds(idx, curNode.min, curNode.max);
             } else {
                 const unsigned int newIdx = m_FinalPtr++;
+                m_Tree.emplace_back();  // nodes are addressed by index: growth is safe
                 m_Tree[mIdx].childs[idx] = (int)newIdx;
                 m_Tree[mIdx].counts[idx] = -1;
                 m_Tree[mIdx].SetBounds(idx, curNode.min, curNode.max);
@@ -332,7 +333,9 @@ void MBVHTree::constructBVH() {
     const std::vector<BVHNode>& pool = m_OriginalTree->m_BVHPool;
     if (pool.empty()) return;
     const double t0 = now_ms();
-    m_Tree.resize(m_OriginalTree->m_PrimitiveCount * 2 + 1);
+    // grown one node at a time (the reference sizes it 2N + 1 up front)
+    m_Tree.reserve(pool.size() / 2 + 1);
+    m_Tree.emplace_back();
     m_FinalPtr = 1;
     const BVHNode& root = pool[0];
     if (root.IsLeaf()) {

@@ -236,6 +236,7 @@ void gen_c2(uint64_t seed, TriangleList& L, SceneConfig& cfg) {
     cfg.opts.spp = 1;
     cfg.opts.accel = CT_ACCEL_BVH4;
     cfg.opts.bsdf_mode = CT_BSDF_TYPED;
+    cfg.opts.sort_by_material = 1;  // mixed BSDFs: shade in material order
     cfg.opts.normal_epsilon = 1e-3f;
     cfg.opts.dist_epsilon = 1e-2f;
     cfg.width = 1920;
@@ -278,6 +279,7 @@ void gen_c3(uint64_t seed, TriangleList& L, SceneConfig& cfg) {
     cfg.opts.rr_start = 3;
     cfg.opts.accel = CT_ACCEL_BVH4;
     cfg.opts.bsdf_mode = CT_BSDF_TYPED;
+    cfg.opts.sort_by_material = 1;  // mixed BSDFs: shade in material order
     cfg.opts.normal_epsilon = 1e-3f;
     cfg.opts.dist_epsilon = 1e-2f;
     cfg.width = 3840;

@@ -462,8 +462,12 @@ int ct_ctx_create(const ct_device_opts* opts, ct_ctx** out) {
     if (hipSetDevice(c->device) != hipSuccess) { delete c; return CT_ERR_DEVICE; }
     hipDeviceProp_t props;
     if (hipGetDeviceProperties(&props, c->device) != hipSuccess) { delete c; return CT_ERR_DEVICE; }
-    c->name = props.name;
-    if (props.gcnArchName[0]) c->name += std::string(" (") + props.gcnArchName + ")";
+    // the marketing name can be empty; the architecture, CU count and memory
+    // identify the part
+    std::ostringstream nm;
+    nm << (props.name[0] ? props.name : "AMD GPU") << " (" << (props.gcnArchName[0] ? props.gcnArchName : "?") << ", "
+       << props.multiProcessorCount << " CUs, " << (props.totalGlobalMem >> 30) << " GiB)";
+    c->name = nm.str();
     c->num_cus = props.multiProcessorCount;
     size_t freeb = 0, totalb = 0;
     if (hipMemGetInfo(&freeb, &totalb) != hipSuccess) { delete c; return CT_ERR_DEVICE; }
