@@ -127,11 +127,11 @@ struct Queues {
 constexpr int kMaxBounce = 64;
 CT_DEV uint32_t* cnt_active(uint32_t* c, int b) { return c + b; }
 CT_DEV uint32_t* cnt_shadow(uint32_t* c, int b) { return c + (kMaxBounce + 1) + b; }
-// Work-fetch counters, [queue][range][bounce]: each XCD range owns a
+// Work-fetch counters, [queue][range][bounce]: each queue range owns a
 // 384-byte slot so ranges never share a cache line (same-line atomics from
-// different XCDs serialise).
+// different XCDs serialise).  Four ranges per XCD.
 constexpr int kXcds = 8;
-constexpr int kFetchRanges = kXcds;
+constexpr int kFetchRanges = 4 * kXcds;
 constexpr int kFetchSlot = 96;
 constexpr int kFetchBase = 192;
 CT_DEV uint32_t* cnt_fetch_ext(uint32_t* c, int b) { return c + kFetchBase + b; }
