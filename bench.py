@@ -15,6 +15,9 @@ frames).
 cpu_baseline: the CPU oracle (FP32 restatement of the reference loop, OpenMP
 over pixels) on whole frames of the same workload, about 10 s of CPU work.
 
+accuracy: frame 0 of the workload on the GPU against the oracle on a strip of
+rows (bit-identical fraction vs FP32, rel-MSE vs FP64).
+
 served (unless --no-serve): render requests on the resident atrium arriving
 as a Poisson stream (resolution 256x256 / 512x512 / 1920x1080 with
 probability .5 / .35 / .15, spp uniform in {1, 4, 16}, depth 8, seeded
@@ -175,6 +178,8 @@ def main() -> int:
         "roofline": roofline,
     }
 
+    if not args.no_cpu:
+        out["accuracy"] = accuracy_strip(ctx, scene, cam, opts, W, H, strip=32 if args.config == 2 else 128)
     if not args.no_serve and args.config == 2:
         out["served"] = serve_leg(ct, ctx, scene, opts, args.seed, chunk=args.serve_chunk)
         out["batched"] = batched_leg(ct, ctx, args.seed)
@@ -184,6 +189,30 @@ def main() -> int:
                                            strip=1 if args.config == 2 else 16)
     print(json.dumps(out))
     return 0
+
+
+def accuracy_strip(ctx, scene, cam, opts, W, H, strip: int) -> dict:
+    """Frame 0 of the workload on the GPU against the CPU oracle on the first
+    H / strip rows: bit-identical fraction against the FP32 restatement and
+    rel-MSE (mean of (x - ref)^2 / (ref^2 + 1e-2)) against the FP64 one."""
+    import numpy as np
+    from oracle import oracle as orc
+    threads = min(16, os.cpu_count() or 1)
+    rows = H // strip
+    ctx.resize(W, H)
+    ctx.trace_frame(cam, opts, stats=False)
+    acc, _ = ctx.read_accum()
+    osc = orc.Scene(scene.arrays())
+    ref32 = np.zeros((H, W, 4), np.float32)
+    ref64 = np.zeros((H, W, 4), np.float64)
+    osc.render(cam, opts, W, H, 0, 1, rows=(0, rows), threads=threads, accum=ref32)
+    osc.render(cam, opts, W, H, 0, 1, rows=(0, rows), precision=64, threads=threads, accum=ref64)
+    g = acc[:rows]
+    same = float(np.mean(g.view(np.uint32) == ref32[:rows].view(np.uint32)))
+    x, r = g[..., :3].astype(np.float64), ref64[:rows, :, :3]
+    rel = float(np.mean((x - r) ** 2 / (r ** 2 + 1e-2)))
+    return {"rows": f"0..{rows - 1} of {H}", "frame": 0, "bit_identical_vs_fp32_oracle": round(same, 6),
+            "rel_mse_vs_fp64_oracle": float(f"{rel:.3e}")}
 
 
 def measured_traffic(config: int, traversal: str):

@@ -179,6 +179,8 @@ _sig = {
     "ct_host_scene_add_triangle": (C.c_int, [_P, C.POINTER(C.c_float), C.POINTER(C.c_float), C.c_uint32,
                                              C.POINTER(C.c_float)]),
     "ct_host_scene_build": (C.c_int, [_P]),
+    "ct_host_scene_load_model": (C.c_int, [_P, C.c_char_p, C.c_float, C.POINTER(C.c_float), C.c_int32, C.c_int32]),
+    "ct_host_scene_last_error": (C.c_char_p, [_P]),
     "ct_host_scene_view": (C.c_int, [_P, C.POINTER(SceneView)]),
     "ct_host_scene_set_sky": (C.c_int, [_P, C.c_int32, C.POINTER(C.c_float)]),
     "ct_host_scene_camera": (C.c_int, [_P, C.c_int32, C.c_int32, C.POINTER(Camera)]),
@@ -266,6 +268,18 @@ class HostScene:
         n = np.ascontiguousarray(n, np.float32).reshape(9)
         uvp = _fp(np.ascontiguousarray(uv, np.float32).reshape(6)) if uv is not None else None
         _check(lib.ct_host_scene_add_triangle(self._h, _fp(p), _fp(n), material, uvp), None)
+
+    def load_model(self, path, scale: float = 1.0, mat=None, material: int = -1, normalize: bool = True) -> None:
+        """TriangleList::loadModel for an OBJ file.  mat: 4x4 transform
+        (row-major numpy-style matrix, applied to column vectors); None =
+        identity."""
+        mp = None
+        if mat is not None:
+            m = np.asarray(mat, np.float32).reshape(4, 4)
+            mp = _fp(np.ascontiguousarray(m.T))  # column-major, like glm
+        r = lib.ct_host_scene_load_model(self._h, str(path).encode(), scale, mp, material, 1 if normalize else 0)
+        if r != CT_OK:
+            raise CTError(r, lib.ct_host_scene_last_error(self._h).decode())
 
     def set_sky(self, enabled: bool, rgb=(0, 0, 0)) -> None:
         a = np.asarray(rgb, np.float32)

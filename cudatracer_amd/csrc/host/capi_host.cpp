@@ -4,6 +4,7 @@
 #include <cstring>
 #include <memory>
 #include <new>
+#include <string>
 
 #include "../../../include/cudatracer.h"
 #include "bvh.h"
@@ -18,6 +19,7 @@ struct ct_host_scene {
     ct::SceneConfig cfg;
     bool generated = false;
     std::vector<ct_material> mats;
+    std::string error;
 };
 
 extern "C" {
@@ -67,6 +69,25 @@ int ct_host_scene_add_material(ct_host_scene* s, const ct_material* m, int32_t k
     s->mbvh.reset();
     return (int)s->list.addMaterial(ct::Material::from_abi(*m), keep_roughness != 0);
 }
+
+int ct_host_scene_load_model(ct_host_scene* s, const char* path, float scale, const float mat[16], int32_t material,
+                             int32_t normalize) {
+    if (!s || !path || !std::isfinite(scale)) return CT_ERR_INVALID_ARG;
+    if (mat)
+        for (int i = 0; i < 16; ++i)
+            if (!std::isfinite(mat[i])) return CT_ERR_INVALID_ARG;
+    if (material >= (int32_t)s->list.m_Materials.size()) return CT_ERR_INVALID_ARG;
+    std::string why;
+    if (!s->list.loadModel(path, scale, mat, material, normalize != 0, &why)) {
+        s->error = why;
+        return why.rfind("cannot open", 0) == 0 ? CT_ERR_INVALID_ARG : CT_ERR_INVALID_SCENE;
+    }
+    s->bvh.reset();
+    s->mbvh.reset();
+    return CT_OK;
+}
+
+const char* ct_host_scene_last_error(const ct_host_scene* s) { return s ? s->error.c_str() : ""; }
 
 int ct_host_scene_add_triangle(ct_host_scene* s, const float p[9], const float n[9], uint32_t material,
                                const float uv[6]) {

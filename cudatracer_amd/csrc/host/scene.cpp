@@ -139,8 +139,6 @@ unsigned int TriangleList::addMaterial(Material mat, bool keepRoughness) {
     return m;
 }
 
-bool TriangleList::loadModel(const std::string&, float) { return false; }
-
 Camera::Camera(int width, int height, float fov, vec3 pos)
     : m_Position(pos), m_Fov(fov), m_Width((float)width), m_Height((float)height),
       m_InvWidth(1.f / (float)width), m_InvHeight(1.f / (float)height) {

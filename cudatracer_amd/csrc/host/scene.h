@@ -73,9 +73,13 @@ public:
     // keepRoughness = true is the extension the config scenes use.
     unsigned int addMaterial(Material mat, bool keepRoughness = false);
 
-    // Assimp import is not available on this platform; OBJ import is a
-    // documented follow-up.  Returns false and leaves the list unchanged.
-    bool loadModel(const std::string& path, float scale = 1.0f);
+    // TriangleList::loadModel (TriangleList.h:49, TriangleList.cpp:63-146)
+    // for Wavefront OBJ files (host/objload.cpp).  mat: column-major 4x4
+    // (glm layout), nullptr = identity.  material < 0: material 0 (MTL files
+    // are not read).  Returns false with a message in *error on failure,
+    // leaving the list unchanged.
+    bool loadModel(const std::string& path, float scale = 1.0f, const float* mat = nullptr, int material = -1,
+                   bool normalize = true, std::string* error = nullptr);
 
     size_t getPrimCount() const { return m_Indices.size(); }
 

@@ -325,6 +325,19 @@ int ct_host_scene_add_material(ct_host_scene* s, const ct_material* m, int32_t k
  * triangle as a light when its material is a Light. */
 int ct_host_scene_add_triangle(ct_host_scene* s, const float p[9], const float n[9],
                                uint32_t material, const float uv[6]);
+/* TriangleList::loadModel (src/Core/TriangleList.h:49, TriangleList.cpp:63-146)
+ * for Wavefront OBJ files: positions mat * (p * scale, 1), normals
+ * mat * (n, 0); mat is a column-major 4x4 (glm layout), NULL = identity;
+ * normalize != 0 first fits the model into [-1, 1] (centred, divided by half
+ * its largest extent).  Polygons become triangle fans with reversed winding,
+ * identical corners are shared, missing normals are generated smooth.
+ * material < 0 uses material 0: MTL materials and textures are not read.
+ * Triangles of a Light material become lights.  CT_ERR_INVALID_ARG if the
+ * file cannot be opened, CT_ERR_INVALID_SCENE if it does not parse;
+ * ct_host_scene_last_error gives the reason. */
+int ct_host_scene_load_model(ct_host_scene* s, const char* path, float scale, const float mat[16], int32_t material,
+                             int32_t normalize);
+const char* ct_host_scene_last_error(const ct_host_scene* s);
 /* BVHTree::constructBVH + MBVHTree::constructBVH. */
 int ct_host_scene_build(ct_host_scene* s);
 int ct_host_scene_view(const ct_host_scene* s, ct_scene_view* out);

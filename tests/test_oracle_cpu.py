@@ -564,3 +564,131 @@ def test_add_material_forces_roughness_like_reference(ct):
     mats = s.arrays()["materials"]
     assert mats[0].roughness == 1.0  # default material 0
     assert mats[i].roughness == 1.0 and abs(mats[j].roughness - 0.05) < 1e-9
+
+
+# ------------------------------------------------------------ OBJ import --
+_CUBE_OBJ = """# unit cube, quads, v/vt/vn and negative indices
+v -1 -1 -1
+v 1 -1 -1
+v 1 1 -1
+v -1 1 -1
+v -1 -1 1
+v 1 -1 1
+v 1 1 1
+v -1 1 1
+vt 0 0
+vt 1 0
+vt 1 1
+vt 0 1
+vn 0 0 -1
+vn 0 0 1
+vn 0 -1 0
+vn 0 1 0
+vn -1 0 0
+vn 1 0 0
+g cube
+s off
+f 1/1/1 4/4/1 3/3/1 2/2/1
+f 5/1/2 6/2/2 7/3/2 8/4/2
+f 1/1/3 2/2/3 6/3/3 5/4/3
+f 4/1/4 8/2/4 7/3/4 3/4/4
+f 1/1/5 5/2/5 8/3/5 4/4/5
+f -7/1/6 -6/2/6 -2/3/6 -3/4/6
+"""
+
+
+def _host_list(ct, s):
+    a = s.arrays()
+    return a["vertices"], a["normals"], a["indices"], a["center_normals"], a["material_idx"], a["light_indices"]
+
+
+def test_obj_import_transform_winding_and_lights(ct, tmp_path):
+    """loadModel: quads split as fans with reversed winding (Assimp's
+    Triangulate + FlipWindingOrder), corners shared, positions
+    mat * (p * scale, 1), normals mat * (n, 0), centre normal = mean of the
+    three, triangles of a Light material registered as lights."""
+    f = tmp_path / "cube.obj"
+    f.write_text(_CUBE_OBJ)
+    s = ct.HostScene.empty()
+    light = s.add_material((2.0, 2.0, 2.0), mtype=ct.MAT_LIGHT)
+    T = np.eye(4, dtype=np.float32)
+    T[:3, 3] = (10.0, -5.0, 2.0)
+    s.load_model(f, scale=3.0, mat=T, material=light, normalize=False)
+    s.build()
+    V, N, I, CN, M, L = _host_list(ct, s)
+    assert I.shape == (12, 3) and V.shape == (24, 3)  # 6 faces x 4 distinct corners
+    assert np.array_equal(L, np.arange(12))
+    assert np.all(M == light)
+    # first face 1 4 3 2 -> fan (1,4,3),(1,3,2) -> reversed (3,4,1),(2,3,1)
+    p = V[I[0]]
+    assert np.allclose(p, [np.array(q) * 3 + (10, -5, 2) for q in ([1, 1, -1], [-1, 1, -1], [-1, -1, -1])])
+    assert np.allclose(N[I[0]], [[0, 0, -1]] * 3)
+    assert np.allclose(CN, N[I].mean(axis=1), atol=1e-7)
+    lo, hi = V.min(0), V.max(0)
+    assert np.allclose(lo, (7, -8, -1)) and np.allclose(hi, (13, -2, 5))
+
+
+def test_obj_import_normalize_and_smooth_normals(ct, tmp_path):
+    """normalize fits the model into [-1, 1] (centred, divided by half the
+    largest extent); a file without normals gets smooth per-position normals
+    (an octahedron's vertex normals point along its axes)."""
+    v = [(0, 1, 4), (0, 1, -2), (3, 1, 1), (-3, 1, 1), (0, 4, 1), (0, -2, 1)]
+    faces = [(1, 3, 5), (1, 5, 4), (1, 4, 6), (1, 6, 3), (2, 5, 3), (2, 4, 5), (2, 6, 4), (2, 3, 6)]
+    f = tmp_path / "octa.obj"
+    f.write_text("".join(f"v {x} {y} {z}\n" for x, y, z in v) + "".join(f"f {a} {b} {c}\n" for a, b, c in faces))
+    s = ct.HostScene.empty()
+    s.load_model(f, normalize=True)
+    s.build()
+    V, N, I, _, M, L = _host_list(ct, s)
+    assert I.shape == (8, 3) and V.shape == (6, 3) and L.size == 0 and np.all(M == 0)
+    assert np.allclose(V.min(0), (-1, -1, -1)) and np.allclose(V.max(0), (1, 1, 1))
+    c = np.array([0.0, 1.0, 1.0])  # centre of the original box, in original units
+    expect = (np.array(v, float) - c) / 3.0
+    assert np.allclose(np.sort(V, axis=0), np.sort(expect, axis=0), atol=1e-6)
+    d = V / np.linalg.norm(V, axis=1, keepdims=True)
+    assert np.allclose(np.linalg.norm(N, axis=1), 1.0, atol=1e-6)
+    assert np.all(np.sum(N * d, axis=1) > 0.99)  # outward along each vertex's axis
+
+
+def test_obj_import_errors(ct, tmp_path):
+    s = ct.HostScene.empty()
+    with pytest.raises(ct.CTError) as e:
+        s.load_model(tmp_path / "missing.obj")
+    assert e.value.code == ct.CT_ERR_INVALID_ARG
+    bad = tmp_path / "bad.obj"
+    bad.write_text("v 0 0 0\nv 1 0 0\nf 1 2 9\n")
+    with pytest.raises(ct.CTError) as e:
+        s.load_model(bad)
+    assert e.value.code == ct.CT_ERR_INVALID_SCENE and "line 3" in str(e.value)
+    with pytest.raises(ct.CTError) as e:  # the list is unchanged: still no triangles
+        s.build()
+    assert e.value.code == ct.CT_ERR_INVALID_SCENE
+
+
+@pytest.mark.skipif(not Path("/root/reference/Models/sphere.obj").exists(),
+                    reason="the reference's model files are only present where the reference is mounted")
+def test_obj_import_reference_sphere(ct, orc):
+    """The reference's demo model Models/sphere.obj (960 faces, 482 vertices,
+    vertex normals): normalised into the unit box it is a closed sphere of
+    radius 1 with outward normals, and the oracle's traversal over the
+    product's trees equals brute force."""
+    s = ct.HostScene.empty()
+    s.load_model("/root/reference/Models/sphere.obj", scale=1.0)
+    s.build()
+    a = s.arrays()
+    V, N, I = a["vertices"], a["normals"], a["indices"]
+    assert I.shape[0] == 960
+    r = np.linalg.norm(V, axis=1)
+    assert np.all(np.abs(r - 1.0) < 0.02)
+    assert np.all(np.sum(N * V, axis=1) > 0.9)
+    osc = orc.Scene(a)
+    rng = np.random.default_rng(3)
+    o = rng.normal(size=(4096, 3))
+    o = (3.0 * o / np.linalg.norm(o, axis=1, keepdims=True)).astype(np.float32)  # on a sphere of radius 3
+    tgt = rng.uniform(-0.5, 0.5, (4096, 3))
+    d = tgt - o
+    d = (d / np.linalg.norm(d, axis=1, keepdims=True)).astype(np.float32)
+    t, p = osc.trace(o, d, 1, tri_eps=1e-6)
+    tb, pb = osc.trace(o, d, 2, tri_eps=1e-6)
+    assert np.array_equal(t, tb) and np.sum(p != pb) <= 2
+    assert np.all(p >= 0)  # every ray aims inside the sphere
