@@ -192,7 +192,7 @@ def test_error_returns(ct, c1):
 
 def _uv_sphere_obj(n_lat: int = 12, n_lon: int = 24) -> str:
     """A UV sphere of radius 1 as OBJ text (positions and normals, quads and
-    triangles at the poles, counter-clockwise seen from outside)."""
+    triangles at the poles)."""
     lines, idx = [], {}
     for i in range(n_lat + 1):
         th = np.pi * i / n_lat
@@ -206,7 +206,12 @@ def _uv_sphere_obj(n_lat: int = 12, n_lon: int = 24) -> str:
         for j in range(n_lon):
             a, b = idx[(i, j)], idx[(i, (j + 1) % n_lon)]
             c, d = idx[(i + 1, (j + 1) % n_lon)], idx[(i + 1, j)]
-            lines.append(f"f {a}//{a} {b}//{b} {c}//{c} {d}//{d}")
+            if i == 0:  # a and b are the pole
+                lines.append(f"f {a}//{a} {c}//{c} {d}//{d}")
+            elif i == n_lat - 1:  # c and d are the pole
+                lines.append(f"f {a}//{a} {b}//{b} {c}//{c}")
+            else:
+                lines.append(f"f {a}//{a} {b}//{b} {c}//{c} {d}//{d}")
     return "\n".join(lines) + "\n"
 
 
