@@ -10,7 +10,10 @@ ray = 28 (origin, direction, tmax) + 8 (t, id) + 64 * quad-node tests + 36 *
 triangle tests; node and triangle tests are counted by an instrumented,
 untimed frame.  achieved = those bytes / the extend launches' device time
 (HIP events on the library stream around every extend launch of the timed
-frames).
+frames).  The any-hit kernel of bounce b runs on a second stream beside the
+extend of bounce b + 1, so `traversal_phase` also reports both kernels'
+algorithmic bytes (shadow ray: 48 + 64 * node tests + 36 * triangle tests)
+over the frame's time outside shading.
 
 cpu_baseline: the CPU oracle (FP32 restatement of the reference loop, OpenMP
 over pixels) on whole frames of the same workload, about 10 s of CPU work.
@@ -105,6 +108,8 @@ def main() -> int:
     ctx.reset()
     closest_rays = cst.primary_rays + cst.extension_rays
     alg_bytes_frame = 36 * closest_rays + 64 * work[0] + 36 * work[1]
+    # shadow ray: 28 in + 12 colour + 4 pixel + 4 result, plus nodes and triangles
+    alg_bytes_shadow = 48 * cst.shadow_rays + 64 * work[2] + 36 * work[3]
 
     for _ in range(args.warmup):
         ctx.trace_frame(cam, opts)
@@ -150,6 +155,18 @@ def main() -> int:
         "node_tests_per_ray": round(work[0] / max(closest_rays, 1), 2),
         "tri_tests_per_ray": round(work[1] / max(closest_rays, 1), 2),
     }
+    # k_connect(b) runs on a second stream beside k_extend(b + 1), so a launch's
+    # duration includes time shared with the other kernel.  Both kernels'
+    # algorithmic bytes over the frame time minus shade and the small stages:
+    phase_ms = (gpu_ms - shade_ms - other_ms) / args.steps
+    shadow_per_frame = sh / args.steps
+    phase_bytes = bytes_per_frame + alg_bytes_shadow * (shadow_per_frame / max(cst.shadow_rays, 1))
+    phase = phase_bytes / (phase_ms * 1e-3) / 1e9 if phase_ms > 0 else 0.0
+    roofline["traversal_phase"] = {
+        "kernels": "k_extend + k_connect (overlapping streams)", "achieved": round(phase, 1),
+        "frac": round(phase / HBM_PEAK_GBS, 4), "ms_per_frame": round(phase_ms, 3),
+        "shadow_node_tests_per_ray": round(work[2] / max(cst.shadow_rays, 1), 2),
+        "shadow_tri_tests_per_ray": round(work[3] / max(cst.shadow_rays, 1), 2)}
 
     out = {
         "metric": METRIC,

@@ -233,6 +233,7 @@ int ct_scene_upload(ct_ctx* c, const ct_scene_view* v) {
     if (ct::build_wide_nodes(*v, fn, wide_bound)) return fail(c, CT_ERR_INVALID_SCENE, "tree bounds are not finite");
 
     HIPCHK(hipStreamSynchronize(c->stream), "scene upload");
+    HIPCHK(hipStreamSynchronize(c->side), "scene upload");
     free_scene(c);
     ctd::DevScene sc{};
     int r = upload_vec(c, tris, &sc.tris, "triangles");
@@ -300,6 +301,7 @@ int ct_resize(ct_ctx* c, int32_t width, int32_t height) {
         return fail(c, CT_ERR_INVALID_ARG, "bad frame size");
     (void)hipSetDevice(c->device);
     HIPCHK(hipStreamSynchronize(c->stream), "resize");
+    HIPCHK(hipStreamSynchronize(c->side), "resize");
     dfree(c, c->accum);
     c->accum = nullptr;
     c->graph_key.clear();

@@ -22,6 +22,11 @@ struct ChunkStats {
 struct ct_ctx {
     int device = 0;
     hipStream_t stream = nullptr;
+    // connect(b) runs here, beside extend(b + 1) on `stream`; fork / join
+    // events order it after shade(b) and before shade(b + 1)
+    hipStream_t side = nullptr;
+    hipEvent_t fork[ctd::kMaxBounce] = {}, join[ctd::kMaxBounce] = {};
+    uint32_t* spill_side = nullptr;  // stack spill area of the side stream's traversal
     std::string err;
     std::string name;
     int num_cus = 0;

@@ -97,6 +97,8 @@ int validate_opts(ct_ctx* c, const ct_frame_opts& o) {
 }
 
 static void free_paths(ct_ctx* c) {
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    if (c->side) (void)hipStreamSynchronize(c->side);
     for (void* p : c->path_bufs) dfree(c, p);
     c->path_bufs.clear();
     c->q = ctd::Queues{};
@@ -131,6 +133,7 @@ int ensure_paths(ct_ctx* c, uint32_t cap) {
     if (r == CT_OK) r = A((void**)&c->q.counters, ctd::kCounterWords * 4, "counters");
     if (r == CT_OK) r = A((void**)&c->q.err, 64, "error flags");
     if (r == CT_OK) r = A((void**)&c->q.spill, (size_t)threads * ctd::kMaxStack * 4, "stack spill");
+    if (r == CT_OK) r = A((void**)&c->spill_side, (size_t)threads * ctd::kMaxStack * 4, "stack spill (side)");
     if (r == CT_OK) r = A((void**)&c->wstats, 64, "work counters");
     if (r == CT_OK) r = A((void**)&c->d_hdr, sizeof(ChunkHdr), "chunk header");
     if (r != CT_OK) {
@@ -250,22 +253,40 @@ static hipError_t enqueue_chunk(ct_ctx* c, float4* accum, const ct_frame_opts& o
     if ((r = ctk::launch_generate(c->shape, c->d_views, c->d_segs, c->d_hdr, c->cap, c->q, s)) != hipSuccess)
         return r;
     rec();
+    // connect(b) on the side stream overlaps extend(b + 1): both read queues
+    // shade(b) wrote, and only connect and shade add to a path's radiance, so
+    // shade(b + 1) and resolve wait for connect(b) (the additions keep their
+    // order).  Its traversal stacks spill to their own area.
+    const bool nee = o.shadow && c->sc.n_lights > 0;
+    ctd::Queues qside = c->q;
+    qside.spill = c->spill_side;
     int sel = 0;
     for (int b = 0; b < o.max_depth; ++b) {
         rec();
         if ((r = ctk::launch_extend(c->shape, c->sc, to, c->q, sel, b, ws, s)) != hipSuccess) return r;
         rec();
+        if (nee && b > 0 && (r = hipStreamWaitEvent(s, c->join[b - 1], 0)) != hipSuccess) return r;
         rec();
         if ((r = ctk::launch_shade(c->shape, c->sc, so, c->d_views, c->d_segs, c->d_hdr, c->q, sel, b, s)) !=
             hipSuccess)
             return r;
         rec();
-        rec();
-        if (o.shadow && c->sc.n_lights > 0)
-            if ((r = ctk::launch_connect(c->shape, c->sc, to, c->q, b, ws, s)) != hipSuccess) return r;
-        rec();
+        if (nee) {
+            if ((r = hipEventRecord(c->fork[b], s)) != hipSuccess) return r;
+            if ((r = hipStreamWaitEvent(c->side, c->fork[b], 0)) != hipSuccess) return r;
+            if (timed) (void)hipEventRecord(c->ev[e], c->side);
+            ++e;
+            if ((r = ctk::launch_connect(c->shape, c->sc, to, qside, b, ws, c->side)) != hipSuccess) return r;
+            if (timed) (void)hipEventRecord(c->ev[e], c->side);
+            ++e;
+            if ((r = hipEventRecord(c->join[b], c->side)) != hipSuccess) return r;
+        } else {
+            rec();
+            rec();
+        }
         sel ^= 1;
     }
+    if (nee && (r = hipStreamWaitEvent(s, c->join[o.max_depth - 1], 0)) != hipSuccess) return r;
     rec();
     if ((r = ctk::launch_resolve(c->shape, c->d_views, c->d_segs, c->d_hdr, c->cap, c->q.lpath, accum, s)) !=
         hipSuccess)
@@ -473,6 +494,13 @@ int ct_ctx_create(const ct_device_opts* opts, ct_ctx** out) {
     if (hipMemGetInfo(&freeb, &totalb) != hipSuccess) { delete c; return CT_ERR_DEVICE; }
     c->budget = d.mem_budget ? d.mem_budget : (uint64_t)(0.9 * (double)freeb);
     if (hipStreamCreate(&c->stream) != hipSuccess) { delete c; return CT_ERR_DEVICE; }
+    if (hipStreamCreate(&c->side) != hipSuccess) { (void)hipStreamDestroy(c->stream); delete c; return CT_ERR_DEVICE; }
+    for (int b = 0; b < ctd::kMaxBounce; ++b)
+        if (hipEventCreateWithFlags(&c->fork[b], hipEventDisableTiming) != hipSuccess ||
+            hipEventCreateWithFlags(&c->join[b], hipEventDisableTiming) != hipSuccess) {
+            delete c;
+            return CT_ERR_DEVICE;
+        }
     for (auto& ev : c->ev)
         if (hipEventCreate(&ev) != hipSuccess) { delete c; return CT_ERR_DEVICE; }
     const int block = d.block_size > 0 ? d.block_size : 256;
@@ -488,11 +516,17 @@ void ct_ctx_destroy(ct_ctx* c) {
     if (!c) return;
     (void)hipSetDevice(c->device);
     if (c->stream) (void)hipStreamSynchronize(c->stream);
+    if (c->side) (void)hipStreamSynchronize(c->side);
     drop_graph(c);
     for (auto& a : c->allocs) (void)hipFree(a.first);
     c->allocs.clear();
     for (auto& ev : c->ev)
         if (ev) (void)hipEventDestroy(ev);
+    for (int b = 0; b < ctd::kMaxBounce; ++b) {
+        if (c->fork[b]) (void)hipEventDestroy(c->fork[b]);
+        if (c->join[b]) (void)hipEventDestroy(c->join[b]);
+    }
+    if (c->side) (void)hipStreamDestroy(c->side);
     if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }
