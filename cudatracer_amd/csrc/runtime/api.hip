@@ -135,7 +135,7 @@ int upload_vec(ct_ctx* c, const std::vector<T>& h, const float4** dst, const cha
     const int r = dalloc(c, &p, h.size() * sizeof(T), what);
     if (r != CT_OK) return r;
     c->scene_bufs.push_back(p);
-    if (!h.empty()) HIPCHK(hipMemcpyAsync(p, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice, c->stream), what);
+    if (!h.empty()) HIPCHK(hipMemcpyAsync(p, h.data(), h.size() * sizeof(T), hipMemcpyHostToDevice, c->ln[0].stream), what);
     *dst = reinterpret_cast<const float4*>(p);
     return CT_OK;
 }
@@ -232,8 +232,7 @@ int ct_scene_upload(ct_ctx* c, const ct_scene_view* v) {
     uint32_t wide_bound = 0;
     if (ct::build_wide_nodes(*v, fn, wide_bound)) return fail(c, CT_ERR_INVALID_SCENE, "tree bounds are not finite");
 
-    HIPCHK(hipStreamSynchronize(c->stream), "scene upload");
-    HIPCHK(hipStreamSynchronize(c->side), "scene upload");
+    sync_lanes(c);
     free_scene(c);
     ctd::DevScene sc{};
     int r = upload_vec(c, tris, &sc.tris, "triangles");
@@ -249,7 +248,7 @@ int ct_scene_upload(ct_ctx* c, const ct_scene_view* v) {
     }
     if (r == CT_OK && !n2.empty()) r = upload_vec(c, n2, &sc.nodes2, "binary nodes");
     if (r != CT_OK) { free_scene(c); return r; }
-    HIPCHK(hipStreamSynchronize(c->stream), "scene upload");
+    HIPCHK(hipStreamSynchronize(c->ln[0].stream), "scene upload");
     sc.n_tris = N;
     sc.n_nodes4 = v->n_mbvh_nodes;
     sc.n_nodes2 = v->n_bvh_nodes;
@@ -270,7 +269,7 @@ int ct_scene_upload(ct_ctx* c, const ct_scene_view* v) {
     c->h_light_idx.assign(v->light_indices, v->light_indices + v->n_lights);
     c->h_light_rows.assign(reinterpret_cast<const float*>(lights.data()),
                            reinterpret_cast<const float*>(lights.data()) + lights.size() * 4);
-    c->graph_key.clear();
+    drop_graphs(c);
     return CT_OK;
 }
 
@@ -283,15 +282,15 @@ int ct_material_update(ct_ctx* c, uint32_t index, const ct_material* mat) {
     float4 rows[2];
     mat_rows(*mat, rows);
     HIPCHK(hipMemcpyAsync((void*)(c->sc.mats + 2 * (size_t)index), rows, sizeof(rows), hipMemcpyHostToDevice,
-                          c->stream),
+                          c->ln[0].stream),
            "material update");
     // lights whose triangle uses this material carry its emission
     const float4 e = make_float4(mat->albedo[0], mat->albedo[1], mat->albedo[2], 0.0f);
     for (size_t l = 0; l < c->h_light_idx.size(); ++l)
         if (c->h_mat_idx[c->h_light_idx[l]] == index)
-            HIPCHK(hipMemcpyAsync((void*)(c->sc.lights + 5 * l + 4), &e, sizeof(e), hipMemcpyHostToDevice, c->stream),
+            HIPCHK(hipMemcpyAsync((void*)(c->sc.lights + 5 * l + 4), &e, sizeof(e), hipMemcpyHostToDevice, c->ln[0].stream),
                    "light emission update");
-    HIPCHK(hipStreamSynchronize(c->stream), "material update");
+    HIPCHK(hipStreamSynchronize(c->ln[0].stream), "material update");
     return CT_OK;
 }
 
@@ -300,11 +299,10 @@ int ct_resize(ct_ctx* c, int32_t width, int32_t height) {
     if (width <= 0 || height <= 0 || (int64_t)width * height > (1ll << 28))
         return fail(c, CT_ERR_INVALID_ARG, "bad frame size");
     (void)hipSetDevice(c->device);
-    HIPCHK(hipStreamSynchronize(c->stream), "resize");
-    HIPCHK(hipStreamSynchronize(c->side), "resize");
+    sync_lanes(c);
     dfree(c, c->accum);
     c->accum = nullptr;
-    c->graph_key.clear();
+    drop_graphs(c);
     const int r = dalloc(c, (void**)&c->accum, (size_t)width * height * sizeof(float4), "accumulator");
     if (r != CT_OK) { c->width = c->height = 0; return r; }
     c->width = width;
@@ -316,8 +314,8 @@ int ct_reset(ct_ctx* c) {
     if (!c) return CT_ERR_INVALID_ARG;
     if (!c->accum) return fail(c, CT_ERR_INVALID_ARG, "no frame buffer (call ct_resize)");
     (void)hipSetDevice(c->device);
-    HIPCHK(hipMemsetAsync(c->accum, 0, (size_t)c->width * c->height * sizeof(float4), c->stream), "reset");
-    HIPCHK(hipStreamSynchronize(c->stream), "reset");
+    HIPCHK(hipMemsetAsync(c->accum, 0, (size_t)c->width * c->height * sizeof(float4), c->ln[0].stream), "reset");
+    HIPCHK(hipStreamSynchronize(c->ln[0].stream), "reset");
     c->samples = 0;
     return CT_OK;
 }
@@ -400,8 +398,8 @@ int ct_read_accum(ct_ctx* c, float* rgba, size_t n_floats, int32_t* samples) {
     const size_t need = (size_t)c->width * c->height * 4;
     if (n_floats < need) return fail(c, CT_ERR_INVALID_ARG, "output buffer too small");
     (void)hipSetDevice(c->device);
-    HIPCHK(hipMemcpyAsync(rgba, c->accum, need * 4, hipMemcpyDeviceToHost, c->stream), "read accumulator");
-    HIPCHK(hipStreamSynchronize(c->stream), "read accumulator");
+    HIPCHK(hipMemcpyAsync(rgba, c->accum, need * 4, hipMemcpyDeviceToHost, c->ln[0].stream), "read accumulator");
+    HIPCHK(hipStreamSynchronize(c->ln[0].stream), "read accumulator");
     if (samples) *samples = c->samples;
     return CT_OK;
 }
@@ -412,8 +410,8 @@ int ct_write_accum(ct_ctx* c, const float* rgba, size_t n_floats, int32_t sample
     const size_t need = (size_t)c->width * c->height * 4;
     if (n_floats < need) return fail(c, CT_ERR_INVALID_ARG, "input buffer too small");
     (void)hipSetDevice(c->device);
-    HIPCHK(hipMemcpyAsync(c->accum, rgba, need * 4, hipMemcpyHostToDevice, c->stream), "write accumulator");
-    HIPCHK(hipStreamSynchronize(c->stream), "write accumulator");
+    HIPCHK(hipMemcpyAsync(c->accum, rgba, need * 4, hipMemcpyHostToDevice, c->ln[0].stream), "write accumulator");
+    HIPCHK(hipStreamSynchronize(c->ln[0].stream), "write accumulator");
     c->samples = samples;
     return CT_OK;
 }
@@ -427,9 +425,9 @@ int ct_read_image(ct_ctx* c, float* rgba, size_t n_floats) {
     void* tmp = nullptr;
     int r = dalloc(c, &tmp, npix * sizeof(float4), "image");
     if (r != CT_OK) return r;
-    hipError_t e = ctk::launch_tonemap(c->accum, (uint32_t)npix, (float)std::max(c->samples, 1), (float4*)tmp, c->stream);
-    if (e == hipSuccess) e = hipMemcpyAsync(rgba, tmp, npix * sizeof(float4), hipMemcpyDeviceToHost, c->stream);
-    if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+    hipError_t e = ctk::launch_tonemap(c->accum, (uint32_t)npix, (float)std::max(c->samples, 1), (float4*)tmp, c->ln[0].stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(rgba, tmp, npix * sizeof(float4), hipMemcpyDeviceToHost, c->ln[0].stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->ln[0].stream);
     dfree(c, tmp);
     if (e != hipSuccess) return hip_fail(c, e, "tonemap");
     return CT_OK;
@@ -464,17 +462,17 @@ int ct_trace_rays(ct_ctx* c, const float* origins, const float* dirs, uint32_t n
             rd[i] = f4(dirs + 3 * (size_t)(base + i), 0.0f);
         }
         const ctd::ChunkHdr h{m, 0u, 0u, 0u};
-        HIPCHK(hipMemcpyAsync(c->q.ray_o[0], ro.data(), m * sizeof(float4), hipMemcpyHostToDevice, c->stream), "rays");
-        HIPCHK(hipMemcpyAsync(c->q.ray_d[0], rd.data(), m * sizeof(float4), hipMemcpyHostToDevice, c->stream), "rays");
-        HIPCHK(hipMemcpyAsync(c->d_hdr, &h, sizeof(h), hipMemcpyHostToDevice, c->stream), "header");
-        HIPCHK(ctk::launch_setup(c->q.counters, c->d_hdr, c->stream), "setup");
-        HIPCHK(ctk::launch_extend(c->shape, c->sc, to, c->q, 0, 0, nullptr, c->stream), "extend");
+        HIPCHK(hipMemcpyAsync(c->ln[0].q.ray_o[0], ro.data(), m * sizeof(float4), hipMemcpyHostToDevice, c->ln[0].stream), "rays");
+        HIPCHK(hipMemcpyAsync(c->ln[0].q.ray_d[0], rd.data(), m * sizeof(float4), hipMemcpyHostToDevice, c->ln[0].stream), "rays");
+        HIPCHK(hipMemcpyAsync(c->ln[0].d_hdr, &h, sizeof(h), hipMemcpyHostToDevice, c->ln[0].stream), "header");
+        HIPCHK(ctk::launch_setup(c->ln[0].q.counters, c->ln[0].d_hdr, c->ln[0].stream), "setup");
+        HIPCHK(ctk::launch_extend(c->shape, c->sc, to, c->ln[0].q, 0, 0, nullptr, c->ln[0].stream), "extend");
         uint32_t errflag = 0;
-        HIPCHK(hipMemcpyAsync(hits.data(), c->q.hit, m * sizeof(float2), hipMemcpyDeviceToHost, c->stream), "hits");
-        HIPCHK(hipMemcpyAsync(&errflag, c->q.err, 4, hipMemcpyDeviceToHost, c->stream), "flags");
-        HIPCHK(hipStreamSynchronize(c->stream), "trace rays");
+        HIPCHK(hipMemcpyAsync(hits.data(), c->ln[0].q.hit, m * sizeof(float2), hipMemcpyDeviceToHost, c->ln[0].stream), "hits");
+        HIPCHK(hipMemcpyAsync(&errflag, c->ln[0].q.err, 4, hipMemcpyDeviceToHost, c->ln[0].stream), "flags");
+        HIPCHK(hipStreamSynchronize(c->ln[0].stream), "trace rays");
         if (errflag) {
-            (void)hipMemsetAsync(c->q.err, 0, 4, c->stream);
+            (void)hipMemsetAsync(c->ln[0].q.err, 0, 4, c->ln[0].stream);
             return fail(c, CT_ERR_STACK_OVERFLOW, "traversal stack bound exceeded");
         }
         for (uint32_t i = 0; i < m; ++i) {
@@ -512,20 +510,20 @@ int ct_trace_shadow_rays(ct_ctx* c, const float* origins, const float* dirs, con
             sd[i] = f4(dirs + 3 * (size_t)(base + i), 0.0f);
         }
         const ctd::ChunkHdr h{m, 0u, 0u, 0u};
-        e = hipMemcpyAsync(c->q.sh_o, so.data(), m * sizeof(float4), hipMemcpyHostToDevice, c->stream);
-        if (e == hipSuccess) e = hipMemcpyAsync(c->q.sh_d, sd.data(), m * sizeof(float4), hipMemcpyHostToDevice, c->stream);
-        if (e == hipSuccess) e = hipMemcpyAsync(c->d_hdr, &h, sizeof(h), hipMemcpyHostToDevice, c->stream);
-        if (e == hipSuccess) e = ctk::launch_setup(c->q.counters, c->d_hdr, c->stream);
-        if (e == hipSuccess) e = ctk::launch_shadow_flags(c->shape, c->sc, to, c->q, m, (uint8_t*)dout, c->stream);
-        if (e == hipSuccess) e = hipMemcpyAsync(occluded_out + base, dout, m, hipMemcpyDeviceToHost, c->stream);
-        if (e == hipSuccess) e = hipMemcpyAsync(&errflag, c->q.err, 4, hipMemcpyDeviceToHost, c->stream);
-        if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+        e = hipMemcpyAsync(c->ln[0].q.sh_o, so.data(), m * sizeof(float4), hipMemcpyHostToDevice, c->ln[0].stream);
+        if (e == hipSuccess) e = hipMemcpyAsync(c->ln[0].q.sh_d, sd.data(), m * sizeof(float4), hipMemcpyHostToDevice, c->ln[0].stream);
+        if (e == hipSuccess) e = hipMemcpyAsync(c->ln[0].d_hdr, &h, sizeof(h), hipMemcpyHostToDevice, c->ln[0].stream);
+        if (e == hipSuccess) e = ctk::launch_setup(c->ln[0].q.counters, c->ln[0].d_hdr, c->ln[0].stream);
+        if (e == hipSuccess) e = ctk::launch_shadow_flags(c->shape, c->sc, to, c->ln[0].q, m, (uint8_t*)dout, c->ln[0].stream);
+        if (e == hipSuccess) e = hipMemcpyAsync(occluded_out + base, dout, m, hipMemcpyDeviceToHost, c->ln[0].stream);
+        if (e == hipSuccess) e = hipMemcpyAsync(&errflag, c->ln[0].q.err, 4, hipMemcpyDeviceToHost, c->ln[0].stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(c->ln[0].stream);
         if (errflag) break;
     }
     dfree(c, dout);
     if (e != hipSuccess) return hip_fail(c, e, "trace shadow rays");
     if (errflag) {
-        (void)hipMemsetAsync(c->q.err, 0, 4, c->stream);
+        (void)hipMemsetAsync(c->ln[0].q.err, 0, 4, c->ln[0].stream);
         return fail(c, CT_ERR_STACK_OVERFLOW, "traversal stack bound exceeded");
     }
     return CT_OK;
@@ -540,20 +538,20 @@ int ct_trace_rays_device(ct_ctx* c, const void* d_rays_o, const void* d_rays_d, 
     (void)hipSetDevice(c->device);
     r = ensure_paths(c, 64);
     if (r != CT_OK) return r;
-    ctd::Queues q = c->q;
+    ctd::Queues q = c->ln[0].q;
     q.ray_o[0] = (float4*)d_rays_o;
     q.ray_d[0] = (float4*)d_rays_d;
     q.hit = (float2*)d_hits;
     ctd::TraceOpts to{o.accel, o.traversal, o.triangle_epsilon};
     const ctd::ChunkHdr h{n, 0u, 0u, 0u};
-    HIPCHK(hipMemcpyAsync(c->d_hdr, &h, sizeof(h), hipMemcpyHostToDevice, c->stream), "header");
-    HIPCHK(ctk::launch_setup(c->q.counters, c->d_hdr, c->stream), "setup");
-    HIPCHK(hipEventRecord(c->ev[0], c->stream), "event");
-    HIPCHK(ctk::launch_extend(c->shape, c->sc, to, q, 0, 0, nullptr, c->stream), "extend");
-    HIPCHK(hipEventRecord(c->ev[1], c->stream), "event");
-    HIPCHK(hipStreamSynchronize(c->stream), "trace rays");
+    HIPCHK(hipMemcpyAsync(c->ln[0].d_hdr, &h, sizeof(h), hipMemcpyHostToDevice, c->ln[0].stream), "header");
+    HIPCHK(ctk::launch_setup(c->ln[0].q.counters, c->ln[0].d_hdr, c->ln[0].stream), "setup");
+    HIPCHK(hipEventRecord(c->ln[0].ev[0], c->ln[0].stream), "event");
+    HIPCHK(ctk::launch_extend(c->shape, c->sc, to, q, 0, 0, nullptr, c->ln[0].stream), "extend");
+    HIPCHK(hipEventRecord(c->ln[0].ev[1], c->ln[0].stream), "event");
+    HIPCHK(hipStreamSynchronize(c->ln[0].stream), "trace rays");
     float ms = 0;
-    HIPCHK(hipEventElapsedTime(&ms, c->ev[0], c->ev[1]), "elapsed");
+    HIPCHK(hipEventElapsedTime(&ms, c->ln[0].ev[0], c->ln[0].ev[1]), "elapsed");
     if (kernel_ms) *kernel_ms = ms;
     return CT_OK;
 }
@@ -585,7 +583,7 @@ int ct_render_batch(ct_ctx* c, const ct_view* views, int32_t n_views, const ct_f
     float4* arena = nullptr;
     r = dalloc(c, (void**)&arena, pix * sizeof(float4), "batch accumulators");
     if (r != CT_OK) return r;
-    hipError_t e = hipMemsetAsync(arena, 0, pix * sizeof(float4), c->stream);
+    hipError_t e = hipMemsetAsync(arena, 0, pix * sizeof(float4), c->ln[0].stream);
     if (e != hipSuccess) { dfree(c, arena); return hip_fail(c, e, "clear batch accumulators"); }
     const uint32_t cap = (uint32_t)std::min<uint64_t>(total, c->chunk_default);
     ChunkStats st;
@@ -596,12 +594,10 @@ int ct_render_batch(ct_ctx* c, const ct_view* views, int32_t n_views, const ct_f
         for (int32_t i = 0; i < n_views && e == hipSuccess; ++i)
             e = hipMemcpyAsync(views[i].accum_out, arena + vd[i].accum_offset,
                                (size_t)views[i].width * views[i].height * sizeof(float4), hipMemcpyDeviceToHost,
-                               c->stream);
-        if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+                               c->ln[0].stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(c->ln[0].stream);
         if (e != hipSuccess) r = hip_fail(c, e, "read batch accumulators");
     }
-    // the captured graph refers to the arena; drop it with the arena
-    if (c->graph_exec) { (void)hipGraphExecDestroy(c->graph_exec); c->graph_exec = nullptr; c->graph_key.clear(); }
     dfree(c, arena);
     if (r != CT_OK) return r;
     if (stats) {
@@ -610,7 +606,7 @@ int ct_render_batch(ct_ctx* c, const ct_view* views, int32_t n_views, const ct_f
         stats->shadow_rays = st.shadow;
         stats->samples = st.samples;
         stats->gpu_ms = st.total_ms;
-        stats->chunks = (int32_t)((total + cap - 1) / cap);
+        stats->chunks = st.chunks;
     }
     return CT_OK;
 }

@@ -17,16 +17,51 @@ struct ChunkStats {
     double extend_ms = 0, shade_ms = 0, connect_ms = 0, other_ms = 0, total_ms = 0;
     uint64_t work[4] = {0, 0, 0, 0};  // closest-hit node/tri tests, shadow node/tri tests
     int extend_launches = 0;
+    int chunks = 0;
 };
 
-struct ct_ctx {
-    int device = 0;
+// One of the two pipelines the chunk driver alternates between.  Each lane
+// has its own queue views (a half of the context's path buffers), counters,
+// chunk tables, streams, events and captured bounce loop, so the chunks of
+// the two lanes run concurrently and fill each other's launch tails.
+struct Lane {
+    ctd::Queues q{};
     hipStream_t stream = nullptr;
     // connect(b) runs here, beside extend(b + 1) on `stream`; fork / join
     // events order it after shade(b) and before shade(b + 1)
     hipStream_t side = nullptr;
     hipEvent_t fork[ctd::kMaxBounce] = {}, join[ctd::kMaxBounce] = {};
+    hipEvent_t resolved = nullptr;   // after this lane's latest resolve
     uint32_t* spill_side = nullptr;  // stack spill area of the side stream's traversal
+    unsigned long long* wstats = nullptr;
+    ctd::ViewDesc* d_views = nullptr;
+    uint32_t d_views_cap = 0;
+    ctd::Segment* d_segs = nullptr;
+    uint32_t d_segs_cap = 0;
+    ctd::ChunkHdr* d_hdr = nullptr;
+    uint32_t* h_read = nullptr;      // pinned read-back: counters, error flag, work counters
+    uint8_t* h_stage = nullptr;      // pinned upload staging of the chunk tables
+    size_t h_stage_bytes = 0;
+    // captured bounce loop
+    hipGraphExec_t graph_exec = nullptr;
+    std::string graph_key;
+    hipEvent_t ev[2 * (3 * ctd::kMaxBounce + 8)] = {};
+    int n_events = 0;
+    // the chunk in flight (launched, not yet finished)
+    bool pending = false;
+    uint32_t n_paths = 0;
+    int max_depth = 0;
+    bool timed = false, count = false;
+};
+// h_read layout (uint32 words)
+constexpr int kReadCounters = 2 * (ctd::kMaxBounce + 1);
+constexpr int kReadErr = kReadCounters;
+constexpr int kReadWork = kReadCounters + 2;  // 4 x uint64, 8-byte aligned
+constexpr int kReadWords = kReadWork + 8;
+
+struct ct_ctx {
+    int device = 0;
+    Lane ln[2];
     std::string err;
     std::string name;
     int num_cus = 0;
@@ -51,24 +86,10 @@ struct ct_ctx {
     int32_t samples = 0;
     float4* accum = nullptr;
 
-    // path state
-    uint32_t cap = 0;
-    ctd::Queues q{};
+    // path state: buffers of 2 * lane_cap slots, lane k uses [k * lane_cap, (k + 1) * lane_cap)
+    uint32_t lane_cap = 0;
     std::vector<void*> path_bufs;
     uint32_t spill_threads = 0;
-    unsigned long long* wstats = nullptr;
-    ctd::ViewDesc* d_views = nullptr;
-    uint32_t d_views_cap = 0;
-    ctd::Segment* d_segs = nullptr;
-    uint32_t d_segs_cap = 0;
-    ctd::ChunkHdr* d_hdr = nullptr;
-
-    // captured bounce loop
-    hipGraphExec_t graph_exec = nullptr;
-    std::string graph_key;
-
-    hipEvent_t ev[2 * (3 * ctd::kMaxBounce + 8)] = {};
-    int n_events = 0;
 };
 
 namespace ctr {
@@ -79,19 +100,20 @@ int dalloc(ct_ctx* c, void** p, size_t bytes, const char* what);
 void dfree(ct_ctx* c, void* p);
 
 int validate_opts(ct_ctx* c, const ct_frame_opts& o);
+// Path buffers for chunks of up to cap paths on each lane.
 int ensure_paths(ct_ctx* c, uint32_t cap);
-int ensure_tables(ct_ctx* c, uint32_t nviews, uint32_t nsegs);
+int ensure_tables(ct_ctx* c, Lane& L, uint32_t nviews, uint32_t nsegs);
+void drop_graphs(ct_ctx* c);
+// Wait for both lanes' streams (side streams included).
+void sync_lanes(ct_ctx* c);
 ctd::ViewDesc make_view(const ct_camera& cam, int width, int height, int spp, uint32_t frame_base,
                         uint64_t accum_offset);
 
-// Run one chunk: views/segments already describe the chunk (host copies are
-// uploaded here).  use_graph replays a captured bounce loop.
-int run_chunk(ct_ctx* c, const std::vector<ctd::ViewDesc>& views, const std::vector<ctd::Segment>& segs,
-              uint32_t n, uint32_t n_tasks, float4* accum, const ct_frame_opts& o, bool timed, bool count,
-              bool use_graph, ChunkStats* st);
-
 // Split a list of (view, first_path, n) runs into chunks of at most cap paths
-// and run them in order.
+// and run them in order, alternating between the two lanes (a workload that
+// fits one chunk is split in two so both lanes work on it).  Work the caller
+// queued on lane 0's stream before the call is ordered before all of it; both
+// lanes are idle when it returns.  Accumulator updates keep chunk order.
 struct Run {
     uint32_t view, first_path, n;
 };

@@ -69,10 +69,23 @@ void dfree(ct_ctx* c, void* p) {
     (void)hipFree(p);
 }
 
-static void drop_graph(ct_ctx* c) {
-    if (c->graph_exec) (void)hipGraphExecDestroy(c->graph_exec);
-    c->graph_exec = nullptr;
-    c->graph_key.clear();
+static void drop_lane_graph(Lane& L) {
+    if (L.graph_exec) (void)hipGraphExecDestroy(L.graph_exec);
+    L.graph_exec = nullptr;
+    L.graph_key.clear();
+}
+
+// Callers have both lanes idle.
+void drop_graphs(ct_ctx* c) {
+    for (Lane& L : c->ln) drop_lane_graph(L);
+}
+
+void sync_lanes(ct_ctx* c) {
+    for (Lane& L : c->ln) {
+        if (L.stream) (void)hipStreamSynchronize(L.stream);
+        if (L.side) (void)hipStreamSynchronize(L.side);
+        L.pending = false;
+    }
 }
 
 int validate_opts(ct_ctx* c, const ct_frame_opts& o) {
@@ -97,73 +110,105 @@ int validate_opts(ct_ctx* c, const ct_frame_opts& o) {
 }
 
 static void free_paths(ct_ctx* c) {
-    if (c->stream) (void)hipStreamSynchronize(c->stream);
-    if (c->side) (void)hipStreamSynchronize(c->side);
+    sync_lanes(c);
     for (void* p : c->path_bufs) dfree(c, p);
     c->path_bufs.clear();
-    c->q = ctd::Queues{};
-    c->cap = 0;
-    drop_graph(c);
+    for (Lane& L : c->ln) {
+        if (L.h_read) (void)hipHostFree(L.h_read);
+        L.h_read = nullptr;
+        L.q = ctd::Queues{};
+        L.wstats = nullptr;
+        L.spill_side = nullptr;
+        L.d_hdr = nullptr;
+    }
+    c->lane_cap = 0;
+    drop_graphs(c);
 }
 
 int ensure_paths(ct_ctx* c, uint32_t cap) {
     cap = std::max<uint32_t>(cap, 64u);
     const uint32_t threads = (uint32_t)(c->shape.trace_blocks * c->shape.block);
-    if (c->cap >= cap && c->spill_threads >= threads && c->q.counters) return CT_OK;
-    cap = std::max(cap, c->cap);
+    if (c->lane_cap >= cap && c->spill_threads >= threads && c->ln[0].q.counters) return CT_OK;
+    cap = std::max(cap, c->lane_cap);
     free_paths(c);
     auto A = [&](void** p, size_t bytes, const char* w) -> int {
         const int r = dalloc(c, p, bytes, w);
         if (r == CT_OK) c->path_bufs.push_back(*p);
         return r;
     };
-    const size_t f4 = (size_t)cap * sizeof(float4);
+    // path buffers shared by the lanes (lane k at offset k * cap)
+    const size_t slots = 2 * (size_t)cap;
+    const size_t f4 = slots * sizeof(float4);
+    ctd::Queues q{};
     int r = CT_OK;
     for (int k = 0; k < 2 && r == CT_OK; ++k) {
-        r = A((void**)&c->q.ray_o[k], f4, "ray origins");
-        if (r == CT_OK) r = A((void**)&c->q.ray_d[k], f4, "ray directions");
-        if (r == CT_OK) r = A((void**)&c->q.thr[k], f4, "throughput");
-        if (r == CT_OK) r = A((void**)&c->q.path[k], (size_t)cap * 4, "path words");
+        r = A((void**)&q.ray_o[k], f4, "ray origins");
+        if (r == CT_OK) r = A((void**)&q.ray_d[k], f4, "ray directions");
+        if (r == CT_OK) r = A((void**)&q.thr[k], f4, "throughput");
+        if (r == CT_OK) r = A((void**)&q.path[k], slots * 4, "path words");
     }
-    if (r == CT_OK) r = A((void**)&c->q.hit, (size_t)cap * sizeof(float2), "hits");
-    if (r == CT_OK) r = A((void**)&c->q.sh_o, f4, "shadow origins");
-    if (r == CT_OK) r = A((void**)&c->q.sh_d, f4, "shadow directions");
-    if (r == CT_OK) r = A((void**)&c->q.sh_c, f4, "shadow colours");
-    if (r == CT_OK) r = A((void**)&c->q.lpath, f4, "path radiance");
-    if (r == CT_OK) r = A((void**)&c->q.counters, ctd::kCounterWords * 4, "counters");
-    if (r == CT_OK) r = A((void**)&c->q.err, 64, "error flags");
-    if (r == CT_OK) r = A((void**)&c->q.spill, (size_t)threads * ctd::kMaxStack * 4, "stack spill");
-    if (r == CT_OK) r = A((void**)&c->spill_side, (size_t)threads * ctd::kMaxStack * 4, "stack spill (side)");
-    if (r == CT_OK) r = A((void**)&c->wstats, 64, "work counters");
-    if (r == CT_OK) r = A((void**)&c->d_hdr, sizeof(ChunkHdr), "chunk header");
+    if (r == CT_OK) r = A((void**)&q.hit, slots * sizeof(float2), "hits");
+    if (r == CT_OK) r = A((void**)&q.sh_o, f4, "shadow origins");
+    if (r == CT_OK) r = A((void**)&q.sh_d, f4, "shadow directions");
+    if (r == CT_OK) r = A((void**)&q.sh_c, f4, "shadow colours");
+    if (r == CT_OK) r = A((void**)&q.lpath, f4, "path radiance");
+    for (int l = 0; l < 2 && r == CT_OK; ++l) {
+        Lane& L = c->ln[l];
+        const size_t o = (size_t)l * cap;
+        L.q = q;
+        for (int k = 0; k < 2; ++k) {
+            L.q.ray_o[k] = q.ray_o[k] + o;
+            L.q.ray_d[k] = q.ray_d[k] + o;
+            L.q.thr[k] = q.thr[k] + o;
+            L.q.path[k] = q.path[k] + o;
+        }
+        L.q.hit = q.hit + o;
+        L.q.sh_o = q.sh_o + o;
+        L.q.sh_d = q.sh_d + o;
+        L.q.sh_c = q.sh_c + o;
+        L.q.lpath = q.lpath + o;
+        r = A((void**)&L.q.counters, ctd::kCounterWords * 4, "counters");
+        if (r == CT_OK) r = A((void**)&L.q.err, 64, "error flags");
+        if (r == CT_OK) r = A((void**)&L.q.spill, (size_t)threads * ctd::kMaxStack * 4, "stack spill");
+        if (r == CT_OK) r = A((void**)&L.spill_side, (size_t)threads * ctd::kMaxStack * 4, "stack spill (side)");
+        if (r == CT_OK) r = A((void**)&L.wstats, 64, "work counters");
+        if (r == CT_OK) r = A((void**)&L.d_hdr, sizeof(ChunkHdr), "chunk header");
+        if (r == CT_OK && hipHostMalloc((void**)&L.h_read, kReadWords * 4, hipHostMallocDefault) != hipSuccess) {
+            L.h_read = nullptr;
+            r = fail(c, CT_ERR_OUT_OF_MEMORY, "pinned read-back buffer");
+        }
+        if (r == CT_OK && hipMemsetAsync(L.q.err, 0, 64, L.stream) != hipSuccess)
+            r = fail(c, CT_ERR_DEVICE, "clear error flags");
+    }
     if (r != CT_OK) {
         free_paths(c);
         return r;
     }
-    HIPCHK(hipMemsetAsync(c->q.err, 0, 64, c->stream), "clear error flags");
-    c->cap = cap;
+    c->lane_cap = cap;
     c->spill_threads = threads;
     return CT_OK;
 }
 
-int ensure_tables(ct_ctx* c, uint32_t nviews, uint32_t nsegs) {
-    if (nviews > c->d_views_cap) {
-        dfree(c, c->d_views);
-        c->d_views = nullptr;
+// The lane is idle here (its previous chunk has finished); only its own
+// captured graph refers to its tables.
+int ensure_tables(ct_ctx* c, Lane& L, uint32_t nviews, uint32_t nsegs) {
+    if (nviews > L.d_views_cap) {
+        dfree(c, L.d_views);
+        L.d_views = nullptr;
         const uint32_t n = std::max(nviews, 64u);
-        const int r = dalloc(c, (void**)&c->d_views, (size_t)n * sizeof(ViewDesc), "view table");
-        if (r != CT_OK) { c->d_views_cap = 0; return r; }
-        c->d_views_cap = n;
-        drop_graph(c);
+        const int r = dalloc(c, (void**)&L.d_views, (size_t)n * sizeof(ViewDesc), "view table");
+        if (r != CT_OK) { L.d_views_cap = 0; return r; }
+        L.d_views_cap = n;
+        drop_lane_graph(L);
     }
-    if (nsegs > c->d_segs_cap) {
-        dfree(c, c->d_segs);
-        c->d_segs = nullptr;
+    if (nsegs > L.d_segs_cap) {
+        dfree(c, L.d_segs);
+        L.d_segs = nullptr;
         const uint32_t n = std::max(nsegs, 256u);
-        const int r = dalloc(c, (void**)&c->d_segs, (size_t)n * sizeof(Segment), "segment table");
-        if (r != CT_OK) { c->d_segs_cap = 0; return r; }
-        c->d_segs_cap = n;
-        drop_graph(c);
+        const int r = dalloc(c, (void**)&L.d_segs, (size_t)n * sizeof(Segment), "segment table");
+        if (r != CT_OK) { L.d_segs_cap = 0; return r; }
+        L.d_segs_cap = n;
+        drop_lane_graph(L);
     }
     return CT_OK;
 }
@@ -235,22 +280,22 @@ static ctd::ShadeOpts shade_opts(const ct_frame_opts& o) {
     return s;
 }
 
-// The kernel sequence of one chunk (setGlobals -> generate -> depth x
-// (extend, shade, connect) -> resolve).  ev != nullptr records an event pair
-// around every kernel.
-static hipError_t enqueue_chunk(ct_ctx* c, float4* accum, const ct_frame_opts& o, bool count, bool timed) {
+// The kernel sequence of one chunk on a lane, up to the resolve
+// (setGlobals -> generate -> depth x (extend, shade, connect)).  timed
+// records an event pair around every kernel.
+static hipError_t enqueue_bounces(ct_ctx* c, Lane& L, const ct_frame_opts& o, bool count, bool timed) {
     const ctd::TraceOpts to = trace_opts(o);
     const ctd::ShadeOpts so = shade_opts(o);
-    hipStream_t s = c->stream;
-    unsigned long long* ws = count ? c->wstats : nullptr;
+    hipStream_t s = L.stream;
+    unsigned long long* ws = count ? L.wstats : nullptr;
     int e = 0;
     auto rec = [&]() {
-        if (timed) (void)hipEventRecord(c->ev[e++], s);
+        if (timed) (void)hipEventRecord(L.ev[e++], s);
     };
     hipError_t r;
     rec();
-    if ((r = ctk::launch_setup(c->q.counters, c->d_hdr, s)) != hipSuccess) return r;
-    if ((r = ctk::launch_generate(c->shape, c->d_views, c->d_segs, c->d_hdr, c->cap, c->q, s)) != hipSuccess)
+    if ((r = ctk::launch_setup(L.q.counters, L.d_hdr, s)) != hipSuccess) return r;
+    if ((r = ctk::launch_generate(c->shape, L.d_views, L.d_segs, L.d_hdr, c->lane_cap, L.q, s)) != hipSuccess)
         return r;
     rec();
     // connect(b) on the side stream overlaps extend(b + 1): both read queues
@@ -258,138 +303,202 @@ static hipError_t enqueue_chunk(ct_ctx* c, float4* accum, const ct_frame_opts& o
     // shade(b + 1) and resolve wait for connect(b) (the additions keep their
     // order).  Its traversal stacks spill to their own area.
     const bool nee = o.shadow && c->sc.n_lights > 0;
-    ctd::Queues qside = c->q;
-    qside.spill = c->spill_side;
+    ctd::Queues qside = L.q;
+    qside.spill = L.spill_side;
     int sel = 0;
     for (int b = 0; b < o.max_depth; ++b) {
         rec();
-        if ((r = ctk::launch_extend(c->shape, c->sc, to, c->q, sel, b, ws, s)) != hipSuccess) return r;
+        if ((r = ctk::launch_extend(c->shape, c->sc, to, L.q, sel, b, ws, s)) != hipSuccess) return r;
         rec();
-        if (nee && b > 0 && (r = hipStreamWaitEvent(s, c->join[b - 1], 0)) != hipSuccess) return r;
+        if (nee && b > 0 && (r = hipStreamWaitEvent(s, L.join[b - 1], 0)) != hipSuccess) return r;
         rec();
-        if ((r = ctk::launch_shade(c->shape, c->sc, so, c->d_views, c->d_segs, c->d_hdr, c->q, sel, b, s)) !=
-            hipSuccess)
+        if ((r = ctk::launch_shade(c->shape, c->sc, so, L.d_views, L.d_segs, L.d_hdr, L.q, sel, b, s)) != hipSuccess)
             return r;
         rec();
         if (nee) {
-            if ((r = hipEventRecord(c->fork[b], s)) != hipSuccess) return r;
-            if ((r = hipStreamWaitEvent(c->side, c->fork[b], 0)) != hipSuccess) return r;
-            if (timed) (void)hipEventRecord(c->ev[e], c->side);
+            if ((r = hipEventRecord(L.fork[b], s)) != hipSuccess) return r;
+            if ((r = hipStreamWaitEvent(L.side, L.fork[b], 0)) != hipSuccess) return r;
+            if (timed) (void)hipEventRecord(L.ev[e], L.side);
             ++e;
-            if ((r = ctk::launch_connect(c->shape, c->sc, to, qside, b, ws, c->side)) != hipSuccess) return r;
-            if (timed) (void)hipEventRecord(c->ev[e], c->side);
+            if ((r = ctk::launch_connect(c->shape, c->sc, to, qside, b, ws, L.side)) != hipSuccess) return r;
+            if (timed) (void)hipEventRecord(L.ev[e], L.side);
             ++e;
-            if ((r = hipEventRecord(c->join[b], c->side)) != hipSuccess) return r;
+            if ((r = hipEventRecord(L.join[b], L.side)) != hipSuccess) return r;
         } else {
             rec();
             rec();
         }
         sel ^= 1;
     }
-    if (nee && (r = hipStreamWaitEvent(s, c->join[o.max_depth - 1], 0)) != hipSuccess) return r;
-    rec();
-    if ((r = ctk::launch_resolve(c->shape, c->d_views, c->d_segs, c->d_hdr, c->cap, c->q.lpath, accum, s)) !=
-        hipSuccess)
-        return r;
-    rec();
-    c->n_events = e;
+    if (nee && (r = hipStreamWaitEvent(s, L.join[o.max_depth - 1], 0)) != hipSuccess) return r;
+    L.n_events = e;
     return hipSuccess;
 }
 
-static std::string graph_key(const ct_ctx* c, const float4* accum, const ct_frame_opts& o) {
+static std::string graph_key(const ct_ctx* c, const Lane& L, const ct_frame_opts& o) {
     std::ostringstream os;
-    os << (const void*)accum << ':' << c->cap << ':' << o.max_depth << ':' << o.rr_start << ':' << o.accel << ':'
-       << o.bsdf_mode << ':' << o.traversal << ':' << o.shadow << ':' << o.indirect << ':' << o.normal_epsilon << ':'
-       << o.dist_epsilon << ':' << o.triangle_epsilon << ':' << o.light_pdf << ':' << o.sort_by_material << ':' << c->shape.block << ':' << c->shape.lds_depth << ':'
-       << c->shape.grid2 << ':' << c->shape.grid4 << ':' << c->shape.grid8 << ':' << c->shape.shade_blocks << ':' << (const void*)c->d_views << ':'
-       << (const void*)c->d_segs << ':' << (const void*)c->sc.tris << ':' << (const void*)c->sc.mats;
+    os << c->lane_cap << ':' << o.max_depth << ':' << o.rr_start << ':' << o.accel << ':' << o.bsdf_mode << ':'
+       << o.traversal << ':' << o.shadow << ':' << o.indirect << ':' << o.normal_epsilon << ':' << o.dist_epsilon
+       << ':' << o.triangle_epsilon << ':' << o.light_pdf << ':' << o.sort_by_material << ':' << c->shape.block << ':'
+       << c->shape.lds_depth << ':' << c->shape.grid2 << ':' << c->shape.grid4 << ':' << c->shape.grid8 << ':'
+       << c->shape.shade_blocks << ':' << (const void*)L.d_views << ':' << (const void*)L.d_segs << ':'
+       << (const void*)L.q.counters << ':' << (const void*)c->sc.tris << ':' << (const void*)c->sc.mats;
     return os.str();
 }
 
-int run_chunk(ct_ctx* c, const std::vector<ViewDesc>& views, const std::vector<Segment>& segs, uint32_t n,
-              uint32_t n_tasks, float4* accum, const ct_frame_opts& o, bool timed, bool count, bool use_graph,
-              ChunkStats* st) {
-    int r = ensure_tables(c, (uint32_t)views.size(), (uint32_t)segs.size());
+static constexpr int kEvTotal0 = 2 * (3 * ctd::kMaxBounce + 8) - 2;  // chunk start / end events
+static constexpr int kEvResolve = 2 * (3 * ctd::kMaxBounce + 8) - 4;  // resolve pair (timed)
+
+// Queue one chunk on lane L: upload its tables, the bounce loop (replayed from
+// the lane's captured graph when use_graph), then the resolve after the
+// previous chunk's resolve (prev_resolved, may be null), then the read-back.
+static int launch_chunk(ct_ctx* c, Lane& L, const std::vector<ViewDesc>& views, const std::vector<Segment>& segs,
+                        uint32_t n, uint32_t n_tasks, float4* accum, const ct_frame_opts& o, bool timed, bool count,
+                        bool use_graph, hipEvent_t prev_resolved) {
+    int r = ensure_tables(c, L, (uint32_t)views.size(), (uint32_t)segs.size());
     if (r != CT_OK) return r;
-    ChunkHdr h{n, n_tasks, (uint32_t)segs.size(), 0u};
-    HIPCHK(hipMemcpyAsync(c->d_views, views.data(), views.size() * sizeof(ViewDesc), hipMemcpyHostToDevice,
-                          c->stream),
-           "upload views");
-    HIPCHK(hipMemcpyAsync(c->d_segs, segs.data(), segs.size() * sizeof(Segment), hipMemcpyHostToDevice, c->stream),
-           "upload segments");
-    HIPCHK(hipMemcpyAsync(c->d_hdr, &h, sizeof(h), hipMemcpyHostToDevice, c->stream), "upload chunk header");
-    if (count) HIPCHK(hipMemsetAsync(c->wstats, 0, 32, c->stream), "clear work counters");
-    hipEvent_t g0 = c->ev[2 * (3 * ctd::kMaxBounce + 8) - 2], g1 = c->ev[2 * (3 * ctd::kMaxBounce + 8) - 1];
+    hipStream_t s = L.stream;
+    // tables staged in pinned memory: the copies run after this call returns,
+    // and the lane's staging is rewritten only once its chunk has finished
+    const size_t vb = views.size() * sizeof(ViewDesc), sb = segs.size() * sizeof(Segment);
+    const size_t need = 256 + vb + sb;
+    if (need > L.h_stage_bytes) {
+        if (L.h_stage) (void)hipHostFree(L.h_stage);
+        L.h_stage = nullptr;
+        L.h_stage_bytes = 0;
+        const size_t bytes = std::max<size_t>(need * 2, 1u << 16);
+        if (hipHostMalloc((void**)&L.h_stage, bytes, hipHostMallocDefault) != hipSuccess) {
+            L.h_stage = nullptr;
+            return fail(c, CT_ERR_OUT_OF_MEMORY, "pinned staging of chunk tables");
+        }
+        L.h_stage_bytes = bytes;
+    }
+    const ChunkHdr h{n, n_tasks, (uint32_t)segs.size(), 0u};
+    std::memcpy(L.h_stage, &h, sizeof(h));
+    std::memcpy(L.h_stage + 256, views.data(), vb);
+    std::memcpy(L.h_stage + 256 + vb, segs.data(), sb);
+    HIPCHK(hipMemcpyAsync(L.d_views, L.h_stage + 256, vb, hipMemcpyHostToDevice, s), "upload views");
+    HIPCHK(hipMemcpyAsync(L.d_segs, L.h_stage + 256 + vb, sb, hipMemcpyHostToDevice, s), "upload segments");
+    HIPCHK(hipMemcpyAsync(L.d_hdr, L.h_stage, sizeof(h), hipMemcpyHostToDevice, s), "upload chunk header");
+    if (count) HIPCHK(hipMemsetAsync(L.wstats, 0, 32, s), "clear work counters");
+    HIPCHK(hipEventRecord(L.ev[kEvTotal0], s), "event");
     if (use_graph && !count && !timed) {
-        const std::string key = graph_key(c, accum, o);
-        if (c->graph_key != key || !c->graph_exec) {
-            drop_graph(c);
+        const std::string key = graph_key(c, L, o);
+        if (L.graph_key != key || !L.graph_exec) {
+            if (L.graph_exec) (void)hipGraphExecDestroy(L.graph_exec);
+            L.graph_exec = nullptr;
+            L.graph_key.clear();
             hipGraph_t g = nullptr;
-            HIPCHK(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal), "begin capture");
-            const hipError_t ce = enqueue_chunk(c, accum, o, false, false);
-            const hipError_t ee = hipStreamEndCapture(c->stream, &g);
+            HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal), "begin capture");
+            const hipError_t ce = enqueue_bounces(c, L, o, false, false);
+            const hipError_t ee = hipStreamEndCapture(s, &g);
             if (ce != hipSuccess) { if (g) (void)hipGraphDestroy(g); return hip_fail(c, ce, "capture bounce loop"); }
             if (ee != hipSuccess) return hip_fail(c, ee, "end capture");
-            const hipError_t ie = hipGraphInstantiate(&c->graph_exec, g, nullptr, nullptr, 0);
+            const hipError_t ie = hipGraphInstantiate(&L.graph_exec, g, nullptr, nullptr, 0);
             (void)hipGraphDestroy(g);
-            if (ie != hipSuccess) { c->graph_exec = nullptr; return hip_fail(c, ie, "instantiate graph"); }
-            c->graph_key = key;
+            if (ie != hipSuccess) { L.graph_exec = nullptr; return hip_fail(c, ie, "instantiate graph"); }
+            L.graph_key = key;
         }
-        HIPCHK(hipEventRecord(g0, c->stream), "event");
-        HIPCHK(hipGraphLaunch(c->graph_exec, c->stream), "launch graph");
-        HIPCHK(hipEventRecord(g1, c->stream), "event");
+        HIPCHK(hipGraphLaunch(L.graph_exec, s), "launch graph");
     } else {
-        HIPCHK(hipEventRecord(g0, c->stream), "event");
-        HIPCHK(enqueue_chunk(c, accum, o, count, timed), "launch chunk");
-        HIPCHK(hipEventRecord(g1, c->stream), "event");
+        HIPCHK(enqueue_bounces(c, L, o, count, timed), "launch chunk");
     }
-    // queue sizes (ray counts) and error flags
-    uint32_t cnt[2 * (ctd::kMaxBounce + 1)];
-    uint32_t errflag = 0;
-    HIPCHK(hipMemcpyAsync(cnt, c->q.counters, sizeof(cnt), hipMemcpyDeviceToHost, c->stream), "read counters");
-    HIPCHK(hipMemcpyAsync(&errflag, c->q.err, 4, hipMemcpyDeviceToHost, c->stream), "read error flags");
-    unsigned long long w[4] = {0, 0, 0, 0};
-    if (count) HIPCHK(hipMemcpyAsync(w, c->wstats, 32, hipMemcpyDeviceToHost, c->stream), "read work counters");
-    HIPCHK(hipStreamSynchronize(c->stream), "chunk");
-    if (errflag) {
-        (void)hipMemsetAsync(c->q.err, 0, 4, c->stream);
+    // accumulator updates in chunk order
+    if (prev_resolved) HIPCHK(hipStreamWaitEvent(s, prev_resolved, 0), "order resolves");
+    if (timed) HIPCHK(hipEventRecord(L.ev[kEvResolve], s), "event");
+    HIPCHK(ctk::launch_resolve(c->shape, L.d_views, L.d_segs, L.d_hdr, c->lane_cap, L.q.lpath, accum, s),
+           "launch resolve");
+    if (timed) HIPCHK(hipEventRecord(L.ev[kEvResolve + 1], s), "event");
+    HIPCHK(hipEventRecord(L.resolved, s), "event");
+    HIPCHK(hipEventRecord(L.ev[kEvTotal0 + 1], s), "event");
+    // queue sizes (ray counts), error flags and work counters
+    HIPCHK(hipMemcpyAsync(L.h_read, L.q.counters, kReadCounters * 4, hipMemcpyDeviceToHost, s), "read counters");
+    HIPCHK(hipMemcpyAsync(L.h_read + kReadErr, L.q.err, 4, hipMemcpyDeviceToHost, s), "read error flags");
+    if (count) HIPCHK(hipMemcpyAsync(L.h_read + kReadWork, L.wstats, 32, hipMemcpyDeviceToHost, s), "read work");
+    L.pending = true;
+    L.n_paths = n;
+    L.max_depth = o.max_depth;
+    L.timed = timed;
+    L.count = count;
+    return CT_OK;
+}
+
+// Wait for lane L's chunk and add its statistics.
+static int finish_chunk(ct_ctx* c, Lane& L, ChunkStats* st) {
+    if (!L.pending) return CT_OK;
+    L.pending = false;
+    HIPCHK(hipStreamSynchronize(L.stream), "chunk");
+    const uint32_t* cnt = L.h_read;
+    if (cnt[kReadErr]) {
+        (void)hipMemsetAsync(L.q.err, 0, 4, L.stream);
         return fail(c, CT_ERR_STACK_OVERFLOW, "traversal stack bound exceeded (malformed tree)");
     }
-    if (st) {
-        st->primary += cnt[0];
-        for (int b = 1; b < o.max_depth; ++b) st->extension += cnt[b];
-        for (int b = 0; b < o.max_depth; ++b) st->shadow += cnt[(ctd::kMaxBounce + 1) + b];
-        st->samples += n;
-        float ms = 0;
-        if (hipEventElapsedTime(&ms, g0, g1) == hipSuccess) st->total_ms += ms;
-        if (timed) {
-            // event pairs: [generate] then per bounce [extend][shade][connect], then [resolve]
-            int e = 0;
-            auto pair = [&]() {
-                float m = 0;
-                (void)hipEventElapsedTime(&m, c->ev[e], c->ev[e + 1]);
-                e += 2;
-                return (double)m;
-            };
-            st->other_ms += pair();
-            for (int b = 0; b < o.max_depth; ++b) {
-                st->extend_ms += pair();
-                st->extend_launches += 1;
-                st->shade_ms += pair();
-                st->connect_ms += pair();
-            }
-            st->other_ms += pair();
+    if (!st) return CT_OK;
+    st->chunks += 1;
+    st->primary += cnt[0];
+    for (int b = 1; b < L.max_depth; ++b) st->extension += cnt[b];
+    for (int b = 0; b < L.max_depth; ++b) st->shadow += cnt[(ctd::kMaxBounce + 1) + b];
+    st->samples += L.n_paths;
+    float ms = 0;
+    if (hipEventElapsedTime(&ms, L.ev[kEvTotal0], L.ev[kEvTotal0 + 1]) == hipSuccess) st->total_ms += ms;
+    if (L.timed) {
+        // event pairs: [generate] then per bounce [extend][shade][connect]; [resolve]
+        int e = 0;
+        auto pair = [&](int i) {
+            float m = 0;
+            (void)hipEventElapsedTime(&m, L.ev[i], L.ev[i + 1]);
+            return (double)m;
+        };
+        st->other_ms += pair(e);
+        e += 2;
+        for (int b = 0; b < L.max_depth; ++b) {
+            st->extend_ms += pair(e);
+            st->extend_launches += 1;
+            st->shade_ms += pair(e + 2);
+            st->connect_ms += pair(e + 4);
+            e += 6;
         }
+        st->other_ms += pair(kEvResolve);
+    }
+    if (L.count) {
+        unsigned long long w[4];
+        std::memcpy(w, cnt + kReadWork, sizeof(w));
         for (int k = 0; k < 4; ++k) st->work[k] += w[k];
     }
     return CT_OK;
 }
 
+// Paths of a workload below which it is not split over the two lanes.
+static constexpr uint64_t kSplitMin = 1u << 18;
+
 int run_paths(ct_ctx* c, const std::vector<ViewDesc>& views, const std::vector<Run>& runs, uint32_t cap,
               float4* accum, const ct_frame_opts& o, bool timed, bool count, bool use_graph, ChunkStats* st) {
+    uint64_t total = 0;
+    for (const Run& run : runs) total += run.n;
+    if (total >= kSplitMin && cap > 0) {
+        // an even number of equal chunks, so both lanes get the same work
+        uint64_t nch = (total + cap - 1) / cap;
+        nch = std::max<uint64_t>(2, nch + (nch & 1));
+        cap = (uint32_t)((total + nch - 1) / nch);
+    }
     int r = ensure_paths(c, cap);
     if (r != CT_OK) return r;
-    cap = std::min(cap, c->cap);
+    cap = std::min(cap, c->lane_cap);
+    // work queued on lane 0 by the caller (accumulator clears) comes first
+    HIPCHK(hipEventRecord(c->ln[0].resolved, c->ln[0].stream), "event");
+    HIPCHK(hipStreamWaitEvent(c->ln[1].stream, c->ln[0].resolved, 0), "order lanes");
+    hipEvent_t prev = nullptr;
+    int lane = 0;
+    auto flush = [&](std::vector<Segment>& segs, uint32_t fill, uint32_t tasks) -> int {
+        Lane& L = c->ln[lane];
+        int rr = finish_chunk(c, L, st);  // the lane's previous chunk
+        if (rr != CT_OK) return rr;
+        rr = launch_chunk(c, L, views, segs, fill, tasks, accum, o, timed, count, use_graph, prev);
+        if (rr != CT_OK) return rr;
+        prev = L.resolved;
+        lane ^= 1;
+        return CT_OK;
+    };
     std::vector<Segment> segs;
     uint32_t fill = 0, tasks = 0;
     for (const Run& run : runs) {
@@ -410,17 +519,23 @@ int run_paths(ct_ctx* c, const std::vector<ViewDesc>& views, const std::vector<R
             first += take;
             left -= take;
             if (fill == cap) {
-                r = run_chunk(c, views, segs, fill, tasks, accum, o, timed, count, use_graph, st);
-                if (r != CT_OK) return r;
+                r = flush(segs, fill, tasks);
+                if (r != CT_OK) { sync_lanes(c); return r; }
                 segs.clear();
                 fill = tasks = 0;
             }
         }
     }
     if (fill > 0) {
-        r = run_chunk(c, views, segs, fill, tasks, accum, o, timed, count, use_graph, st);
-        if (r != CT_OK) return r;
+        r = flush(segs, fill, tasks);
+        if (r != CT_OK) { sync_lanes(c); return r; }
     }
+    for (int k = 0; k < 2; ++k) {
+        r = finish_chunk(c, c->ln[lane], st);
+        lane ^= 1;
+        if (r != CT_OK) { sync_lanes(c); return r; }
+    }
+    // lane 1's work is done: later work on lane 0 needs no further ordering
     return CT_OK;
 }
 
@@ -493,16 +608,19 @@ int ct_ctx_create(const ct_device_opts* opts, ct_ctx** out) {
     size_t freeb = 0, totalb = 0;
     if (hipMemGetInfo(&freeb, &totalb) != hipSuccess) { delete c; return CT_ERR_DEVICE; }
     c->budget = d.mem_budget ? d.mem_budget : (uint64_t)(0.9 * (double)freeb);
-    if (hipStreamCreate(&c->stream) != hipSuccess) { delete c; return CT_ERR_DEVICE; }
-    if (hipStreamCreate(&c->side) != hipSuccess) { (void)hipStreamDestroy(c->stream); delete c; return CT_ERR_DEVICE; }
-    for (int b = 0; b < ctd::kMaxBounce; ++b)
-        if (hipEventCreateWithFlags(&c->fork[b], hipEventDisableTiming) != hipSuccess ||
-            hipEventCreateWithFlags(&c->join[b], hipEventDisableTiming) != hipSuccess) {
-            delete c;
+    for (Lane& L : c->ln) {
+        bool ok = hipStreamCreate(&L.stream) == hipSuccess && hipStreamCreate(&L.side) == hipSuccess &&
+                  hipEventCreateWithFlags(&L.resolved, hipEventDisableTiming) == hipSuccess;
+        for (int b = 0; b < ctd::kMaxBounce && ok; ++b)
+            ok = hipEventCreateWithFlags(&L.fork[b], hipEventDisableTiming) == hipSuccess &&
+                 hipEventCreateWithFlags(&L.join[b], hipEventDisableTiming) == hipSuccess;
+        for (auto& ev : L.ev)
+            if (ok) ok = hipEventCreate(&ev) == hipSuccess;
+        if (!ok) {
+            ct_ctx_destroy(c);
             return CT_ERR_DEVICE;
         }
-    for (auto& ev : c->ev)
-        if (hipEventCreate(&ev) != hipSuccess) { delete c; return CT_ERR_DEVICE; }
+    }
     const int block = d.block_size > 0 ? d.block_size : 256;
     const int lds = d.stack_lds > 0 ? d.stack_lds : 16;
     if (block % 64 != 0 || block > 1024 || lds > ctd::kMaxStack) { delete c; return CT_ERR_INVALID_ARG; }
@@ -515,19 +633,23 @@ int ct_ctx_create(const ct_device_opts* opts, ct_ctx** out) {
 void ct_ctx_destroy(ct_ctx* c) {
     if (!c) return;
     (void)hipSetDevice(c->device);
-    if (c->stream) (void)hipStreamSynchronize(c->stream);
-    if (c->side) (void)hipStreamSynchronize(c->side);
-    drop_graph(c);
+    sync_lanes(c);
+    drop_graphs(c);
     for (auto& a : c->allocs) (void)hipFree(a.first);
     c->allocs.clear();
-    for (auto& ev : c->ev)
-        if (ev) (void)hipEventDestroy(ev);
-    for (int b = 0; b < ctd::kMaxBounce; ++b) {
-        if (c->fork[b]) (void)hipEventDestroy(c->fork[b]);
-        if (c->join[b]) (void)hipEventDestroy(c->join[b]);
+    for (Lane& L : c->ln) {
+        if (L.h_read) (void)hipHostFree(L.h_read);
+        if (L.h_stage) (void)hipHostFree(L.h_stage);
+        for (auto& ev : L.ev)
+            if (ev) (void)hipEventDestroy(ev);
+        for (int b = 0; b < ctd::kMaxBounce; ++b) {
+            if (L.fork[b]) (void)hipEventDestroy(L.fork[b]);
+            if (L.join[b]) (void)hipEventDestroy(L.join[b]);
+        }
+        if (L.resolved) (void)hipEventDestroy(L.resolved);
+        if (L.side) (void)hipStreamDestroy(L.side);
+        if (L.stream) (void)hipStreamDestroy(L.stream);
     }
-    if (c->side) (void)hipStreamDestroy(c->side);
-    if (c->stream) (void)hipStreamDestroy(c->stream);
     delete c;
 }
 

@@ -241,7 +241,7 @@ void ct_server::loop() {
         // zero accumulators of requests starting in this chunk, then trace
         int rc = CT_OK;
         for (const auto& f : fresh)
-            if (rc == CT_OK && hipMemsetAsync(arena_ptr + f.first, 0, f.second * sizeof(float4), ctx->stream) != hipSuccess)
+            if (rc == CT_OK && hipMemsetAsync(arena_ptr + f.first, 0, f.second * sizeof(float4), ctx->ln[0].stream) != hipSuccess)
                 rc = CT_ERR_DEVICE;
         ChunkStats st;
         if (rc == CT_OK)
@@ -408,7 +408,7 @@ void ct_server_destroy(ct_server* s) {
     }
     if (s->worker.joinable()) s->worker.join();
     if (s->copier.joinable()) s->copier.join();
-    (void)hipStreamSynchronize(s->ctx->stream);
+    sync_lanes(s->ctx);
     if (s->copy_stream) (void)hipStreamDestroy(s->copy_stream);
     dfree(s->ctx, s->arena_ptr);
     delete s;
