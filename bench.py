@@ -131,7 +131,7 @@ def main() -> int:
         conn_ms += st.connect_ms
         other_ms += st.other_ms
         gpu_ms += st.gpu_ms
-        launches += opts.max_depth
+        launches += st.extend_launches
     t1 = time.perf_counter()
     wall = t1 - t0
     rays = prim + ext + sh
@@ -141,7 +141,7 @@ def main() -> int:
     # roofline of the extend kernel: algorithmic bytes per launch / mean duration
     closest_per_frame = (prim + ext) / args.steps
     bytes_per_frame = alg_bytes_frame * (closest_per_frame / max(closest_rays, 1))
-    bytes_per_launch = bytes_per_frame / opts.max_depth
+    bytes_per_launch = bytes_per_frame * args.steps / max(launches, 1)
     mean_launch_ms = ext_ms / max(launches, 1)
     achieved = bytes_per_launch / (mean_launch_ms * 1e-3) / 1e9
     roofline = {

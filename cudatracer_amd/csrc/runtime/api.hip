@@ -337,6 +337,8 @@ static void fill_stats(const ChunkStats& s, ct_frame_stats* out) {
     out->samples = s.samples;
     out->gpu_ms = s.total_ms;
     out->extend_ms = s.extend_ms;
+    out->chunks = (uint32_t)s.chunks;
+    out->extend_launches = (uint32_t)s.extend_launches;
     out->shade_ms = s.shade_ms;
     out->connect_ms = s.connect_ms;
     out->other_ms = s.other_ms;

@@ -38,9 +38,10 @@ using namespace ctr;
 namespace {
 
 constexpr uint64_t kBytesPerPath = 176;  // path-state queue bytes per in-flight path sample
-// Default paths per chunk: small enough that a request arriving while a
-// chunk runs starts within a few milliseconds, large enough to fill the GPU.
-constexpr uint32_t kServerChunk = 1u << 21;
+// Default paths per chunk (split over the two lanes): small enough that a
+// request arriving while a chunk runs starts within a few milliseconds,
+// large enough that both lanes stay full.
+constexpr uint32_t kServerChunk = 1u << 23;
 
 enum ReqState { PENDING = 0, ACTIVE = 1, COPYING = 2, DONE = 3 };
 

@@ -27,7 +27,7 @@
 extern "C" {
 #endif
 
-#define CT_ABI_VERSION 1
+#define CT_ABI_VERSION 2
 
 /* ---------------------------------------------------------------- status */
 enum {
@@ -165,8 +165,12 @@ typedef struct ct_frame_stats {
     uint64_t extension_rays;  /* closest-hit rays after the first segment     */
     uint64_t shadow_rays;     /* any-hit rays traced                          */
     uint64_t samples;         /* path samples completed                       */
-    double gpu_ms;            /* device time of the frame (HIP events)        */
-    double extend_ms, shade_ms, connect_ms, other_ms; /* per-stage device time, if profiled */
+    double gpu_ms;            /* device time of the frame's chunks (HIP events; chunks on
+                                 the two lanes overlap, so this can exceed wall time) */
+    double extend_ms, shade_ms, connect_ms, other_ms; /* per-stage device time, if profiled;
+                                 stages of different chunks overlap */
+    uint32_t chunks;          /* wavefront chunks the frame was split into    */
+    uint32_t extend_launches; /* closest-hit kernel launches (chunks x depth) */
 } ct_frame_stats;
 
 /* Fill with the reference defaults (depth 5, 1 spp, no RR, BVH4, reference BSDF,
