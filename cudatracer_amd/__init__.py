@@ -34,6 +34,7 @@ BSDF_REFERENCE, BSDF_TYPED = 0, 1
 TRAV_EXACT, TRAV_FAST = 0, 1
 LIGHTPDF_REFERENCE, LIGHTPDF_UNIFORM = 0, 1
 SCENE_C1, SCENE_C2, SCENE_C3 = 1, 2, 3
+CT_BUILD_WIDE = 1
 
 
 class CTError(RuntimeError):
@@ -99,6 +100,15 @@ class FrameStats(C.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
+class BuildStats(C.Structure):
+    _fields_ = [("n_bvh_nodes", C.c_uint32), ("n_mbvh_nodes", C.c_uint32), ("levels", C.c_uint32),
+                ("max_depth", C.c_uint32), ("max_chain", C.c_uint32), ("device_ms", C.c_float),
+                ("total_ms", C.c_float), ("reserved", C.c_uint32)]
+
+    def as_dict(self) -> dict:
+        return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
+
+
 class DeviceOpts(C.Structure):
     _fields_ = [("device", C.c_int32), ("mem_budget", C.c_uint64), ("block_size", C.c_int32),
                 ("stack_lds", C.c_int32), ("chunk_paths", C.c_int32)]
@@ -148,6 +158,10 @@ _sig = {
     "ct_last_error": (C.c_char_p, [_P]),
     "ct_device_name": (C.c_int, [_P, C.c_char_p, C.c_size_t]),
     "ct_scene_upload": (C.c_int, [_P, C.POINTER(SceneView)]),
+    "ct_scene_upload_build": (C.c_int, [_P, C.POINTER(SceneView), C.c_uint32, C.POINTER(BuildStats)]),
+    "ct_scene_tree_sizes": (C.c_int, [_P, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32), C.POINTER(C.c_uint32)]),
+    "ct_scene_read_trees": (C.c_int, [_P, C.c_void_p, C.c_uint32, C.c_void_p, C.c_uint32, C.c_void_p,
+                                      C.c_uint32]),
     "ct_material_update": (C.c_int, [_P, C.c_uint32, C.POINTER(Material)]),
     "ct_resize": (C.c_int, [_P, C.c_int32, C.c_int32]),
     "ct_reset": (C.c_int, [_P]),
@@ -244,11 +258,14 @@ class HostScene:
         return cls(h)
 
     @classmethod
-    def generate(cls, config: int, seed: int = 0) -> "HostScene":
+    def generate(cls, config: int, seed: int = 0, build: bool = True) -> "HostScene":
+        """A configuration scene; build=False skips the host tree build (for
+        Context.upload_build, which builds the trees on the device)."""
         h = _P()
         _check(lib.ct_host_scene_generate(config, seed, C.byref(h)), None)
         s = cls(h)
-        s.build()
+        if build:
+            s.build()
         return s
 
     def add_material(self, albedo, mtype=MAT_LAMBERTIAN, roughness=0.0, ior=1.0, keep_roughness=False) -> int:
@@ -320,7 +337,7 @@ class HostScene:
         nv, nt = v.n_vertices, v.n_triangles
 
         def arr(ptr, n, dt):
-            if n == 0:
+            if n == 0 or not ptr:
                 return np.zeros(0, dt)
             return np.ctypeslib.as_array(ptr, shape=(n,)).copy()
 
@@ -337,8 +354,8 @@ class HostScene:
         }
         out["materials"] = (Material * v.n_materials).from_buffer_copy(
             C.string_at(v.materials, C.sizeof(Material) * v.n_materials))
-        out["bvh"] = np.frombuffer(C.string_at(v.bvh_nodes, 32 * v.n_bvh_nodes), np.uint8).copy()
-        out["mbvh"] = np.frombuffer(C.string_at(v.mbvh_nodes, 128 * v.n_mbvh_nodes), np.uint8).copy()
+        out["bvh"] = arr(C.cast(v.bvh_nodes, C.POINTER(C.c_uint8)), 32 * v.n_bvh_nodes, np.uint8)
+        out["mbvh"] = arr(C.cast(v.mbvh_nodes, C.POINTER(C.c_uint8)), 128 * v.n_mbvh_nodes, np.uint8)
         return out
 
     def __del__(self):
@@ -379,6 +396,26 @@ class Context:
     def upload(self, scene: HostScene) -> None:
         v = scene.view()
         _check(lib.ct_scene_upload(self._h, C.byref(v)), self._h)
+
+    def upload_build(self, scene, wide: bool = False) -> dict:
+        """Upload a scene and build its trees on the device (byte-identical to
+        the host build).  scene: HostScene or SceneView."""
+        v = scene.view() if isinstance(scene, HostScene) else scene
+        st = BuildStats()
+        _check(lib.ct_scene_upload_build(self._h, C.byref(v), CT_BUILD_WIDE if wide else 0, C.byref(st)), self._h)
+        return st.as_dict()
+
+    def read_trees(self) -> dict:
+        """The current scene's binary tree, quad tree (raw bytes, host export
+        layouts) and primitive order."""
+        n2, n4, nt = C.c_uint32(), C.c_uint32(), C.c_uint32()
+        _check(lib.ct_scene_tree_sizes(self._h, C.byref(n2), C.byref(n4), C.byref(nt)), self._h)
+        bvh = np.zeros(32 * n2.value, np.uint8)
+        mbvh = np.zeros(128 * n4.value, np.uint8)
+        prims = np.zeros(nt.value, np.uint32)
+        _check(lib.ct_scene_read_trees(self._h, bvh.ctypes.data, n2.value, mbvh.ctypes.data, n4.value,
+                                       prims.ctypes.data, nt.value), self._h)
+        return {"bvh": bvh, "mbvh": mbvh, "prim_indices": prims}
 
     def upload_view(self, v: SceneView) -> None:
         _check(lib.ct_scene_upload(self._h, C.byref(v)), self._h)

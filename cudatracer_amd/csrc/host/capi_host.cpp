@@ -118,7 +118,6 @@ int ct_host_scene_build(ct_host_scene* s) {
 int ct_host_scene_view(const ct_host_scene* cs, ct_scene_view* out) {
     if (!cs || !out) return CT_ERR_INVALID_ARG;
     ct_host_scene* s = const_cast<ct_host_scene*>(cs);
-    if (!s->bvh || !s->mbvh) return CT_ERR_INVALID_ARG;
     s->mats.clear();
     for (const auto& m : s->list.m_Materials) s->mats.push_back(m.to_abi());
     std::memset(out, 0, sizeof(*out));
@@ -135,11 +134,14 @@ int ct_host_scene_view(const ct_host_scene* cs, ct_scene_view* out) {
     out->light_indices = L.m_LightIndices.data();
     out->n_materials = (uint32_t)s->mats.size();
     out->materials = s->mats.data();
-    out->prim_indices = s->bvh->m_PrimitiveIndices.data();
-    out->n_bvh_nodes = (uint32_t)s->bvh->m_BVHPool.size();
-    out->bvh_nodes = reinterpret_cast<const ct_bvh_node*>(s->bvh->m_BVHPool.data());
-    out->n_mbvh_nodes = (uint32_t)s->mbvh->m_Tree.size();
-    out->mbvh_nodes = reinterpret_cast<const ct_mbvh_node*>(s->mbvh->m_Tree.data());
+    // trees only after ct_host_scene_build (a view without them suits ct_scene_upload_build)
+    if (s->bvh && s->mbvh) {
+        out->prim_indices = s->bvh->m_PrimitiveIndices.data();
+        out->n_bvh_nodes = (uint32_t)s->bvh->m_BVHPool.size();
+        out->bvh_nodes = reinterpret_cast<const ct_bvh_node*>(s->bvh->m_BVHPool.data());
+        out->n_mbvh_nodes = (uint32_t)s->mbvh->m_Tree.size();
+        out->mbvh_nodes = reinterpret_cast<const ct_mbvh_node*>(s->mbvh->m_Tree.data());
+    }
     out->sky_enabled = s->cfg.sky_enabled;
     out->sky[0] = s->cfg.sky.x;
     out->sky[1] = s->cfg.sky.y;

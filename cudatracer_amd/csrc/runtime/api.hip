@@ -156,21 +156,14 @@ void light_rows(const ct_scene_view* v, uint32_t tri, const ct_material* mats, f
     rows[4] = f4(mats[v->material_idx[tri]].albedo, 0.0f);
 }
 
-}  // namespace
-
-extern "C" {
-
-int ct_scene_upload(ct_ctx* c, const ct_scene_view* v) {
-    if (!c || !v) return CT_ERR_INVALID_ARG;
-    (void)hipSetDevice(c->device);
+// Checks shared by both upload paths (everything but the trees).
+int check_geometry(ct_ctx* c, const ct_scene_view* v) {
     const uint32_t N = v->n_triangles, V = v->n_vertices;
     if (N == 0) return fail(c, CT_ERR_INVALID_SCENE, "scene has no triangles");
-    if (N >= (1u << 31)) return fail(c, CT_ERR_INVALID_SCENE, "too many triangles");
+    if (N >= (1u << 30)) return fail(c, CT_ERR_INVALID_SCENE, "too many triangles");
     if (!v->vertices || !v->normals || !v->indices || !v->center_normals || !v->material_idx || !v->materials ||
-        !v->prim_indices || v->n_materials == 0 || (v->n_lights && !v->light_indices))
+        v->n_materials == 0 || (v->n_lights && !v->light_indices))
         return fail(c, CT_ERR_INVALID_ARG, "scene view is missing an array");
-    if (v->n_bvh_nodes == 0 && v->n_mbvh_nodes == 0)
-        return fail(c, CT_ERR_INVALID_ARG, "scene view has no acceleration tree");
     for (uint64_t i = 0; i < 3ull * N; ++i)
         if (v->indices[i] >= V) return fail(c, CT_ERR_INVALID_SCENE, "vertex index out of range");
     for (uint32_t i = 0; i < V; ++i)
@@ -179,6 +172,76 @@ int ct_scene_upload(ct_ctx* c, const ct_scene_view* v) {
         if (v->material_idx[i] >= v->n_materials) return fail(c, CT_ERR_INVALID_SCENE, "material index out of range");
     for (uint32_t i = 0; i < v->n_lights; ++i)
         if (v->light_indices[i] >= N) return fail(c, CT_ERR_INVALID_SCENE, "light index out of range");
+    return CT_OK;
+}
+
+// Host staging of the tree-independent device arrays.
+struct Shading {
+    std::vector<float4> pos, nrm, mats, lights;
+};
+
+void stage_shading(const ct_scene_view* v, Shading& sh) {
+    const uint32_t N = v->n_triangles;
+    sh.pos.resize((size_t)N * 3);
+    sh.nrm.resize((size_t)N * 4);
+    for (uint32_t prim = 0; prim < N; ++prim) {
+        for (int j = 0; j < 3; ++j) {
+            const uint32_t vi = v->indices[3 * (size_t)prim + j];
+            sh.pos[3 * (size_t)prim + j] = f4(v->vertices + 3 * (size_t)vi, j == 0 ? ubits(v->material_idx[prim]) : 0.0f);
+            sh.nrm[4 * (size_t)prim + j] = f4(v->normals + 3 * (size_t)vi, 0.0f);
+        }
+        sh.nrm[4 * (size_t)prim + 3] = f4(v->center_normals + 3 * (size_t)prim, 0.0f);
+    }
+    sh.mats.resize((size_t)v->n_materials * 2);
+    for (uint32_t m = 0; m < v->n_materials; ++m) mat_rows(v->materials[m], &sh.mats[2 * (size_t)m]);
+    sh.lights.resize((size_t)std::max<uint32_t>(v->n_lights, 1u) * 5);
+    for (uint32_t l = 0; l < v->n_lights; ++l) light_rows(v, v->light_indices[l], v->materials, &sh.lights[5 * (size_t)l]);
+}
+
+int upload_shading(ct_ctx* c, const Shading& sh, ctd::DevScene& sc) {
+    int r = upload_vec(c, sh.pos, &sc.sh_pos, "shading positions");
+    if (r == CT_OK) r = upload_vec(c, sh.nrm, &sc.sh_nrm, "shading normals");
+    if (r == CT_OK) r = upload_vec(c, sh.mats, &sc.mats, "materials");
+    if (r == CT_OK) r = upload_vec(c, sh.lights, &sc.lights, "lights");
+    return r;
+}
+
+// Make an uploaded scene current (tree fields of sc already set).
+void install_scene(ct_ctx* c, const ct_scene_view* v, ctd::DevScene sc, const Shading& sh, uint32_t wide_bound) {
+    const uint32_t N = v->n_triangles;
+    sc.n_tris = N;
+    sc.n_mats = v->n_materials;
+    sc.n_lights = v->n_lights;
+    sc.sky_enabled = v->sky_enabled ? 1 : 0;
+    sc.sky_r = v->sky[0];
+    sc.sky_g = v->sky[1];
+    sc.sky_b = v->sky[2];
+    c->sc = sc;
+    c->has_bvh4 = sc.n_nodes4 > 0;
+    c->has_bvh2 = sc.n_nodes2 > 0;
+    c->wide_stack_bound = wide_bound;
+    c->has_scene = true;
+    c->h_mats.assign(v->materials, v->materials + v->n_materials);
+    c->h_mat_idx.assign(v->material_idx, v->material_idx + N);
+    c->h_light_idx.assign(v->light_indices, v->light_indices + v->n_lights);
+    c->h_light_rows.assign(reinterpret_cast<const float*>(sh.lights.data()),
+                           reinterpret_cast<const float*>(sh.lights.data()) + sh.lights.size() * 4);
+    drop_graphs(c);
+}
+
+}  // namespace
+
+extern "C" {
+
+int ct_scene_upload(ct_ctx* c, const ct_scene_view* v) {
+    if (!c || !v) return CT_ERR_INVALID_ARG;
+    (void)hipSetDevice(c->device);
+    int r = check_geometry(c, v);
+    if (r != CT_OK) return r;
+    const uint32_t N = v->n_triangles;
+    if (!v->prim_indices) return fail(c, CT_ERR_INVALID_ARG, "scene view is missing an array");
+    if (v->n_bvh_nodes == 0 && v->n_mbvh_nodes == 0)
+        return fail(c, CT_ERR_INVALID_ARG, "scene view has no acceleration tree");
     {
         std::vector<uint8_t> seen(N, 0);
         for (uint32_t i = 0; i < N; ++i) {
@@ -204,7 +267,7 @@ int ct_scene_upload(ct_ctx* c, const ct_scene_view* v) {
     }
 
     // host staging of the device layouts
-    std::vector<float4> tris((size_t)N * 3), sh_pos((size_t)N * 3), sh_nrm((size_t)N * 4);
+    std::vector<float4> tris((size_t)N * 3);
     for (uint32_t k = 0; k < N; ++k) {
         const uint32_t prim = v->prim_indices[k];
         const float* p0 = v->vertices + 3 * (size_t)v->indices[3 * (size_t)prim];
@@ -214,18 +277,8 @@ int ct_scene_upload(ct_ctx* c, const ct_scene_view* v) {
         tris[3 * (size_t)k + 1] = make_float4(p1[0] - p0[0], p1[1] - p0[1], p1[2] - p0[2], 0.0f);
         tris[3 * (size_t)k + 2] = make_float4(p2[0] - p0[0], p2[1] - p0[1], p2[2] - p0[2], 0.0f);
     }
-    for (uint32_t prim = 0; prim < N; ++prim) {
-        for (int j = 0; j < 3; ++j) {
-            const uint32_t vi = v->indices[3 * (size_t)prim + j];
-            sh_pos[3 * (size_t)prim + j] = f4(v->vertices + 3 * (size_t)vi, j == 0 ? ubits(v->material_idx[prim]) : 0.0f);
-            sh_nrm[4 * (size_t)prim + j] = f4(v->normals + 3 * (size_t)vi, 0.0f);
-        }
-        sh_nrm[4 * (size_t)prim + 3] = f4(v->center_normals + 3 * (size_t)prim, 0.0f);
-    }
-    std::vector<float4> mats((size_t)v->n_materials * 2);
-    for (uint32_t m = 0; m < v->n_materials; ++m) mat_rows(v->materials[m], &mats[2 * (size_t)m]);
-    std::vector<float4> lights((size_t)std::max<uint32_t>(v->n_lights, 1u) * 5);
-    for (uint32_t l = 0; l < v->n_lights; ++l) light_rows(v, v->light_indices[l], v->materials, &lights[5 * (size_t)l]);
+    Shading sh;
+    stage_shading(v, sh);
     std::vector<ct_mbvh_node> n4(v->mbvh_nodes, v->mbvh_nodes + v->n_mbvh_nodes);
     std::vector<ct_bvh_node> n2(v->bvh_nodes, v->bvh_nodes + v->n_bvh_nodes);
     std::vector<uint32_t> fn;
@@ -235,11 +288,8 @@ int ct_scene_upload(ct_ctx* c, const ct_scene_view* v) {
     sync_lanes(c);
     free_scene(c);
     ctd::DevScene sc{};
-    int r = upload_vec(c, tris, &sc.tris, "triangles");
-    if (r == CT_OK) r = upload_vec(c, sh_pos, &sc.sh_pos, "shading positions");
-    if (r == CT_OK) r = upload_vec(c, sh_nrm, &sc.sh_nrm, "shading normals");
-    if (r == CT_OK) r = upload_vec(c, mats, &sc.mats, "materials");
-    if (r == CT_OK) r = upload_vec(c, lights, &sc.lights, "lights");
+    r = upload_vec(c, tris, &sc.tris, "triangles");
+    if (r == CT_OK) r = upload_shading(c, sh, sc);
     if (r == CT_OK && !n4.empty()) r = upload_vec(c, n4, &sc.nodes4, "quad nodes");
     if (r == CT_OK) {
         const float4* fp = nullptr;
@@ -249,27 +299,92 @@ int ct_scene_upload(ct_ctx* c, const ct_scene_view* v) {
     if (r == CT_OK && !n2.empty()) r = upload_vec(c, n2, &sc.nodes2, "binary nodes");
     if (r != CT_OK) { free_scene(c); return r; }
     HIPCHK(hipStreamSynchronize(c->ln[0].stream), "scene upload");
-    sc.n_tris = N;
     sc.n_nodes4 = v->n_mbvh_nodes;
     sc.n_nodes2 = v->n_bvh_nodes;
     sc.n_fnodes = (uint32_t)(fn.size() / ct::kWideNodeWords);
-    sc.n_mats = v->n_materials;
-    sc.n_lights = v->n_lights;
-    sc.sky_enabled = v->sky_enabled ? 1 : 0;
-    sc.sky_r = v->sky[0];
-    sc.sky_g = v->sky[1];
-    sc.sky_b = v->sky[2];
-    c->sc = sc;
-    c->has_bvh4 = v->n_mbvh_nodes > 0;
-    c->has_bvh2 = v->n_bvh_nodes > 0;
-    c->wide_stack_bound = wide_bound;
-    c->has_scene = true;
-    c->h_mats.assign(v->materials, v->materials + v->n_materials);
-    c->h_mat_idx.assign(v->material_idx, v->material_idx + N);
-    c->h_light_idx.assign(v->light_indices, v->light_indices + v->n_lights);
-    c->h_light_rows.assign(reinterpret_cast<const float*>(lights.data()),
-                           reinterpret_cast<const float*>(lights.data()) + lights.size() * 4);
-    drop_graphs(c);
+    install_scene(c, v, sc, sh, wide_bound);
+    return CT_OK;
+}
+
+int ct_scene_upload_build(ct_ctx* c, const ct_scene_view* v, uint32_t flags, ct_build_stats* st) {
+    if (!c || !v) return CT_ERR_INVALID_ARG;
+    if (flags & ~(uint32_t)CT_BUILD_WIDE) return fail(c, CT_ERR_INVALID_ARG, "unknown build flags");
+    (void)hipSetDevice(c->device);
+    const auto t0 = std::chrono::steady_clock::now();
+    int r = check_geometry(c, v);
+    if (r != CT_OK) return r;
+    Shading sh;
+    stage_shading(v, sh);
+    sync_lanes(c);
+    free_scene(c);
+    ctd::DevScene sc{};
+    ct_build_stats bs{};
+    DeviceTrees t;
+    r = device_build(c, v, &t, &bs);
+    for (void* p : {(void*)t.tris, (void*)t.nodes2, (void*)t.nodes4})
+        if (p) c->scene_bufs.push_back(p);
+    if (r != CT_OK) { free_scene(c); return r; }
+    sc.tris = t.tris;
+    sc.nodes2 = reinterpret_cast<const float4*>(t.nodes2);
+    sc.nodes4 = reinterpret_cast<const float4*>(t.nodes4);
+    sc.n_nodes2 = t.n2;
+    sc.n_nodes4 = t.n4;
+    r = upload_shading(c, sh, sc);
+    uint32_t wide_bound = 0;
+    if (r == CT_OK && (flags & CT_BUILD_WIDE)) {
+        // wide nodes are collapsed on the host from the device-built trees
+        std::vector<ct_bvh_node> n2(t.n2);
+        std::vector<ct_mbvh_node> n4(t.n4);
+        HIPCHK(hipMemcpy(n2.data(), t.nodes2, n2.size() * sizeof(ct_bvh_node), hipMemcpyDeviceToHost), "read trees");
+        HIPCHK(hipMemcpy(n4.data(), t.nodes4, n4.size() * sizeof(ct_mbvh_node), hipMemcpyDeviceToHost), "read trees");
+        ct_scene_view tv = *v;
+        tv.n_bvh_nodes = t.n2;
+        tv.bvh_nodes = n2.data();
+        tv.n_mbvh_nodes = t.n4;
+        tv.mbvh_nodes = n4.data();
+        std::vector<uint32_t> fn;
+        if (ct::build_wide_nodes(tv, fn, wide_bound)) r = fail(c, CT_ERR_INVALID_SCENE, "tree bounds are not finite");
+        const float4* fp = nullptr;
+        if (r == CT_OK) r = upload_vec(c, fn, &fp, "wide nodes");
+        sc.fnodes = reinterpret_cast<const uint4*>(fp);
+        sc.n_fnodes = (uint32_t)(fn.size() / ct::kWideNodeWords);
+    }
+    if (r != CT_OK) { free_scene(c); return r; }
+    HIPCHK(hipStreamSynchronize(c->ln[0].stream), "scene upload");
+    install_scene(c, v, sc, sh, wide_bound);
+    bs.total_ms = (float)std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    if (st) *st = bs;
+    return CT_OK;
+}
+
+int ct_scene_tree_sizes(ct_ctx* c, uint32_t* n_bvh, uint32_t* n_mbvh, uint32_t* n_prims) {
+    if (!c) return CT_ERR_INVALID_ARG;
+    if (!c->has_scene) return fail(c, CT_ERR_NO_SCENE, "no scene uploaded");
+    if (n_bvh) *n_bvh = c->has_bvh2 ? c->sc.n_nodes2 : 0;
+    if (n_mbvh) *n_mbvh = c->has_bvh4 ? c->sc.n_nodes4 : 0;
+    if (n_prims) *n_prims = c->sc.n_tris;
+    return CT_OK;
+}
+
+int ct_scene_read_trees(ct_ctx* c, ct_bvh_node* bvh, uint32_t cap_bvh, ct_mbvh_node* mbvh, uint32_t cap_mbvh,
+                        uint32_t* prim_indices, uint32_t cap_prims) {
+    if (!c) return CT_ERR_INVALID_ARG;
+    if (!c->has_scene) return fail(c, CT_ERR_NO_SCENE, "no scene uploaded");
+    (void)hipSetDevice(c->device);
+    sync_lanes(c);
+    const uint32_t n2 = c->has_bvh2 ? c->sc.n_nodes2 : 0, n4 = c->has_bvh4 ? c->sc.n_nodes4 : 0, N = c->sc.n_tris;
+    if ((bvh && cap_bvh < n2) || (mbvh && cap_mbvh < n4) || (prim_indices && cap_prims < N))
+        return fail(c, CT_ERR_INVALID_ARG, "output capacity too small");
+    if (bvh && n2) HIPCHK(hipMemcpy(bvh, c->sc.nodes2, n2 * sizeof(ct_bvh_node), hipMemcpyDeviceToHost), "read trees");
+    if (mbvh && n4)
+        HIPCHK(hipMemcpy(mbvh, c->sc.nodes4, n4 * sizeof(ct_mbvh_node), hipMemcpyDeviceToHost), "read trees");
+    if (prim_indices) {
+        // primitive id of slot k: the w word of its first record, without the last-in-leaf bit
+        HIPCHK(hipMemcpy2D(prim_indices, 4, reinterpret_cast<const char*>(c->sc.tris) + 12, 3 * sizeof(float4), 4, N,
+                           hipMemcpyDeviceToHost),
+               "read trees");
+        for (uint32_t k = 0; k < N; ++k) prim_indices[k] &= 0x7FFFFFFFu;
+    }
     return CT_OK;
 }
 

@@ -122,5 +122,19 @@ int run_paths(ct_ctx* c, const std::vector<ctd::ViewDesc>& views, const std::vec
 
 }  // namespace ctr
 
+namespace ctr {
+// Trees and triangle records built on the device (allocated with dalloc;
+// the caller owns them, also when the build fails part-way).
+struct DeviceTrees {
+    float4* tris = nullptr;
+    ct_bvh_node* nodes2 = nullptr;
+    ct_mbvh_node* nodes4 = nullptr;
+    uint32_t n2 = 0, n4 = 0;
+};
+// Build the binary and quad trees of v's triangles on lane 0's stream; the
+// result equals the host build byte for byte.
+int device_build(ct_ctx* c, const ct_scene_view* v, DeviceTrees* out, ct_build_stats* st);
+}  // namespace ctr
+
 // Set the launch shape (block, LDS stack depth) and size the persistent grid.
 extern "C" void ctr_apply_shape(ct_ctx* c, int block, int lds_depth);

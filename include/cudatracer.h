@@ -27,7 +27,7 @@
 extern "C" {
 #endif
 
-#define CT_ABI_VERSION 2
+#define CT_ABI_VERSION 3
 
 /* ---------------------------------------------------------------- status */
 enum {
@@ -197,6 +197,32 @@ int ct_device_name(const ct_ctx* ctx, char* buf, size_t len);
 
 /* Application::allocateTriangles (Application.cpp:156-193) + node export. */
 int ct_scene_upload(ct_ctx* ctx, const ct_scene_view* scene);
+/* BVHTree::constructBVH + MBVHTree::constructBVH (BVHTree.cpp:22-58,
+ * MBVHTree.cpp:15-30, run by Application::init, Application.cpp:132-135) on
+ * the device, then the upload of ct_scene_upload.  The view's prim_indices
+ * and trees are ignored.  The binary tree, quad tree and primitive order are
+ * byte-identical to the host build of the same view (the reference's
+ * single-threaded node numbering).  CT_BUILD_WIDE also collapses the wide
+ * tree of CT_TRAV_FAST (on the host, from the device-built trees); without it
+ * CT_TRAV_FAST is refused for this scene.  stats may be NULL.  On failure no
+ * scene is loaded. */
+enum { CT_BUILD_WIDE = 1 };
+typedef struct ct_build_stats {
+    uint32_t n_bvh_nodes, n_mbvh_nodes;
+    uint32_t levels;      /* tree levels processed                         */
+    uint32_t max_depth;   /* BVHTree::m_MaxDepth (root depth 2)            */
+    uint32_t max_chain;   /* longest partition swap chain if >= 16, else 0  */
+    float device_ms;      /* first to last build kernel (read-backs included) */
+    float total_ms;       /* whole call: checks, staging, build, uploads    */
+    uint32_t reserved;
+} ct_build_stats;
+int ct_scene_upload_build(ct_ctx* ctx, const ct_scene_view* scene, uint32_t flags, ct_build_stats* stats);
+/* Node counts of the current scene's trees (0 for a tree not present). */
+int ct_scene_tree_sizes(ct_ctx* ctx, uint32_t* n_bvh, uint32_t* n_mbvh, uint32_t* n_prims);
+/* Copy the current scene's trees and primitive order back (BVHTree::m_BVHPool,
+ * MBVHTree::m_Tree, m_PrimitiveIndices layouts); NULL outputs are skipped. */
+int ct_scene_read_trees(ct_ctx* ctx, ct_bvh_node* bvh, uint32_t cap_bvh, ct_mbvh_node* mbvh, uint32_t cap_mbvh,
+                        uint32_t* prim_indices, uint32_t cap_prims);
 /* Application::updateMaterials (Application.cpp:222-225). */
 int ct_material_update(ct_ctx* ctx, uint32_t index, const ct_material* mat);
 /* Resize callback + allocateBuffers (Application.cpp:64-75,141-154); resets. */
