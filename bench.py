@@ -73,6 +73,9 @@ def main() -> int:
     ap.add_argument("--cpu-seconds", type=float, default=10.0, help="minimum CPU-baseline work")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--no-build", action="store_true", help="skip the device tree-build leg")
+    ap.add_argument("--build-only", action="store_true",
+                    help="only the tree-build leg (device build vs host build of the workload's scene)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (calibrates FETCH_SIZE / WRITE_SIZE)")
     ap.add_argument("--summarize", nargs="+", metavar="DB", help="reduce rocprofv3 result databases and exit")
@@ -102,6 +105,14 @@ def main() -> int:
     cam = scene.camera(W, H)
     st_scene = scene.stats()
     ctx = ct.Context(0, block_size=args.block, stack_lds=args.stack_lds)
+    if args.build_only:
+        out = {"workload": WORKLOADS[args.config] % (st_scene["triangles"], args.traversal), "device": ctx.name(),
+               "build": build_leg(ctx, scene, wide=args.traversal == "fast", reps=max(args.steps, 1))}
+        ctx.close()
+        if not args.no_cpu and args.config == 2:
+            out["build"]["cpu_baseline"] = cpu_build_baseline(scene)
+        print(json.dumps(out))
+        return 0
     ctx.upload(scene)
     ctx.resize(W, H)
 
@@ -204,7 +215,12 @@ def main() -> int:
     if not args.no_serve and args.config == 2:
         out["served"] = serve_leg(ct, ctx, scene, opts, args.seed, chunk=args.serve_chunk)
         out["batched"] = batched_leg(ct, ctx, args.seed)
+    if not args.no_build:
+        # replaces the scene with its device-built twin (identical trees and records)
+        out["build"] = build_leg(ctx, scene, wide=args.traversal == "fast")
     ctx.close()
+    if not args.no_cpu and not args.no_build and args.config == 2:
+        out["build"]["cpu_baseline"] = cpu_build_baseline(scene)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(scene, cam, opts, W, H, args.cpu_seconds,
                                            strip=1 if args.config == 2 else 16)
@@ -390,6 +406,10 @@ def batched_leg(ct, ctx, seed: int, n_views: int = 64) -> dict:
             "bit_exact_vs_plain": f"{same}/8"}
 
 
+def _kname(name: str) -> str:
+    return name.replace("(anonymous namespace)::", "").split("(")[0]
+
+
 def summarize(paths: list) -> int:
     """Reduce rocprofv3 result databases (rocpd SQLite) to per-kernel
     duration statistics and per-dispatch counter averages."""
@@ -403,7 +423,7 @@ def summarize(paths: list) -> int:
         print("| kernel | calls | total ms | avg us | min us | max us | % |")
         print("|---|---|---|---|---|---|---|")
         for r in rows:
-            print(f"| {r[0].split('(')[0]} | {r[1]} | {r[2] / 1e6:.3f} | {r[3] / 1e3:.1f} | {r[4] / 1e3:.1f} | "
+            print(f"| {_kname(r[0])} | {r[1]} | {r[2] / 1e6:.3f} | {r[3] / 1e3:.1f} | {r[4] / 1e3:.1f} | "
                   f"{r[5] / 1e3:.1f} | {100 * r[2] / tot:.1f} |")
         # a counter can come as one row per hardware instance (XCD, SE):
         # hardware counters are summed per dispatch (GRBM_GUI_ACTIVE, a clock
@@ -418,7 +438,7 @@ def summarize(paths: list) -> int:
             print("| kernel | counter | mean per dispatch | dispatches |")
             print("|---|---|---|---|")
             for r in pmc:
-                print(f"| {r[0].split('(')[0]} | {r[1]} | {r[2]:.1f} | {r[3]} |")
+                print(f"| {_kname(r[0])} | {r[1]} | {r[2]:.1f} | {r[3]} |")
     return 0
 
 
@@ -433,6 +453,53 @@ def copy_probe(mib: int) -> int:
     torch.cuda.synchronize()
     print(json.dumps({"copy_probe_bytes": n * 4, "dispatches": 5}))
     return 0
+
+
+def build_leg(ctx, scene, wide: bool, reps: int = 3) -> dict:
+    """Device tree build (ct_scene_upload_build) of the workload's scene
+    against the host path it replaces: the host BVHTree + MBVHTree builds
+    (single thread) followed by ct_scene_upload.  Best of reps device builds;
+    the trees read back must equal the host build byte for byte."""
+    import numpy as np
+    hs = scene.stats()
+    t0 = time.perf_counter()
+    ctx.upload(scene)
+    host_upload_ms = (time.perf_counter() - t0) * 1e3
+    best = None
+    for _ in range(reps):
+        st = ctx.upload_build(scene, wide=wide)
+        if best is None or st["device_ms"] < best["device_ms"]:
+            best = st
+    a = scene.arrays()
+    t = ctx.read_trees()
+    same = all(np.array_equal(t[k], a[k]) for k in ("bvh", "mbvh", "prim_indices"))
+    host_build_ms = (hs["bvh_build_us"] + hs["mbvh_build_us"]) / 1e3
+    return {"triangles": hs["triangles"], "bvh_nodes": best["n_bvh_nodes"], "mbvh_nodes": best["n_mbvh_nodes"],
+            "levels": best["levels"], "max_swap_chain": best["max_chain"],
+            "device_build_ms": round(best["device_ms"], 2),
+            "upload_build_ms": round(best["total_ms"], 1),
+            "host_build_ms": round(host_build_ms, 1),
+            "host_upload_ms": round(host_upload_ms, 1),
+            "host_path_ms": round(host_build_ms + host_upload_ms, 1),
+            "speedup_build": round(host_build_ms / max(best["device_ms"], 1e-3), 1),
+            "speedup_scene_swap": round((host_build_ms + host_upload_ms) / max(best["total_ms"], 1e-3), 1),
+            "wide_tree": bool(wide), "identical_to_host": bool(same),
+            "note": "host_upload_ms includes the host collapse of the wide tree; upload_build_ms includes "
+                    "it only with wide_tree"}
+
+
+def cpu_build_baseline(scene) -> dict:
+    """The oracle's restatement of the reference builder (30 full plane
+    scans per node, BVHNode.cu:121-191, then the quad collapse) on one host
+    thread, for the same scene."""
+    from oracle import oracle as orc
+    osc = orc.Scene(scene.arrays(), build=False)
+    t0 = time.perf_counter()
+    if orc.lib.or_scene_build(osc._h) != 0:
+        raise RuntimeError("oracle build failed")
+    dt = time.perf_counter() - t0
+    return {"value": round(dt * 1e3, 1), "unit": "ms per build", "cores": 1, "kind": "port",
+            "sample": f"one build of the {scene.stats()['triangles']}-triangle scene"}
 
 
 def cpu_baseline(scene, cam, opts, W, H, min_seconds: float, max_frames: int = 256, strip: int = 1) -> dict:

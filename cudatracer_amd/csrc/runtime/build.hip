@@ -44,6 +44,8 @@ constexpr int kPlanes = kBins - 1;
 constexpr int kCands = 3 * kPlanes;
 constexpr int kBlock = 256;
 constexpr uint32_t kChunk = 2048;  // slots per workgroup
+constexpr uint32_t kTiny = 256;    // nodes up to this many slots take a one-wave workgroup
+constexpr uint32_t kGroup = 64;    // chunk bins reduced per block in the first stage
 constexpr int kBinWords = 3 * kBins * 7;  // per axis and bin: ordered min xyz, max xyz, count
 constexpr uint32_t kMaxPrims = 4;  // BVHNode.cu:6
 constexpr int kMaxDepth = 64;      // BVHNode.cu:7; the root has depth 2
@@ -114,7 +116,8 @@ __device__ void write_record(const Geo& g, uint32_t pos, uint32_t prim, bool las
     g.tris[3 * (size_t)pos + 2] = make_float4(p2[0] - x0, p2[1] - y0, p2[2] - z0, 0.0f);
 }
 
-// Exclusive scan over the block (all threads call it).
+// Exclusive scan over a block of BS threads (all threads call it).
+template <int BS = kBlock>
 __device__ uint32_t block_scan(uint32_t v, uint32_t* s_w, uint32_t& total) {
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     uint32_t x = v;
@@ -127,7 +130,7 @@ __device__ uint32_t block_scan(uint32_t v, uint32_t* s_w, uint32_t& total) {
     __syncthreads();
     uint32_t base = 0, tot = 0;
 #pragma unroll
-    for (int k = 0; k < kBlock / 64; ++k) {
+    for (int k = 0; k < BS / 64; ++k) {
         const uint32_t s = s_w[k];
         base += k < w ? s : 0u;
         tot += s;
@@ -150,9 +153,9 @@ struct BinLds {
 
 __device__ void init_bins(BinLds& s, const NodeRec& nd) {
     const int t = threadIdx.x;
-    if (t < kBinWords) {
-        const int k = t % 7;
-        s.bins[t] = k < 3 ? ord(1e34f) : k < 6 ? ord(-1e34f) : 0u;
+    for (int w = t; w < kBinWords; w += blockDim.x) {
+        const int k = w % 7;
+        s.bins[w] = k < 3 ? ord(1e34f) : k < 6 ? ord(-1e34f) : 0u;
     }
     if (t < kCands) {
         const int axis = t / kPlanes, i = t % kPlanes + 1;
@@ -339,35 +342,52 @@ __global__ __launch_bounds__(kBlock) void k_root(const float* part, uint32_t npa
 
 // Per level node: chunk counts of the nodes that try to split (count >= 4,
 // depth < 64); clears the split flag.
-__global__ void k_plan(const NodeRec* nodes, uint32_t n, int level, uint32_t* ns, uint32_t* nm, uint32_t* split) {
+// Per level node: which of the three node kernels splits it (tiny: one wave,
+// single: one workgroup, multi: chunks), for the nodes that try to split
+// (count >= 4, depth < 64); clears the split flag.
+__global__ void k_plan(const NodeRec* nodes, uint32_t n, int level, uint32_t* nt, uint32_t* ns, uint32_t* nm,
+                       uint32_t* split) {
     const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= n) return;
     const uint32_t count = nodes[j].count;
     const bool act = active(count, level);
     const uint32_t nch = (count + kChunk - 1) / kChunk;
-    ns[j] = act && nch == 1 ? 1u : 0u;
+    nt[j] = act && count <= kTiny ? 1u : 0u;
+    ns[j] = act && count > kTiny && nch == 1 ? 1u : 0u;
     nm[j] = act && nch > 1 ? nch : 0u;
     split[j] = 0;
 }
 
-__global__ void k_chunks(uint32_t n, const uint32_t* ns, const uint32_t* off_s, const uint32_t* nm,
-                         const uint32_t* off_m, uint32_t* sch, uint32_t* mch_node, uint32_t* mch_k) {
+__global__ void k_lists(uint32_t n, const uint32_t* nt, const uint32_t* off_t, const uint32_t* ns,
+                        const uint32_t* off_s, uint32_t* tch, uint32_t* sch) {
     const uint32_t j = blockIdx.x * blockDim.x + threadIdx.x;
     if (j >= n) return;
+    if (nt[j]) tch[off_t[j]] = j;
     if (ns[j]) sch[off_s[j]] = j;
-    const uint32_t m = nm[j];
-    for (uint32_t k = 0; k < m; ++k) {
-        mch_node[off_m[j] + k] = j;
-        mch_k[off_m[j] + k] = k;
-    }
 }
 
-// A node of at most kChunk primitives, from bins to partition.
-__global__ __launch_bounds__(kBlock) void k_single(const NodeRec* nodes, int level, const uint32_t* sch, Slots b,
-                                                   Geo g, NodeRec* kid, uint32_t* split, uint32_t* err,
-                                                   uint32_t* chain) {
+// Chunk c of the large nodes: its node is the last with chunk offset <= c.
+__global__ void k_mchunks(uint32_t tm, uint32_t n, const uint32_t* off_m, uint32_t* mch_node, uint32_t* mch_k) {
+    const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= tm) return;
+    uint32_t lo = 0, hi = n;  // first index with off_m > c
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi) / 2;
+        if (off_m[mid] <= c) lo = mid + 1;
+        else hi = mid;
+    }
+    const uint32_t j = lo - 1;
+    mch_node[c] = j;
+    mch_k[c] = c - off_m[j];
+}
+
+// A node of at most CAP primitives, from bins to partition, by one
+// workgroup of BS threads.
+template <int BS, uint32_t CAP>
+__global__ __launch_bounds__(BS) void k_single(const NodeRec* nodes, int level, const uint32_t* sch, Slots b, Geo g,
+                                               NodeRec* kid, uint32_t* split, uint32_t* err, uint32_t* chain) {
     __shared__ BinLds s;
-    __shared__ uint32_t s_rf[kChunk];
+    __shared__ uint32_t s_rf[CAP];
     const uint32_t j = sch[blockIdx.x];
     const NodeRec nd = nodes[j];
     const uint32_t first = nd.first, count = nd.count;
@@ -375,24 +395,24 @@ __global__ __launch_bounds__(kBlock) void k_single(const NodeRec* nodes, int lev
     const float4* ibox = b.box[in];
     init_bins(s, nd);
     __syncthreads();
-    for (uint32_t r = threadIdx.x; r < count; r += kBlock)
+    for (uint32_t r = threadIdx.x; r < count; r += BS)
         bin_slot(s, ibox[2 * (size_t)(first + r)], ibox[2 * (size_t)(first + r) + 1]);
     __syncthreads();
     decide(s, nd, kid + 2 * (size_t)j, split + j, nullptr, err);
     const Decision d = s.dec;
     if (!d.split) {  // SAH leaf
-        for (uint32_t r = threadIdx.x; r < count; r += kBlock)
+        for (uint32_t r = threadIdx.x; r < count; r += BS)
             write_record(g, first + r, b.ref[in][first + r], r == count - 1);
         return;
     }
     uint32_t running = 0;
-    for (uint32_t base = 0; base < count; base += kBlock) {
+    for (uint32_t base = 0; base < count; base += BS) {
         const uint32_t r = base + threadIdx.x;
         uint32_t f = 0;
         if (r < count)
             f = centroid(ibox[2 * (size_t)(first + r)], ibox[2 * (size_t)(first + r) + 1], d.axis) <= d.coord ? 1u : 0u;
         uint32_t tot;
-        const uint32_t ex = block_scan(f, s.w, tot);
+        const uint32_t ex = block_scan<BS>(f, s.w, tot);
         if (r < count) s_rf[r] = (running + ex) | (f << 31);
         running += tot;
     }
@@ -401,7 +421,7 @@ __global__ __launch_bounds__(kBlock) void k_single(const NodeRec* nodes, int lev
     const uint32_t nl = d.nleft, nr = count - nl;
     const bool lfin = !active(nl, level + 1), rfin = !active(nr, level + 1);
     uint32_t steps_max = 0;
-    for (uint32_t r = threadIdx.x; r < count; r += kBlock) {
+    for (uint32_t r = threadIdx.x; r < count; r += BS) {
         const uint32_t w = s_rf[r];
         if (w >> 31) {
             const uint32_t dst = w & kRank;
@@ -438,8 +458,28 @@ __global__ __launch_bounds__(kBlock) void k_mbin(const NodeRec* nodes, int level
     if (threadIdx.x < kBinWords) part[(size_t)c * kBinWords + threadIdx.x] = s.bins[threadIdx.x];
 }
 
-// Reduce a large node's chunk bins and decide (one block per node, run by
-// the block of its first chunk).
+__device__ __forceinline__ uint32_t bin_merge(int kind, uint32_t v, uint32_t x) {
+    return kind < 3 ? min(v, x) : kind < 6 ? max(v, x) : v + x;
+}
+
+// First reduction stage: the block of chunk k (k a multiple of kGroup)
+// merges the bins of chunks k .. k + kGroup - 1 of its node into chunk k's.
+__global__ __launch_bounds__(kBlock) void k_mreduce(const NodeRec* nodes, const uint32_t* mch_node,
+                                                    const uint32_t* mch_k, uint32_t* part) {
+    const uint32_t c = blockIdx.x, k = mch_k[c];
+    if (k % kGroup != 0) return;
+    const uint32_t nch = (nodes[mch_node[c]].count + kChunk - 1) / kChunk;
+    const uint32_t end = min(nch, k + kGroup);
+    const int t = threadIdx.x;
+    if (t >= kBinWords) return;
+    const int kind = t % 7;
+    uint32_t v = part[(size_t)c * kBinWords + t];
+    for (uint32_t q = k + 1; q < end; ++q) v = bin_merge(kind, v, part[(size_t)(c + q - k) * kBinWords + t]);
+    part[(size_t)c * kBinWords + t] = v;
+}
+
+// Second stage and decision (one block per node, run by the block of its
+// first chunk).
 __global__ __launch_bounds__(kBlock) void k_mdecide(const NodeRec* nodes, const uint32_t* mch_node,
                                                     const uint32_t* mch_k, const uint32_t* part, NodeRec* kid,
                                                     uint32_t* split, Decision* dec, uint32_t* err) {
@@ -454,10 +494,7 @@ __global__ __launch_bounds__(kBlock) void k_mdecide(const NodeRec* nodes, const 
     if (t < kBinWords) {
         const int kind = t % 7;
         uint32_t v = s.bins[t];
-        for (uint32_t q = 0; q < nch; ++q) {
-            const uint32_t x = part[(size_t)(c + q) * kBinWords + t];
-            v = kind < 3 ? min(v, x) : kind < 6 ? max(v, x) : v + x;
-        }
+        for (uint32_t q = 0; q < nch; q += kGroup) v = bin_merge(kind, v, part[(size_t)(c + q) * kBinWords + t]);
         s.bins[t] = v;
     }
     __syncthreads();
@@ -783,7 +820,8 @@ int device_build(ct_ctx* c, const ct_scene_view* v, DeviceTrees* out, ct_build_s
     const uint64_t Nn = 2 * (uint64_t)N + 2;                        // nodes of all levels
     const uint64_t Mc = 2 * (uint64_t)N / kChunk + 2;               // chunks of large nodes
     float* d_vert;
-    uint32_t *d_idx, *ref0, *ref1, *rf, *ns, *nm, *off_s, *off_m, *split, *coff, *sch, *mch_node, *mch_k, *part,
+    uint32_t *d_idx, *ref0, *ref1, *rf, *nt, *ns, *nm, *off_t, *off_s, *off_m, *split, *coff, *tch, *sch, *mch_node,
+        *mch_k, *part,
         *chunk_left, *chunk_pre, *S, *SQ, *R, *Q, *sums, *cnt;
     float4 *box0, *box1;
     NodeRec *nodes, *kid;
@@ -807,6 +845,9 @@ int device_build(ct_ctx* c, const ct_scene_view* v, DeviceTrees* out, ct_build_s
     GET(SQ, Nn, "build: subtree counts");
     GET(R, Nn, "build: ranks");
     GET(Q, Nn, "build: ranks");
+    GET(nt, W, "build: level");
+    GET(off_t, W, "build: level");
+    GET(tch, W, "build: level");
     GET(ns, W, "build: level");
     GET(nm, W, "build: level");
     GET(off_s, W, "build: level");
@@ -885,19 +926,27 @@ int device_build(ct_ctx* c, const ct_scene_view* v, DeviceTrees* out, ct_build_s
     for (int level = 0; n > 0; ++level) {
         levels.emplace_back(g0, n);
         const NodeRec* lv = nodes + g0;
-        k_plan<<<blocks_for(n, tpb), tpb, 0, s>>>(lv, n, level, ns, nm, split);
+        k_plan<<<blocks_for(n, tpb), tpb, 0, s>>>(lv, n, level, nt, ns, nm, split);
         BLAUNCH("build: plan");
-        if ((r = scan(ns, off_s, n, 0)) || (r = scan(nm, off_m, n, 1)) || (r = readback(2))) return r;
-        const uint32_t Ts = h[0], Tm = h[1];
-        if (Ts > W || Tm > Mc) return fail(c, CT_ERR_DEVICE, "build: chunk list overflow");
-        k_chunks<<<blocks_for(n, tpb), tpb, 0, s>>>(n, ns, off_s, nm, off_m, sch, mch_node, mch_k);
-        BLAUNCH("build: chunks");
+        if ((r = scan(ns, off_s, n, 0)) || (r = scan(nm, off_m, n, 1)) || (r = scan(nt, off_t, n, 6)) ||
+            (r = readback(7)))
+            return r;
+        const uint32_t Ts = h[0], Tm = h[1], Tt = h[6];
+        if (Ts > W || Tt > W || Tm > Mc) return fail(c, CT_ERR_DEVICE, "build: chunk list overflow");
+        k_lists<<<blocks_for(n, tpb), tpb, 0, s>>>(n, nt, off_t, ns, off_s, tch, sch);
+        BLAUNCH("build: node lists");
+        if (Tt) {
+            k_single<64, kTiny><<<Tt, 64, 0, s>>>(lv, level, tch, slots, geo, kid, split, cnt + 4, cnt + 5);
+            BLAUNCH("build: tiny nodes");
+        }
         if (Ts) {
-            k_single<<<Ts, kBlock, 0, s>>>(lv, level, sch, slots, geo, kid, split, cnt + 4, cnt + 5);
+            k_single<kBlock, kChunk><<<Ts, kBlock, 0, s>>>(lv, level, sch, slots, geo, kid, split, cnt + 4, cnt + 5);
             BLAUNCH("build: small nodes");
         }
         if (Tm) {
+            k_mchunks<<<blocks_for(Tm, tpb), tpb, 0, s>>>(Tm, n, off_m, mch_node, mch_k);
             k_mbin<<<Tm, kBlock, 0, s>>>(lv, level, mch_node, mch_k, slots, part);
+            k_mreduce<<<Tm, kBlock, 0, s>>>(lv, mch_node, mch_k, part);
             k_mdecide<<<Tm, kBlock, 0, s>>>(lv, mch_node, mch_k, part, kid, split, dec, cnt + 4);
             k_mrank<<<Tm, kBlock, 0, s>>>(lv, level, mch_node, mch_k, slots, dec, rf, chunk_left);
             BLAUNCH("build: large nodes");
