@@ -475,7 +475,7 @@ def build_leg(ctx, scene, wide: bool, reps: int = 3) -> dict:
     same = all(np.array_equal(t[k], a[k]) for k in ("bvh", "mbvh", "prim_indices"))
     host_build_ms = (hs["bvh_build_us"] + hs["mbvh_build_us"]) / 1e3
     return {"triangles": hs["triangles"], "bvh_nodes": best["n_bvh_nodes"], "mbvh_nodes": best["n_mbvh_nodes"],
-            "levels": best["levels"], "max_swap_chain": best["max_chain"],
+            "levels": best["levels"], "long_swap_chains": best["long_chains"], "jump_rounds": best["jump_rounds"],
             "device_build_ms": round(best["device_ms"], 2),
             "upload_build_ms": round(best["total_ms"], 1),
             "host_build_ms": round(host_build_ms, 1),

@@ -102,11 +102,11 @@ class FrameStats(C.Structure):
 
 class BuildStats(C.Structure):
     _fields_ = [("n_bvh_nodes", C.c_uint32), ("n_mbvh_nodes", C.c_uint32), ("levels", C.c_uint32),
-                ("max_depth", C.c_uint32), ("max_chain", C.c_uint32), ("device_ms", C.c_float),
-                ("total_ms", C.c_float), ("reserved", C.c_uint32)]
+                ("max_depth", C.c_uint32), ("long_chains", C.c_uint32), ("jump_rounds", C.c_uint32),
+                ("device_ms", C.c_float), ("total_ms", C.c_float)]
 
     def as_dict(self) -> dict:
-        return {k: getattr(self, k) for k, _ in self._fields_ if k != "reserved"}
+        return {k: getattr(self, k) for k, _ in self._fields_}
 
 
 class DeviceOpts(C.Structure):

@@ -46,6 +46,7 @@ constexpr int kBlock = 256;
 constexpr uint32_t kChunk = 2048;  // slots per workgroup
 constexpr uint32_t kTiny = 256;    // nodes up to this many slots take a one-wave workgroup
 constexpr uint32_t kGroup = 64;    // chunk bins reduced per block in the first stage
+constexpr uint32_t kChaseCap = 256;  // longer swap chains of large nodes are resolved by pointer jumping
 constexpr int kBinWords = 3 * kBins * 7;  // per axis and bin: ordered min xyz, max xyz, count
 constexpr uint32_t kMaxPrims = 4;  // BVHNode.cu:6
 constexpr int kMaxDepth = 64;      // BVHNode.cu:7; the root has depth 2
@@ -151,21 +152,22 @@ struct BinLds {
     uint32_t w[kBlock / 64];
 };
 
-__device__ void init_bins(BinLds& s, const NodeRec& nd) {
+__device__ __forceinline__ void init_bins(BinLds& s, const NodeRec& nd) {
     const int t = threadIdx.x;
     for (int w = t; w < kBinWords; w += blockDim.x) {
         const int k = w % 7;
         s.bins[w] = k < 3 ? ord(1e34f) : k < 6 ? ord(-1e34f) : 0u;
     }
-    if (t < kCands) {
-        const int axis = t / kPlanes, i = t % kPlanes + 1;
-        const float len = nd.mx[axis] - nd.mn[axis];
+    if (t < kPlanes) {
+        // plane i of every axis (constant field indices keep the record in registers)
+        const int i = t + 1;
         const float off = (float)i / (float)kBins;
-        s.planes[axis * kBins + i] = nd.mn[axis] + len * off;
+#pragma unroll
+        for (int axis = 0; axis < 3; ++axis) s.planes[axis * kBins + i] = nd.mn[axis] + (nd.mx[axis] - nd.mn[axis]) * off;
     }
 }
 
-__device__ void bin_slot(BinLds& s, const float4& lo, const float4& hi) {
+__device__ __forceinline__ void bin_slot(BinLds& s, const float4& lo, const float4& hi) {
     const float c[3] = {(lo.x + hi.x) * 0.5f, (lo.y + hi.y) * 0.5f, (lo.z + hi.z) * 0.5f};
     const uint32_t o[6] = {ord(lo.x), ord(lo.y), ord(lo.z), ord(hi.x), ord(hi.y), ord(hi.z)};
 #pragma unroll
@@ -183,10 +185,60 @@ __device__ void bin_slot(BinLds& s, const float4& lo, const float4& hi) {
     }
 }
 
+// Bin a run of consecutive slots: slots that fall in the same bin as the
+// previous one are merged in registers first (neighbouring slots are
+// spatially close), which cuts the LDS atomics on the few hot bin words.
+__device__ void bin_run(BinLds& s, const float4* box, size_t first, uint32_t n) {
+    int cur[3] = {-1, -1, -1};
+    float mn[3][3], mx[3][3];
+    uint32_t cnt[3] = {0, 0, 0};
+    auto flush = [&](int axis) {
+        if (cur[axis] < 0) return;
+        uint32_t* w = s.bins + (axis * kBins + (cur[axis] - 1)) * 7;
+        atomicMin(w + 0, ord(mn[axis][0]));
+        atomicMin(w + 1, ord(mn[axis][1]));
+        atomicMin(w + 2, ord(mn[axis][2]));
+        atomicMax(w + 3, ord(mx[axis][0]));
+        atomicMax(w + 4, ord(mx[axis][1]));
+        atomicMax(w + 5, ord(mx[axis][2]));
+        atomicAdd(w + 6, cnt[axis]);
+    };
+    for (uint32_t q = 0; q < n; ++q) {
+        const float4 lo = box[2 * (first + q)], hi = box[2 * (first + q) + 1];
+        const float c[3] = {(lo.x + hi.x) * 0.5f, (lo.y + hi.y) * 0.5f, (lo.z + hi.z) * 0.5f};
+#pragma unroll
+        for (int axis = 0; axis < 3; ++axis) {
+            int b = 1;
+            while (b < kBins && !(c[axis] <= s.planes[axis * kBins + b])) ++b;
+            if (b != cur[axis]) {
+                flush(axis);
+                cur[axis] = b;
+                mn[axis][0] = lo.x;
+                mn[axis][1] = lo.y;
+                mn[axis][2] = lo.z;
+                mx[axis][0] = hi.x;
+                mx[axis][1] = hi.y;
+                mx[axis][2] = hi.z;
+                cnt[axis] = 1;
+            } else {
+                mn[axis][0] = smin(mn[axis][0], lo.x);
+                mn[axis][1] = smin(mn[axis][1], lo.y);
+                mn[axis][2] = smin(mn[axis][2], lo.z);
+                mx[axis][0] = smax(mx[axis][0], hi.x);
+                mx[axis][1] = smax(mx[axis][1], hi.y);
+                mx[axis][2] = smax(mx[axis][2], hi.z);
+                ++cnt[axis];
+            }
+        }
+    }
+#pragma unroll
+    for (int axis = 0; axis < 3; ++axis) flush(axis);
+}
+
 // Costs of the 30 candidates (one thread each), then the first minimum in
 // axis-outer, plane-inner order (BVHNode.cu:133-165).  Ends with a barrier;
 // s.dec holds the decision.
-__device__ void decide(BinLds& s, const NodeRec& nd, NodeRec* kid, uint32_t* split_flag, Decision* dec_out,
+__device__ __forceinline__ void decide(BinLds& s, const NodeRec& nd, NodeRec* kid, uint32_t* split_flag, Decision* dec_out,
                        uint32_t* err) {
     const int t = threadIdx.x;
     if (t < kCands) {
@@ -194,17 +246,23 @@ __device__ void decide(BinLds& s, const NodeRec& nd, NodeRec* kid, uint32_t* spl
         float lmn[3] = {1e34f, 1e34f, 1e34f}, lmx[3] = {-1e34f, -1e34f, -1e34f};
         float rmn[3] = {1e34f, 1e34f, 1e34f}, rmx[3] = {-1e34f, -1e34f, -1e34f};
         uint32_t lc = 0, rc = 0;
-        for (int b = 1; b <= kBins; ++b) {
+        for (int b = 1; b <= i; ++b) {
             const uint32_t* w = s.bins + (axis * kBins + (b - 1)) * 7;
-            float* mn = b <= i ? lmn : rmn;
-            float* mx = b <= i ? lmx : rmx;
 #pragma unroll
             for (int k = 0; k < 3; ++k) {
-                mn[k] = smin(mn[k], unord(w[k]));
-                mx[k] = smax(mx[k], unord(w[3 + k]));
+                lmn[k] = smin(lmn[k], unord(w[k]));
+                lmx[k] = smax(lmx[k], unord(w[3 + k]));
             }
-            if (b <= i) lc += w[6];
-            else rc += w[6];
+            lc += w[6];
+        }
+        for (int b = i + 1; b <= kBins; ++b) {
+            const uint32_t* w = s.bins + (axis * kBins + (b - 1)) * 7;
+#pragma unroll
+            for (int k = 0; k < 3; ++k) {
+                rmn[k] = smin(rmn[k], unord(w[k]));
+                rmx[k] = smax(rmx[k], unord(w[3 + k]));
+            }
+            rc += w[6];
         }
         s.cost[t] = area(lmn, lmx) * (float)(int)lc + area(rmn, rmx) * (float)(int)rc;
         s.lcount[t] = lc;
@@ -217,14 +275,28 @@ __device__ void decide(BinLds& s, const NodeRec& nd, NodeRec* kid, uint32_t* spl
         }
     }
     __syncthreads();
-    if (t == 0) {
-        float lowest = 1e34f;
-        int best = -1;
-        for (int k = 0; k < kCands; ++k)
-            if (s.cost[k] < lowest) {
-                lowest = s.cost[k];
-                best = k;
+    if (t < 32) {
+        // first minimum below 1e34 in candidate order (strict <, as the
+        // sequential scan keeps the first of equal costs): the smallest cost,
+        // ties to the smaller index, NaN never chosen
+        float key = INFINITY;
+        int best = 255;
+        if (t < kCands && s.cost[t] < 1e34f) {
+            key = s.cost[t];
+            best = t;
+        }
+#pragma unroll
+        for (int d = 16; d >= 1; d >>= 1) {
+            const float ok = __shfl_xor(key, d, 32);
+            const int ob = __shfl_xor(best, d, 32);
+            if (ok < key || (ok == key && ob < best)) {
+                key = ok;
+                best = ob;
             }
+        }
+        if (best == 255) best = -1;
+        const float lowest = best >= 0 ? key : 1e34f;
+    if (t == 0) {
         const float parent = area(nd.mn, nd.mx) * (float)(int)nd.count;
         Decision d{0u, 0u, 0.0f, 0u};
         if (!(parent < lowest)) {
@@ -256,6 +328,7 @@ __device__ void decide(BinLds& s, const NodeRec& nd, NodeRec* kid, uint32_t* spl
         *split_flag = d.split;
         if (dec_out) *dec_out = d;
         s.dec = d;
+    }
     }
     __syncthreads();
 }
@@ -385,7 +458,7 @@ __global__ void k_mchunks(uint32_t tm, uint32_t n, const uint32_t* off_m, uint32
 // workgroup of BS threads.
 template <int BS, uint32_t CAP>
 __global__ __launch_bounds__(BS) void k_single(const NodeRec* nodes, int level, const uint32_t* sch, Slots b, Geo g,
-                                               NodeRec* kid, uint32_t* split, uint32_t* err, uint32_t* chain) {
+                                               NodeRec* kid, uint32_t* split, uint32_t* err) {
     __shared__ BinLds s;
     __shared__ uint32_t s_rf[CAP];
     const uint32_t j = sch[blockIdx.x];
@@ -393,10 +466,21 @@ __global__ __launch_bounds__(BS) void k_single(const NodeRec* nodes, int level, 
     const uint32_t first = nd.first, count = nd.count;
     const int in = level & 1;
     const float4* ibox = b.box[in];
+    constexpr int kPer = (int)(CAP / BS);
+    float cen[kPer][3];  // slot centroids, kept for the partition test
     init_bins(s, nd);
     __syncthreads();
-    for (uint32_t r = threadIdx.x; r < count; r += BS)
-        bin_slot(s, ibox[2 * (size_t)(first + r)], ibox[2 * (size_t)(first + r) + 1]);
+#pragma unroll
+    for (int k = 0; k < kPer; ++k) {
+        const uint32_t r = k * BS + threadIdx.x;
+        if (r < count) {
+            const float4 lo = ibox[2 * (size_t)(first + r)], hi = ibox[2 * (size_t)(first + r) + 1];
+            bin_slot(s, lo, hi);
+            cen[k][0] = (lo.x + hi.x) * 0.5f;
+            cen[k][1] = (lo.y + hi.y) * 0.5f;
+            cen[k][2] = (lo.z + hi.z) * 0.5f;
+        }
+    }
     __syncthreads();
     decide(s, nd, kid + 2 * (size_t)j, split + j, nullptr, err);
     const Decision d = s.dec;
@@ -406,21 +490,22 @@ __global__ __launch_bounds__(BS) void k_single(const NodeRec* nodes, int level, 
         return;
     }
     uint32_t running = 0;
-    for (uint32_t base = 0; base < count; base += BS) {
-        const uint32_t r = base + threadIdx.x;
-        uint32_t f = 0;
-        if (r < count)
-            f = centroid(ibox[2 * (size_t)(first + r)], ibox[2 * (size_t)(first + r) + 1], d.axis) <= d.coord ? 1u : 0u;
-        uint32_t tot;
-        const uint32_t ex = block_scan<BS>(f, s.w, tot);
-        if (r < count) s_rf[r] = (running + ex) | (f << 31);
-        running += tot;
+#pragma unroll
+    for (int k = 0; k < kPer; ++k) {
+        if ((uint32_t)k * BS < count) {  // block-uniform
+            const uint32_t r = k * BS + threadIdx.x;
+            const float cv = d.axis == 0 ? cen[k][0] : d.axis == 1 ? cen[k][1] : cen[k][2];
+            const uint32_t f = r < count && cv <= d.coord ? 1u : 0u;
+            uint32_t tot;
+            const uint32_t ex = block_scan<BS>(f, s.w, tot);
+            if (r < count) s_rf[r] = (running + ex) | (f << 31);
+            running += tot;
+        }
     }
     __syncthreads();
     if (threadIdx.x == 0 && running != d.nleft) atomicOr(err, kErrCount);
     const uint32_t nl = d.nleft, nr = count - nl;
     const bool lfin = !active(nl, level + 1), rfin = !active(nr, level + 1);
-    uint32_t steps_max = 0;
     for (uint32_t r = threadIdx.x; r < count; r += BS) {
         const uint32_t w = s_rf[r];
         if (w >> 31) {
@@ -428,18 +513,15 @@ __global__ __launch_bounds__(BS) void k_single(const NodeRec* nodes, int level, 
             move_slot(b, in, g, first + r, first + dst, lfin, dst == nl - 1);
         }
         if (r >= nl) {
-            uint32_t q = r, steps = 0;
+            uint32_t q = r;
             for (;;) {
                 const uint32_t x = s_rf[q];
                 if (!(x >> 31) || (x & kRank) >= q) break;
                 q = x & kRank;
-                ++steps;
             }
-            steps_max = max(steps_max, steps);
             move_slot(b, in, g, first + q, first + r, rfin, r == count - 1);
         }
     }
-    if (steps_max >= 16) atomicMax(chain, steps_max);
 }
 
 // Bins of one chunk of a large node.
@@ -452,8 +534,9 @@ __global__ __launch_bounds__(kBlock) void k_mbin(const NodeRec* nodes, int level
     init_bins(s, nd);
     __syncthreads();
     const uint32_t lo = k * kChunk, hi = min(nd.count, lo + kChunk);
-    for (uint32_t r = lo + threadIdx.x; r < hi; r += kBlock)
-        bin_slot(s, ibox[2 * (size_t)(nd.first + r)], ibox[2 * (size_t)(nd.first + r) + 1]);
+    constexpr uint32_t kRun = kChunk / kBlock;
+    const uint32_t r0 = lo + threadIdx.x * kRun;
+    if (r0 < hi) bin_run(s, ibox, (size_t)nd.first + r0, min(kRun, hi - r0));
     __syncthreads();
     if (threadIdx.x < kBinWords) part[(size_t)c * kBinWords + threadIdx.x] = s.bins[threadIdx.x];
 }
@@ -535,7 +618,9 @@ __global__ __launch_bounds__(kBlock) void k_mrank(const NodeRec* nodes, int leve
 __global__ __launch_bounds__(kBlock) void k_mmove(const NodeRec* nodes, int level, const uint32_t* mch_node,
                                                   const uint32_t* mch_k, Slots b, Geo g, const Decision* dec,
                                                   const uint32_t* rf, const uint32_t* chunk_pre,
-                                                  const uint32_t* chunk_left, uint32_t* err, uint32_t* chain) {
+                                                  const uint32_t* chunk_left, uint32_t* err, uint32_t* npend,
+                                                  uint32_t* pend_dst, uint32_t* pend_q, uint32_t* jflag,
+                                                  uint32_t* jmax) {
     const uint32_t c = blockIdx.x, j = mch_node[c], k = mch_k[c];
     const Decision d = dec[j];
     const NodeRec nd = nodes[j];
@@ -554,7 +639,6 @@ __global__ __launch_bounds__(kBlock) void k_mmove(const NodeRec* nodes, int leve
     }
     const uint32_t nl = d.nleft, nr = count - nl;
     const bool lfin = !active(nl, level + 1), rfin = !active(nr, level + 1);
-    uint32_t steps_max = 0;
     for (uint32_t r = lo + threadIdx.x; r < hi; r += kBlock) {
         const uint32_t w = rf[first + r];
         if (w >> 31) {
@@ -562,20 +646,65 @@ __global__ __launch_bounds__(kBlock) void k_mmove(const NodeRec* nodes, int leve
             move_slot(b, in, g, first + r, first + dst, lfin, dst == nl - 1);
         }
         if (r >= nl) {
-            uint32_t q = r, steps = 0;
-            for (;;) {
+            uint32_t q = r;
+            bool done = false;
+            for (uint32_t steps = 0; steps < kChaseCap; ++steps) {
                 const uint32_t x = rf[first + q];
-                if (!(x >> 31)) break;
+                if (!(x >> 31)) { done = true; break; }
                 const uint32_t rk = chunk_pre[c0 + q / kChunk] - base + (x & kRank);
-                if (rk >= q) break;
+                if (rk >= q) { done = true; break; }
                 q = rk;
-                ++steps;
             }
-            steps_max = max(steps_max, steps);
-            move_slot(b, in, g, first + q, first + r, rfin, r == count - 1);
+            if (done) {
+                move_slot(b, in, g, first + q, first + r, rfin, r == count - 1);
+            } else {  // finished by pointer jumping (k_jump), then k_pending
+                if (!jflag[j]) {
+                    jflag[j] = 1;  // benign race: every writer stores 1
+                    atomicMax(jmax, count);
+                }
+                const uint32_t i = atomicAdd(npend, 1u);
+                pend_dst[i] = (first + r) | (rfin ? 0x80000000u : 0u) | (r == count - 1 ? 0x40000000u : 0u);
+                pend_q[i] = first + q;
+            }
         }
     }
-    if (steps_max >= 16) atomicMax(chain, steps_max);
+}
+
+// Pointer jumping over the swap chains of the large split nodes: J[p] is
+// the slot p's chain leads to (rank(p) while p holds a left primitive with
+// a smaller rank, else p itself).
+__global__ __launch_bounds__(kBlock) void k_jinit(const NodeRec* nodes, const uint32_t* mch_node,
+                                                  const uint32_t* mch_k, const uint32_t* jflag, const uint32_t* rf,
+                                                  const uint32_t* chunk_pre, uint32_t* J) {
+    const uint32_t c = blockIdx.x, j = mch_node[c], k = mch_k[c];
+    if (!jflag[j]) return;
+    const NodeRec nd = nodes[j];
+    const uint32_t lo = k * kChunk, hi = min(nd.count, lo + kChunk), base = chunk_pre[c - k];
+    for (uint32_t r = lo + threadIdx.x; r < hi; r += kBlock) {
+        const uint32_t x = rf[nd.first + r];
+        const uint32_t rk = chunk_pre[c] - base + (x & kRank);
+        J[nd.first + r] = nd.first + ((x >> 31) && rk < r ? rk : r);
+    }
+}
+
+// One doubling round over the slots of the nodes with long chains: after
+// round m, J[p] is 2^m steps down p's chain (or its end).
+__global__ __launch_bounds__(kBlock) void k_jump(const NodeRec* nodes, const uint32_t* mch_node,
+                                                 const uint32_t* mch_k, const uint32_t* jflag, const uint32_t* Jin,
+                                                 uint32_t* Jout) {
+    const uint32_t c = blockIdx.x, j = mch_node[c], k = mch_k[c];
+    if (!jflag[j]) return;
+    const NodeRec nd = nodes[j];
+    const uint32_t lo = k * kChunk, hi = min(nd.count, lo + kChunk);
+    for (uint32_t r = lo + threadIdx.x; r < hi; r += kBlock) Jout[nd.first + r] = Jin[Jin[nd.first + r]];
+}
+
+__global__ void k_pending(uint32_t n, const uint32_t* pend_dst, const uint32_t* pend_q, const uint32_t* J, Slots b,
+                          int level, Geo g) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t d = pend_dst[i];
+    move_slot(b, level & 1, g, J[pend_q[i]], d & 0x3FFFFFFFu, (d >> 31) != 0, ((d >> 30) & 1u) != 0);
 }
 
 // Children of the split nodes, in level order, appended after the level.
@@ -821,7 +950,7 @@ int device_build(ct_ctx* c, const ct_scene_view* v, DeviceTrees* out, ct_build_s
     const uint64_t Mc = 2 * (uint64_t)N / kChunk + 2;               // chunks of large nodes
     float* d_vert;
     uint32_t *d_idx, *ref0, *ref1, *rf, *nt, *ns, *nm, *off_t, *off_s, *off_m, *split, *coff, *tch, *sch, *mch_node,
-        *mch_k, *part,
+        *mch_k, *part, *J0, *J1, *pend_dst, *pend_q, *jflag,
         *chunk_left, *chunk_pre, *S, *SQ, *R, *Q, *sums, *cnt;
     float4 *box0, *box1;
     NodeRec *nodes, *kid;
@@ -839,6 +968,11 @@ int device_build(ct_ctx* c, const ct_scene_view* v, DeviceTrees* out, ct_build_s
     GET(ref0, N, "build: slot ids");
     GET(ref1, N, "build: slot ids");
     GET(rf, N, "build: ranks");
+    GET(J0, N, "build: chain jumps");
+    GET(J1, N, "build: chain jumps");
+    GET(pend_dst, N, "build: long chains");
+    GET(pend_q, N, "build: long chains");
+    GET(jflag, W, "build: long chains");
     GET(nodes, Nn, "build: nodes");
     GET(child, Nn, "build: children");
     GET(S, Nn, "build: subtree counts");
@@ -921,6 +1055,7 @@ int device_build(ct_ctx* c, const ct_scene_view* v, DeviceTrees* out, ct_build_s
     };
 
     std::vector<std::pair<uint32_t, uint32_t>> levels;  // [g0, n) per level
+    uint32_t long_chains = 0, jump_rounds = 0;
     uint32_t g0 = 0, n = 1;
     const uint32_t tpb = 256;
     for (int level = 0; n > 0; ++level) {
@@ -936,11 +1071,11 @@ int device_build(ct_ctx* c, const ct_scene_view* v, DeviceTrees* out, ct_build_s
         k_lists<<<blocks_for(n, tpb), tpb, 0, s>>>(n, nt, off_t, ns, off_s, tch, sch);
         BLAUNCH("build: node lists");
         if (Tt) {
-            k_single<64, kTiny><<<Tt, 64, 0, s>>>(lv, level, tch, slots, geo, kid, split, cnt + 4, cnt + 5);
+            k_single<64, kTiny><<<Tt, 64, 0, s>>>(lv, level, tch, slots, geo, kid, split, cnt + 4);
             BLAUNCH("build: tiny nodes");
         }
         if (Ts) {
-            k_single<kBlock, kChunk><<<Ts, kBlock, 0, s>>>(lv, level, sch, slots, geo, kid, split, cnt + 4, cnt + 5);
+            k_single<kBlock, kChunk><<<Ts, kBlock, 0, s>>>(lv, level, sch, slots, geo, kid, split, cnt + 4);
             BLAUNCH("build: small nodes");
         }
         if (Tm) {
@@ -951,11 +1086,32 @@ int device_build(ct_ctx* c, const ct_scene_view* v, DeviceTrees* out, ct_build_s
             k_mrank<<<Tm, kBlock, 0, s>>>(lv, level, mch_node, mch_k, slots, dec, rf, chunk_left);
             BLAUNCH("build: large nodes");
             if ((r = scan(chunk_left, chunk_pre, Tm, 3))) return r;
+            BCHK(hipMemsetAsync(cnt + 7, 0, 8, s), "build: counters");
+            BCHK(hipMemsetAsync(jflag, 0, (size_t)n * 4, s), "build: counters");
             k_mmove<<<Tm, kBlock, 0, s>>>(lv, level, mch_node, mch_k, slots, geo, dec, rf, chunk_pre, chunk_left,
-                                          cnt + 4, cnt + 5);
+                                          cnt + 4, cnt + 7, pend_dst, pend_q, jflag, cnt + 8);
             BLAUNCH("build: large nodes");
         }
-        if ((r = scan(split, coff, n, 2)) || (r = readback(6))) return r;
+        if ((r = scan(split, coff, n, 2)) || (r = readback(9))) return r;
+        const uint32_t npend = Tm ? h[7] : 0;
+        if (npend) {
+            // swap chains longer than the in-kernel chase: pointer jumping to
+            // their ends in the nodes that have them; a chain is shorter than
+            // its node, so ceil(log2(largest such node)) rounds reach every end
+            long_chains += npend;
+            k_jinit<<<Tm, kBlock, 0, s>>>(lv, mch_node, mch_k, jflag, rf, chunk_pre, J0);
+            BLAUNCH("build: chain jumps");
+            uint32_t* Jc = J0;
+            uint32_t* Jn = J1;
+            for (uint32_t span = 1; span < h[8]; span *= 2) {
+                k_jump<<<Tm, kBlock, 0, s>>>(lv, mch_node, mch_k, jflag, Jc, Jn);
+                BLAUNCH("build: chain jumps");
+                std::swap(Jc, Jn);
+                ++jump_rounds;
+            }
+            k_pending<<<blocks_for(npend, tpb), tpb, 0, s>>>(npend, pend_dst, pend_q, Jc, slots, level, geo);
+            BLAUNCH("build: long chains");
+        }
         if (h[4]) return fail(c, CT_ERR_INVALID_SCENE, "device build: scene extent overflows the SAH cost");
         const uint32_t nsplit = h[2];
         const uint32_t next0 = g0 + n;
@@ -990,7 +1146,6 @@ int device_build(ct_ctx* c, const ct_scene_view* v, DeviceTrees* out, ct_build_s
     if (h[4] & kErrCount) return fail(c, CT_ERR_DEVICE, "device build: partition count mismatch");
     const uint32_t n2 = 2 + 2 * h[8];
     const uint32_t n4 = h[8] ? h[9] : 1;
-    const uint32_t chain = h[5];
     void *p2 = nullptr, *p4 = nullptr;
     if ((r = dalloc(c, &p2, (size_t)n2 * sizeof(ct_bvh_node), "binary nodes"))) return r;
     out->nodes2 = static_cast<ct_bvh_node*>(p2);
@@ -1015,7 +1170,8 @@ int device_build(ct_ctx* c, const ct_scene_view* v, DeviceTrees* out, ct_build_s
         st->n_mbvh_nodes = n4;
         st->levels = n_levels;
         st->max_depth = n_levels + 1;  // the reference's m_MaxDepth: root depth 2
-        st->max_chain = chain;
+        st->long_chains = long_chains;
+        st->jump_rounds = jump_rounds;
         st->device_ms = ms;
         st->total_ms =
             (float)std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();

@@ -211,10 +211,10 @@ typedef struct ct_build_stats {
     uint32_t n_bvh_nodes, n_mbvh_nodes;
     uint32_t levels;      /* tree levels processed                         */
     uint32_t max_depth;   /* BVHTree::m_MaxDepth (root depth 2)            */
-    uint32_t max_chain;   /* longest partition swap chain if >= 16, else 0  */
+    uint32_t long_chains; /* partition slots whose swap chain exceeded 256 steps */
+    uint32_t jump_rounds; /* pointer-jumping rounds spent on those chains   */
     float device_ms;      /* first to last build kernel (read-backs included) */
     float total_ms;       /* whole call: checks, staging, build, uploads    */
-    uint32_t reserved;
 } ct_build_stats;
 int ct_scene_upload_build(ct_ctx* ctx, const ct_scene_view* scene, uint32_t flags, ct_build_stats* stats);
 /* Node counts of the current scene's trees (0 for a tree not present). */
