@@ -33,7 +33,7 @@ ACCEL_BVH2, ACCEL_BVH4 = 0, 1
 BSDF_REFERENCE, BSDF_TYPED = 0, 1
 TRAV_EXACT, TRAV_FAST = 0, 1
 LIGHTPDF_REFERENCE, LIGHTPDF_UNIFORM = 0, 1
-SCENE_C1, SCENE_C2, SCENE_C3 = 1, 2, 3
+SCENE_C1, SCENE_C2, SCENE_C3, SCENE_C2_TEXTURED = 1, 2, 3, 4
 CT_BUILD_WIDE = 1
 
 
@@ -60,6 +60,10 @@ class MBVHNode(C.Structure):
                 ("childs", C.c_int32 * 4), ("counts", C.c_int32 * 4)]
 
 
+class Texture(C.Structure):
+    _fields_ = [("width", C.c_uint32), ("height", C.c_uint32), ("rgba", C.POINTER(C.c_float))]
+
+
 class SceneView(C.Structure):
     _fields_ = [("n_vertices", C.c_uint32), ("vertices", C.POINTER(C.c_float)), ("normals", C.POINTER(C.c_float)),
                 ("texcoords", C.POINTER(C.c_float)), ("n_triangles", C.c_uint32),
@@ -69,7 +73,8 @@ class SceneView(C.Structure):
                 ("materials", C.POINTER(Material)), ("prim_indices", C.POINTER(C.c_uint32)),
                 ("n_bvh_nodes", C.c_uint32), ("bvh_nodes", C.POINTER(BVHNode)),
                 ("n_mbvh_nodes", C.c_uint32), ("mbvh_nodes", C.POINTER(MBVHNode)),
-                ("sky_enabled", C.c_int32), ("sky", C.c_float * 3)]
+                ("sky_enabled", C.c_int32), ("sky", C.c_float * 3),
+                ("n_textures", C.c_uint32), ("textures", C.POINTER(Texture)), ("sky_texture", C.c_int32)]
 
 
 class Camera(C.Structure):
@@ -198,6 +203,9 @@ _sig = {
     "ct_host_scene_last_error": (C.c_char_p, [_P]),
     "ct_host_scene_view": (C.c_int, [_P, C.POINTER(SceneView)]),
     "ct_host_scene_set_sky": (C.c_int, [_P, C.c_int32, C.POINTER(C.c_float)]),
+    "ct_host_scene_add_texture": (C.c_int, [_P, C.c_uint32, C.c_uint32, C.POINTER(C.c_float)]),
+    "ct_host_scene_load_texture": (C.c_int, [_P, C.c_char_p]),
+    "ct_host_scene_set_sky_texture": (C.c_int, [_P, C.c_int32]),
     "ct_host_scene_camera": (C.c_int, [_P, C.c_int32, C.c_int32, C.POINTER(Camera)]),
     "ct_host_scene_frame_opts": (C.c_int, [_P, C.POINTER(FrameOpts), C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
     "ct_host_scene_stats": (C.c_int, [_P, C.POINTER(C.c_uint64)]),
@@ -268,12 +276,14 @@ class HostScene:
             s.build()
         return s
 
-    def add_material(self, albedo, mtype=MAT_LAMBERTIAN, roughness=0.0, ior=1.0, keep_roughness=False) -> int:
+    def add_material(self, albedo, mtype=MAT_LAMBERTIAN, roughness=0.0, ior=1.0, keep_roughness=False,
+                     diffuse_tex: int = -1, normal_tex: int = -1) -> int:
         """TriangleList::addMaterial: roughness is forced to 1 unless keep_roughness."""
         m = Material()
         m.albedo[:] = list(map(float, albedo))
         m.refract_idx = ior
-        m.diffuse_tex = m.normal_tex = m.mask_tex = m.displace_tex = -1
+        m.diffuse_tex, m.normal_tex = diffuse_tex, normal_tex
+        m.mask_tex = m.displace_tex = -1
         m.type = mtype
         m.roughness = roughness
         r = lib.ct_host_scene_add_material(self._h, C.byref(m), 1 if keep_roughness else 0)
@@ -298,6 +308,27 @@ class HostScene:
         r = lib.ct_host_scene_load_model(self._h, str(path).encode(), scale, mp, material, 1 if normalize else 0)
         if r != CT_OK:
             raise CTError(r, lib.ct_host_scene_last_error(self._h).decode())
+
+    def add_texture(self, rgba) -> int:
+        """A texture from an (height, width, 4) float array, row 0 at the
+        bottom (TriangleList::loadTexture's texel order).  Returns its index."""
+        a = np.ascontiguousarray(rgba, np.float32)
+        if a.ndim != 3 or a.shape[2] != 4:
+            raise ValueError("texture must be (height, width, 4)")
+        r = lib.ct_host_scene_add_texture(self._h, a.shape[1], a.shape[0], _fp(a))
+        if r < 0:
+            raise CTError(-r, "add_texture")
+        return r
+
+    def load_texture(self, path) -> int:
+        """TriangleList::loadTexture for binary PPM (P6) / PFM files."""
+        r = lib.ct_host_scene_load_texture(self._h, str(path).encode())
+        if r < 0:
+            raise CTError(-r, lib.ct_host_scene_last_error(self._h).decode())
+        return r
+
+    def set_sky_texture(self, texture: int) -> None:
+        _check(lib.ct_host_scene_set_sky_texture(self._h, texture), None)
 
     def set_sky(self, enabled: bool, rgb=(0, 0, 0)) -> None:
         a = np.asarray(rgb, np.float32)
@@ -349,8 +380,13 @@ class HostScene:
             "material_idx": arr(v.material_idx, nt, np.uint32),
             "light_indices": arr(v.light_indices, v.n_lights, np.uint32),
             "prim_indices": arr(v.prim_indices, nt, np.uint32),
+            "texcoords": arr(v.texcoords, 2 * nv, np.float32).reshape(-1, 2),
             "sky_enabled": int(v.sky_enabled),
             "sky": np.array(list(v.sky), np.float32),
+            "textures": [arr(v.textures[t].rgba, 4 * v.textures[t].width * v.textures[t].height,
+                             np.float32).reshape(v.textures[t].height, v.textures[t].width, 4)
+                         for t in range(v.n_textures)],
+            "sky_texture": int(v.sky_texture),
         }
         out["materials"] = (Material * v.n_materials).from_buffer_copy(
             C.string_at(v.materials, C.sizeof(Material) * v.n_materials))

@@ -46,6 +46,19 @@ CT_DEV void tangent_frame(f3 N, f3& T, f3& B) {
         T = mk3(b, 1.0f - N.y * N.y * a, -N.y);
     }
 }
+// convertToLocalSpace + sampleToWorld (Random.cuh:37-53): the frame the
+// reference rotates normal-map samples with.
+CT_DEV f3 sample_to_world(f3 s, f3 N) {
+    const f3 W = ct_fabsf(N.x) > 0.99f ? mk3(0.0f, 1.0f, 0.0f) : mk3(1.0f, 0.0f, 0.0f);
+    const f3 T = normalize(cross(N, W));
+    const f3 B = cross(N, T);
+    return s.x * T + s.y * B + s.z * N;
+}
+// Nearest texel of texture t at (u, v) (SceneData::getTextureColor, SceneData.cuh:81-97).
+CT_DEV float4 tex_fetch(const float4* __restrict__ texels, const uint4* __restrict__ info, int t, float u, float v) {
+    const uint4 ti = info[t];
+    return texels[ti.x + ct_texel_index(ti.y, ti.z, u, v)];
+}
 CT_DEV f3 tangent_to_world(f3 V, f3 N) {
     f3 T, B;
     tangent_frame(N, T, B);

@@ -292,7 +292,14 @@ __global__ void __launch_bounds__(kShadeBlock) k_shade(DevScene sc, ShadeOpts so
             bool contrib = false;
             if (!(t < 1e34f)) {
                 if (sc.sky_enabled) {
-                    add = (thr / lastPdf) * mk3(sc.sky_r, sc.sky_g, sc.sky_b);
+                    f3 sky = mk3(sc.sky_r, sc.sky_g, sc.sky_b);
+                    if (sc.sky_tex >= 0) {  // Kernel.cu:96-100
+                        float u, v;
+                        ct_sky_uv(dir.x, dir.y, dir.z, &u, &v);
+                        const float4 c = tex_fetch(sc.texels, sc.tex_info, sc.sky_tex, u, v);
+                        sky = mk3(c.x, c.y, c.z);
+                    }
+                    add = (thr / lastPdf) * sky;
                     contrib = true;
                 }
             } else {
@@ -309,6 +316,19 @@ __global__ void __launch_bounds__(kShadeBlock) k_shade(DevScene sc, ShadeOpts so
                 f3 N = xyz(NC);
                 const f3 bary = bary_coords(hitp, N, p0, p1, p2);
                 f3 iN = bary.x * xyz(N0) + bary.y * xyz(N1) + bary.z * xyz(N2);
+                // textures: diffuse colour and normal map (Kernel.cu:106-112,146)
+                const uint32_t tpack = __float_as_uint(M1.w);
+                const int dtex = (int)(tpack & 0xFFFFu) - 1, ntex = (int)(tpack >> 16) - 1;
+                float tu = 0.0f, tv = 0.0f;
+                if (dtex >= 0 || ntex >= 0) {
+                    const float4 U0 = sc.sh_uv[2 * (size_t)prim], U1 = sc.sh_uv[2 * (size_t)prim + 1];
+                    tu = (bary.x * U0.x + bary.y * U0.z) + bary.z * U1.x;
+                    tv = (bary.x * U0.y + bary.y * U0.w) + bary.z * U1.y;
+                }
+                if (ntex >= 0) {
+                    const float4 c = tex_fetch(sc.texels, sc.tex_info, ntex, tu, tv);
+                    iN = sample_to_world(normalize(mk3(c.x, c.y, c.z)), iN);
+                }
                 if (mtype == MAT_LIGHT) {
                     if (so.indirect) {
                         const float DdotNL = -dot(dir, iN);
@@ -332,7 +352,11 @@ __global__ void __launch_bounds__(kShadeBlock) k_shade(DevScene sc, ShadeOpts so
                     uint32_t seed = ct_shade_seed(si.pixel, si.frame, ct_path_length_key(bounce));
                     const float r3 = ct_rand_float(&seed);
                     const float r4 = ct_rand_float(&seed);
-                    const f3 color = xyz(M0);
+                    f3 color = xyz(M0);
+                    if (dtex >= 0) {
+                        const float4 c = tex_fetch(sc.texels, sc.tex_info, dtex, tu, tv);
+                        color = mk3(c.x, c.y, c.z);
+                    }
                     const f3 wiw = -dir;
                     f3 wo = mk3(0, 0, 0), f = mk3(0, 0, 0);
                     float pdf = 0.0f;

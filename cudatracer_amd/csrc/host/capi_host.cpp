@@ -19,6 +19,7 @@ struct ct_host_scene {
     ct::SceneConfig cfg;
     bool generated = false;
     std::vector<ct_material> mats;
+    std::vector<ct_texture> textures;
     std::string error;
 };
 
@@ -146,6 +147,35 @@ int ct_host_scene_view(const ct_host_scene* cs, ct_scene_view* out) {
     out->sky[0] = s->cfg.sky.x;
     out->sky[1] = s->cfg.sky.y;
     out->sky[2] = s->cfg.sky.z;
+    s->textures.clear();
+    for (const auto& t : L.m_Textures) s->textures.push_back(ct_texture{t.width, t.height, t.rgba.data()});
+    out->n_textures = (uint32_t)s->textures.size();
+    out->textures = s->textures.data();
+    out->sky_texture = s->cfg.sky_texture;
+    return CT_OK;
+}
+
+int ct_host_scene_add_texture(ct_host_scene* s, uint32_t width, uint32_t height, const float* rgba) {
+    if (!s || !rgba || width == 0 || height == 0) return -CT_ERR_INVALID_ARG;
+    const int idx = s->list.addTexture(width, height, rgba);
+    return idx < 0 ? -CT_ERR_INVALID_ARG : idx;
+}
+
+int ct_host_scene_load_texture(ct_host_scene* s, const char* path) {
+    if (!s || !path) return -CT_ERR_INVALID_ARG;
+    std::string why;
+    const int idx = s->list.loadTexture(path, &why);
+    if (idx < 0) {
+        s->error = why;
+        return why.rfind("cannot open", 0) == 0 ? -CT_ERR_INVALID_ARG : -CT_ERR_INVALID_SCENE;
+    }
+    return idx;
+}
+
+int ct_host_scene_set_sky_texture(ct_host_scene* s, int32_t texture) {
+    if (!s || texture < -1 || texture >= (int32_t)s->list.m_Textures.size()) return CT_ERR_INVALID_ARG;
+    s->cfg.sky_texture = texture;
+    if (texture >= 0) s->cfg.sky_enabled = 1;  // loadSkybox enables the sky box (Application.cpp:85)
     return CT_OK;
 }
 

@@ -105,8 +105,11 @@ AABB triangle_bounds(vec3 p0, vec3 p1, vec3 p2) {
 }
 
 TriangleList::TriangleList() {
-    // TriangleList.cpp:8-13: "Have at least 1 material"
-    addMaterial(Material::lambertian(vec3(1.0f), 1));
+    // TriangleList.cpp:8-13: "Have at least 1 material".  The reference passes
+    // 1.0f as the second argument, the int diffuse texture index, so its
+    // material 0 samples texture 1 (out of range without two textures);
+    // here material 0 is the untextured white Lambert it was meant to be.
+    addMaterial(Material::lambertian(vec3(1.0f)));
 }
 
 void TriangleList::addTriangle(vec3 p0, vec3 p1, vec3 p2, vec3 n0, vec3 n1, vec3 n2, unsigned int matIdx, vec2 t0,

@@ -6,6 +6,8 @@
 // ct_scene_view of their arrays.
 #pragma once
 
+#include <cstdint>
+#include <map>
 #include <string>
 #include <vector>
 
@@ -56,6 +58,13 @@ struct Material {
     static Material from_abi(const ct_material& m);
 };
 
+// core::Surface's float texel buffer (src/Utils/Surface.cpp:33-100): RGBA,
+// row 0 at the bottom of the image (FreeImage scan-line order).
+struct Texture {
+    uint32_t width = 0, height = 0;
+    std::vector<float> rgba;
+};
+
 // Axis-aligned box in the padded form of triangle::getBounds
 // (Triangle.cuh:59-65): [min - 0.001, max + 0.01].
 struct AABB {
@@ -81,6 +90,13 @@ public:
     bool loadModel(const std::string& path, float scale = 1.0f, const float* mat = nullptr, int material = -1,
                    bool normalize = true, std::string* error = nullptr);
 
+    // TriangleList::loadTexture (TriangleList.cpp:377-391) for binary PPM (P6)
+    // and PFM files (host/texload.cpp); a path loaded before keeps its index.
+    // Returns the texture index, or -1 with a message in *error.
+    int loadTexture(const std::string& path, std::string* error = nullptr);
+    // The same for a texture already in memory (width * height RGBA floats).
+    int addTexture(uint32_t width, uint32_t height, const float* rgba);
+
     size_t getPrimCount() const { return m_Indices.size(); }
 
     std::vector<vec3> m_Vertices;
@@ -92,6 +108,8 @@ public:
     std::vector<Material> m_Materials;
     std::vector<AABB> m_AABBs;
     std::vector<uvec3> m_Indices;
+    std::vector<Texture> m_Textures;
+    std::map<std::string, int> m_LoadedTextures;
 };
 
 class Camera {

@@ -11,15 +11,18 @@ namespace {
 vec3 face_normal(vec3 a, vec3 b, vec3 c) { return normalize(cross(b - a, c - a)); }
 
 // Flat-shaded triangle whose normal is oriented along `towards`.
-void add_flat(TriangleList& L, vec3 a, vec3 b, vec3 c, vec3 towards, unsigned mat) {
+void add_flat(TriangleList& L, vec3 a, vec3 b, vec3 c, vec3 towards, unsigned mat, vec2 ta = vec2(),
+              vec2 tb = vec2(), vec2 tc = vec2()) {
     vec3 n = face_normal(a, b, c);
     if (dot(n, towards) < 0.0f) n = -n;
-    L.addTriangle(a, b, c, n, n, n, mat);
+    L.addTriangle(a, b, c, n, n, n, mat, ta, tb, tc);
 }
 
-void add_quad(TriangleList& L, vec3 p0, vec3 p1, vec3 p2, vec3 p3, vec3 normal, unsigned mat) {
-    add_flat(L, p0, p1, p2, normal, mat);
-    add_flat(L, p0, p2, p3, normal, mat);
+void add_quad(TriangleList& L, vec3 p0, vec3 p1, vec3 p2, vec3 p3, vec3 normal, unsigned mat,
+              const vec2* uv = nullptr) {
+    const vec2 z;
+    add_flat(L, p0, p1, p2, normal, mat, uv ? uv[0] : z, uv ? uv[1] : z, uv ? uv[2] : z);
+    add_flat(L, p0, p2, p3, normal, mat, uv ? uv[0] : z, uv ? uv[2] : z, uv ? uv[3] : z);
 }
 
 // Axis-aligned-in-local-frame box rotated by `yaw` about +y; outward normals.
@@ -39,7 +42,8 @@ void add_box(TriangleList& L, vec3 c, vec3 h, float yaw, unsigned mat) {
     }
 }
 
-// Open cylinder along +y: `seg` segments around, `rings` along the height.
+// Open cylinder along +y: `seg` segments around, `rings` along the height;
+// texture coordinates u = 2 * angle / 2pi, v = height fraction.
 void add_cylinder(TriangleList& L, vec3 base, float r, float height, int seg, int rings, unsigned mat) {
     const float twopi = 6.28318530717958647692f;
     for (int j = 0; j < rings; ++j) {
@@ -52,7 +56,10 @@ void add_cylinder(TriangleList& L, vec3 base, float r, float height, int seg, in
             const vec3 p2(base.x + r * std::cos(a1), y1, base.z + r * std::sin(a1));
             const vec3 p3(base.x + r * std::cos(a0), y1, base.z + r * std::sin(a0));
             const float am = 0.5f * (a0 + a1);
-            add_quad(L, p0, p1, p2, p3, vec3(std::cos(am), 0.0f, std::sin(am)), mat);
+            const float u0 = 2.0f * (float)i / (float)seg, u1 = 2.0f * (float)(i + 1) / (float)seg;
+            const float v0 = (float)j / (float)rings, v1 = (float)(j + 1) / (float)rings;
+            const vec2 uv[4] = {vec2(u0, v0), vec2(u1, v0), vec2(u1, v1), vec2(u0, v1)};
+            add_quad(L, p0, p1, p2, p3, vec3(std::cos(am), 0.0f, std::sin(am)), mat, uv);
         }
     }
 }
@@ -179,9 +186,9 @@ unsigned c2_material(TriangleList& L, SplitMix64& rng) {
 }
 
 // Tessellated plane at height y over [-ext, ext]^2 with n x n cells,
-// skipping cells inside |x|,|z| < hole.
+// skipping cells inside |x|,|z| < hole; texture coordinates (x, z) * uv_scale.
 void add_grid(TriangleList& L, float y, float ext, int n, float hole, vec3 normal, unsigned mat,
-              const std::function<float(float, float)>& height = nullptr) {
+              const std::function<float(float, float)>& height = nullptr, float uv_scale = 0.0f) {
     const float step = 2.0f * ext / (float)n;
     auto P = [&](int i, int j) {
         const float x = -ext + step * (float)i, z = -ext + step * (float)j;
@@ -191,13 +198,105 @@ void add_grid(TriangleList& L, float y, float ext, int n, float hole, vec3 norma
         for (int i = 0; i < n; ++i) {
             const float cx = -ext + step * ((float)i + 0.5f), cz = -ext + step * ((float)j + 0.5f);
             if (hole > 0.0f && std::fabs(cx) < hole && std::fabs(cz) < hole) continue;
-            add_quad(L, P(i, j), P(i + 1, j), P(i + 1, j + 1), P(i, j + 1), normal, mat);
+            const vec3 q[4] = {P(i, j), P(i + 1, j), P(i + 1, j + 1), P(i, j + 1)};
+            vec2 uv[4];
+            for (int k = 0; k < 4; ++k) uv[k] = vec2(q[k].x * uv_scale, q[k].z * uv_scale);
+            add_quad(L, q[0], q[1], q[2], q[3], normal, mat, uv);
         }
+}
+
+// Procedural textures of the textured atrium (RGBA, row 0 at the bottom).
+// Floor tiles: 8 x 8 tiles of two tones with dark grout and value-noise grain.
+int tiles_texture(TriangleList& L, uint64_t seed) {
+    const uint32_t n = 1024;
+    const ValueNoise vn{seed ^ 0x7431ull};
+    std::vector<float> t((size_t)n * n * 4);
+    for (uint32_t y = 0; y < n; ++y)
+        for (uint32_t x = 0; x < n; ++x) {
+            const bool grout = (x % 128) < 4 || (y % 128) < 4;
+            const bool dark = ((x / 128) + (y / 128)) & 1;
+            const float g = 0.06f * vn.eval((float)x / 24.0f, (float)y / 24.0f, 0.5f);
+            const vec3 c = grout ? vec3(0.12f) : dark ? vec3(0.32f, 0.30f, 0.28f) : vec3(0.72f, 0.68f, 0.62f);
+            float* p = &t[4 * ((size_t)x + (size_t)y * n)];
+            p[0] = c.x + g;
+            p[1] = c.y + g;
+            p[2] = c.z + g;
+            p[3] = 1.0f;
+        }
+    return L.addTexture(n, n, t.data());
+}
+
+// Tangent-space normals of a bump field h = 0.5 sin(2pi x/64) sin(2pi y/64),
+// stored as floats (the reference normalises the raw texel colour).
+int bumps_texture(TriangleList& L) {
+    const uint32_t n = 512;
+    const float k = 6.28318530717958647692f / 64.0f;
+    std::vector<float> t((size_t)n * n * 4);
+    for (uint32_t y = 0; y < n; ++y)
+        for (uint32_t x = 0; x < n; ++x) {
+            const float dx = 0.5f * k * std::cos(k * (float)x) * std::sin(k * (float)y);
+            const float dy = 0.5f * k * std::sin(k * (float)x) * std::cos(k * (float)y);
+            const vec3 nn = normalize(vec3(-4.0f * dx, -4.0f * dy, 1.0f));
+            float* p = &t[4 * ((size_t)x + (size_t)y * n)];
+            p[0] = nn.x;
+            p[1] = nn.y;
+            p[2] = nn.z;
+            p[3] = 1.0f;
+        }
+    return L.addTexture(n, n, t.data());
+}
+
+// Column stone: banded fbm noise.
+int marble_texture(TriangleList& L, uint64_t seed) {
+    const uint32_t n = 512;
+    const ValueNoise vn{seed ^ 0x3a7bull};
+    std::vector<float> t((size_t)n * n * 4);
+    for (uint32_t y = 0; y < n; ++y)
+        for (uint32_t x = 0; x < n; ++x) {
+            const float f = vn.fbm(vec3((float)x / 64.0f, (float)y / 64.0f, 0.25f), 4);
+            const float b = 0.5f + 0.5f * std::sin(0.05f * (float)y + 6.0f * f);
+            float* p = &t[4 * ((size_t)x + (size_t)y * n)];
+            p[0] = 0.55f + 0.35f * b;
+            p[1] = 0.52f + 0.33f * b;
+            p[2] = 0.50f + 0.30f * b;
+            p[3] = 1.0f;
+        }
+    return L.addTexture(n, n, t.data());
+}
+
+// Equirectangular sky (u = azimuth, v = 1 - polar angle / pi): zenith-to-
+// horizon gradient, dim ground, and a small bright sun.
+int sky_texture(TriangleList& L) {
+    const uint32_t w = 1024, h = 512;
+    const float pi = 3.14159265358979323846f;
+    std::vector<float> t((size_t)w * h * 4);
+    for (uint32_t y = 0; y < h; ++y)
+        for (uint32_t x = 0; x < w; ++x) {
+            const float v = ((float)y + 0.5f) / (float)h, u = ((float)x + 0.5f) / (float)w;
+            const float elev = (v - 0.5f) * pi;  // -pi/2 (nadir) .. pi/2 (zenith)
+            vec3 c;
+            if (elev < 0.0f) {
+                c = vec3(0.22f, 0.2f, 0.18f);
+            } else {
+                const float s = std::sin(elev);
+                c = vec3(0.75f, 0.8f, 0.88f) * (1.0f - s) + vec3(0.22f, 0.36f, 0.66f) * s;
+            }
+            const float du = std::fabs(u - 0.3f), dv = std::fabs(v - 0.8f);
+            if (du * du * 4.0f + dv * dv < 0.0004f) c = vec3(30.0f, 28.0f, 24.0f);
+            float* p = &t[4 * ((size_t)x + (size_t)y * w)];
+            p[0] = c.x;
+            p[1] = c.y;
+            p[2] = c.z;
+            p[3] = 1.0f;
+        }
+    return L.addTexture(w, h, t.data());
 }
 
 // C2: atrium of ~260k triangles: 2 tessellated floors, 24 columns of 2,048
 // triangles, 400 random boxes (log-uniform size), 4 quad lights + constant sky.
-void gen_c2(uint64_t seed, TriangleList& L, SceneConfig& cfg) {
+// textured: the same geometry with tiled, bump-mapped floors, stone columns
+// (the Lambert and GGX ones) and a sky texture instead of the constant sky.
+void gen_c2(uint64_t seed, TriangleList& L, SceneConfig& cfg, bool textured = false) {
     SplitMix64 rng(seed);
     // lights first (light ordinal == triangle id)
     const unsigned lightMat = L.addMaterial(Material::light(vec3(40.0f, 38.0f, 34.0f)));
@@ -206,15 +305,29 @@ void gen_c2(uint64_t seed, TriangleList& L, SceneConfig& cfg) {
     for (auto& c : lxz)
         add_quad(L, vec3(c[0] - lh, ly, c[1] - lh), vec3(c[0] + lh, ly, c[1] - lh), vec3(c[0] + lh, ly, c[1] + lh),
                  vec3(c[0] - lh, ly, c[1] + lh), vec3(0, -1, 0), lightMat);
-    const unsigned floorMat = L.addMaterial(Material::lambertian(vec3(0.6f, 0.58f, 0.55f)), true);
-    const unsigned upperMat = L.addMaterial(Material::lambertian(vec3(0.5f, 0.52f, 0.6f)), true);
-    add_grid(L, 0.0f, 100.0f, 236, 0.0f, vec3(0, 1, 0), floorMat);
-    add_grid(L, 30.0f, 100.0f, 236, 40.0f, vec3(0, -1, 0), upperMat);
+    int tiles = -1, bumps = -1, marble = -1;
+    if (textured) {
+        tiles = tiles_texture(L, seed);
+        bumps = bumps_texture(L);
+        marble = marble_texture(L, seed);
+        cfg.sky_texture = sky_texture(L);
+    }
+    const unsigned floorMat = L.addMaterial(
+        textured ? Material::lambertian(vec3(1.0f), tiles, bumps) : Material::lambertian(vec3(0.6f, 0.58f, 0.55f)),
+        true);
+    const unsigned upperMat = L.addMaterial(
+        textured ? Material::lambertian(vec3(1.0f), tiles, bumps) : Material::lambertian(vec3(0.5f, 0.52f, 0.6f)),
+        true);
+    add_grid(L, 0.0f, 100.0f, 236, 0.0f, vec3(0, 1, 0), floorMat, nullptr, 0.125f);
+    add_grid(L, 30.0f, 100.0f, 236, 40.0f, vec3(0, -1, 0), upperMat, nullptr, 0.125f);
     // 24 columns: 64 segments x 16 rings x 2 = 2048 triangles each
     for (int iz = 0; iz < 4; ++iz)
         for (int ix = 0; ix < 6; ++ix) {
             const float x = -75.0f + 30.0f * (float)ix, z = -67.5f + 45.0f * (float)iz;
-            add_cylinder(L, vec3(x, 0.0f, z), 2.0f, 30.0f, 64, 16, c2_material(L, rng));
+            const unsigned m = c2_material(L, rng);
+            if (textured && (L.m_Materials[m].type == Lambertian || L.m_Materials[m].type == GGX))
+                L.m_Materials[m].diffuseTex = marble;
+            add_cylinder(L, vec3(x, 0.0f, z), 2.0f, 30.0f, 64, 16, m);
         }
     // 400 boxes, edge lengths log-uniform in [0.1, 2] * 5 world units
     for (int b = 0; b < 400; ++b) {
@@ -294,6 +407,7 @@ bool generate_scene(int config, uint64_t seed, TriangleList& list, SceneConfig& 
     switch (config) {
         case CT_SCENE_C1_BOX_SPHERE: gen_c1(seed, list, cfg); return true;
         case CT_SCENE_C2_ATRIUM: gen_c2(seed, list, cfg); return true;
+        case CT_SCENE_C2_TEXTURED: gen_c2(seed, list, cfg, true); return true;
         case CT_SCENE_C3_DISPLACED: gen_c3(seed, list, cfg); return true;
         default: return false;
     }

@@ -18,6 +18,7 @@ struct SceneConfig {
     float cam_fov = 70.0f;
     int sky_enabled = 0;
     vec3 sky;
+    int sky_texture = -1;  // SceneData::skyboxTexture
     ct_frame_opts opts;   // the configuration's depth / spp / accel / epsilons
     int width = 256, height = 256;
 };

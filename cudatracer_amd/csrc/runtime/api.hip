@@ -140,9 +140,17 @@ int upload_vec(ct_ctx* c, const std::vector<T>& h, const float4** dst, const cha
     return CT_OK;
 }
 
+// Row 1 .w: texture indices + 1 (0 = none), diffuse in bits 0-15, normal in 16-31.
 void mat_rows(const ct_material& m, float4* rows) {
+    const uint32_t tex = ((uint32_t)(m.diffuse_tex + 1) & 0xFFFFu) | (((uint32_t)(m.normal_tex + 1) & 0xFFFFu) << 16);
     rows[0] = make_float4(m.albedo[0], m.albedo[1], m.albedo[2], m.roughness);
-    rows[1] = make_float4(ubits((uint32_t)m.type), m.refract_idx, std::max(m.roughness, 1e-3f), 0.0f);
+    rows[1] = make_float4(ubits((uint32_t)m.type), m.refract_idx, std::max(m.roughness, 1e-3f), ubits(tex));
+}
+
+// Texture indices of a material: -1 or an existing texture.
+bool mat_textures_ok(const ct_material& m, uint32_t n_tex) {
+    return m.diffuse_tex >= -1 && m.normal_tex >= -1 && (m.diffuse_tex < 0 || (uint32_t)m.diffuse_tex < n_tex) &&
+           (m.normal_tex < 0 || (uint32_t)m.normal_tex < n_tex);
 }
 
 void light_rows(const ct_scene_view* v, uint32_t tri, const ct_material* mats, float4* rows) {
@@ -172,12 +180,28 @@ int check_geometry(ct_ctx* c, const ct_scene_view* v) {
         if (v->material_idx[i] >= v->n_materials) return fail(c, CT_ERR_INVALID_SCENE, "material index out of range");
     for (uint32_t i = 0; i < v->n_lights; ++i)
         if (v->light_indices[i] >= N) return fail(c, CT_ERR_INVALID_SCENE, "light index out of range");
+    if (v->n_textures > 0xFFFEu) return fail(c, CT_ERR_INVALID_SCENE, "too many textures");
+    if (v->n_textures && !v->textures) return fail(c, CT_ERR_INVALID_ARG, "scene view is missing an array");
+    uint64_t texels = 0;
+    for (uint32_t t = 0; t < v->n_textures; ++t) {
+        const ct_texture& x = v->textures[t];
+        if (x.width == 0 || x.height == 0 || !x.rgba) return fail(c, CT_ERR_INVALID_SCENE, "empty texture");
+        texels += (uint64_t)x.width * x.height;
+    }
+    if (texels >= (1ull << 32)) return fail(c, CT_ERR_INVALID_SCENE, "textures too large");
+    for (uint32_t m = 0; m < v->n_materials; ++m)
+        if (!mat_textures_ok(v->materials[m], v->n_textures))
+            return fail(c, CT_ERR_INVALID_SCENE, "material texture index out of range");
+    if (v->n_textures && (v->sky_texture < -1 || (v->sky_texture >= 0 && (uint32_t)v->sky_texture >= v->n_textures)))
+        return fail(c, CT_ERR_INVALID_SCENE, "sky texture index out of range");
     return CT_OK;
 }
 
 // Host staging of the tree-independent device arrays.
 struct Shading {
     std::vector<float4> pos, nrm, mats, lights;
+    std::vector<float4> texels, uv;  // uv only when the scene has textures
+    std::vector<uint4> tex_info;
 };
 
 void stage_shading(const ct_scene_view* v, Shading& sh) {
@@ -196,6 +220,28 @@ void stage_shading(const ct_scene_view* v, Shading& sh) {
     for (uint32_t m = 0; m < v->n_materials; ++m) mat_rows(v->materials[m], &sh.mats[2 * (size_t)m]);
     sh.lights.resize((size_t)std::max<uint32_t>(v->n_lights, 1u) * 5);
     for (uint32_t l = 0; l < v->n_lights; ++l) light_rows(v, v->light_indices[l], v->materials, &sh.lights[5 * (size_t)l]);
+    // TriangleList::createTextureBuffer (TriangleList.cpp:148-178)
+    for (uint32_t t = 0; t < v->n_textures; ++t) {
+        const ct_texture& x = v->textures[t];
+        sh.tex_info.push_back(make_uint4((uint32_t)sh.texels.size(), x.width, x.height, 0u));
+        const size_t n = (size_t)x.width * x.height;
+        for (size_t i = 0; i < n; ++i)
+            sh.texels.push_back(make_float4(x.rgba[4 * i], x.rgba[4 * i + 1], x.rgba[4 * i + 2], x.rgba[4 * i + 3]));
+    }
+    if (v->n_textures) {
+        sh.uv.resize((size_t)N * 2);
+        for (uint32_t prim = 0; prim < N; ++prim) {
+            float t[6] = {0, 0, 0, 0, 0, 0};
+            if (v->texcoords)
+                for (int j = 0; j < 3; ++j) {
+                    const uint32_t vi = v->indices[3 * (size_t)prim + j];
+                    t[2 * j] = v->texcoords[2 * (size_t)vi];
+                    t[2 * j + 1] = v->texcoords[2 * (size_t)vi + 1];
+                }
+            sh.uv[2 * (size_t)prim] = make_float4(t[0], t[1], t[2], t[3]);
+            sh.uv[2 * (size_t)prim + 1] = make_float4(t[4], t[5], 0.0f, 0.0f);
+        }
+    }
 }
 
 int upload_shading(ct_ctx* c, const Shading& sh, ctd::DevScene& sc) {
@@ -203,6 +249,13 @@ int upload_shading(ct_ctx* c, const Shading& sh, ctd::DevScene& sc) {
     if (r == CT_OK) r = upload_vec(c, sh.nrm, &sc.sh_nrm, "shading normals");
     if (r == CT_OK) r = upload_vec(c, sh.mats, &sc.mats, "materials");
     if (r == CT_OK) r = upload_vec(c, sh.lights, &sc.lights, "lights");
+    if (r == CT_OK && !sh.texels.empty()) {
+        r = upload_vec(c, sh.texels, &sc.texels, "textures");
+        const float4* ti = nullptr;
+        if (r == CT_OK) r = upload_vec(c, sh.tex_info, &ti, "texture table");
+        sc.tex_info = reinterpret_cast<const uint4*>(ti);
+        if (r == CT_OK) r = upload_vec(c, sh.uv, &sc.sh_uv, "texture coordinates");
+    }
     return r;
 }
 
@@ -213,6 +266,8 @@ void install_scene(ct_ctx* c, const ct_scene_view* v, ctd::DevScene sc, const Sh
     sc.n_mats = v->n_materials;
     sc.n_lights = v->n_lights;
     sc.sky_enabled = v->sky_enabled ? 1 : 0;
+    sc.n_tex = v->n_textures;
+    sc.sky_tex = v->n_textures ? v->sky_texture : -1;
     sc.sky_r = v->sky[0];
     sc.sky_g = v->sky[1];
     sc.sky_b = v->sky[2];
@@ -392,6 +447,7 @@ int ct_material_update(ct_ctx* c, uint32_t index, const ct_material* mat) {
     if (!c || !mat) return CT_ERR_INVALID_ARG;
     if (!c->has_scene) return fail(c, CT_ERR_NO_SCENE, "no scene uploaded");
     if (index >= c->h_mats.size()) return fail(c, CT_ERR_INVALID_ARG, "material index out of range");
+    if (!mat_textures_ok(*mat, c->sc.n_tex)) return fail(c, CT_ERR_INVALID_ARG, "material texture index out of range");
     (void)hipSetDevice(c->device);
     c->h_mats[index] = *mat;
     float4 rows[2];

@@ -271,6 +271,72 @@ CT_HD float ct_acosf(float x) {
     return (2.0f * CT_PIO2_HI - 2.0f * ct_asin_poly(t)) + 2.0f * CT_PIO2_MID;
 }
 
+/* --------------------------------------------------------------- atan2 --- */
+/* atan(t) for |t| <= 1/2: Taylor series t - t^3/3 + ... to t^27
+ * (truncation below 2e-9 relative). */
+CT_HD float ct_atan_poly(float t) {
+    const float z = t * t;
+    float p = -0x1.2f684cp-5f;                 /* -1/27 */
+    p = ct_fmaf(p, z,  0x1.47ae14p-5f);        /*  1/25 */
+    p = ct_fmaf(p, z, -0x1.642c86p-5f);        /* -1/23 */
+    p = ct_fmaf(p, z,  0x1.861862p-5f);        /*  1/21 */
+    p = ct_fmaf(p, z, -0x1.af286cp-5f);        /* -1/19 */
+    p = ct_fmaf(p, z,  0x1.e1e1e2p-5f);        /*  1/17 */
+    p = ct_fmaf(p, z, -0x1.111112p-4f);        /* -1/15 */
+    p = ct_fmaf(p, z,  0x1.3b13b2p-4f);        /*  1/13 */
+    p = ct_fmaf(p, z, -0x1.745d18p-4f);        /* -1/11 */
+    p = ct_fmaf(p, z,  0x1.c71c72p-4f);        /*  1/9  */
+    p = ct_fmaf(p, z, -0x1.24924ap-3f);        /* -1/7  */
+    p = ct_fmaf(p, z,  0x1.99999ap-3f);        /*  1/5  */
+    p = ct_fmaf(p, z, -0x1.555556p-2f);        /* -1/3  */
+    return ct_fmaf(p * z, t, t);
+}
+/* atan(a) for a in [0, 1]: above 1/2, atan(a) = pi/4 + atan((a-1)/(a+1)),
+ * where a - 1 is exact (Sterbenz) and |(a-1)/(a+1)| <= 1/3. */
+CT_HD float ct_atan01(float a) {
+    if (a <= 0.5f) return ct_atan_poly(a);
+    const float t = (a - 1.0f) / (a + 1.0f);
+    return (0x1.921fb6p-1f + ct_atan_poly(t)) + -0x1.777a5cp-26f;
+}
+/* atan2(y, x) for finite arguments: octant reduction to [0, 1], then the
+ * quadrant from the signs (atan2(+-0, -0) = +-pi, atan2(+-0, +0) = +-0). */
+CT_HD float ct_atan2f(float y, float x) {
+    const float ax = ct_fabsf(x), ay = ct_fabsf(y);
+    float r;
+    if (ay == 0.0f && ax == 0.0f) r = 0.0f;
+    else if (ay <= ax) r = ct_atan01(ay / ax);
+    else r = (CT_PIO2_HI - ct_atan01(ax / ay)) + CT_PIO2_MID;
+    if (ct_f2u(x) >> 31) r = (0x1.921fb6p+1f - r) + -0x1.777a5cp-24f;
+    return (ct_f2u(y) >> 31) ? -r : r;
+}
+
+/* ------------------------------------------------------------ textures --- */
+/* Nearest-texel address of SceneData::getTextureColor (SceneData.cuh:81-97):
+ * wrap each coordinate into [0, 1) by fmod (x - trunc(x): exact, and the
+ * sign of a zero result does not matter below), add 1 if negative, then
+ * uint(x * (width - 1)).  Returns the texel index inside the texture.  The
+ * normal-map lookup (SceneData.cuh:99-114) skips the fmod, which reads
+ * outside the texture for coordinates above 1; it shares this wrap here. */
+CT_HD float ct_wrap01(float x) {
+    const float a = ct_fabsf(x);
+    float f;
+    if (a < 8388608.0f) f = x - (float)(int32_t)x;  /* |x| < 2^23: trunc via int */
+    else f = 0.0f;                                  /* already an integer */
+    if (f < 0.0f) f += 1.0f;
+    return f;
+}
+CT_HD uint32_t ct_texel_index(uint32_t width, uint32_t height, float u, float v) {
+    const float x = ct_wrap01(u), y = ct_wrap01(v);
+    const uint32_t ix = (uint32_t)(x * (float)(width - 1));
+    const uint32_t iy = (uint32_t)(y * (float)(height - 1));
+    return ix + iy * width;
+}
+/* Sky texture coordinates of a miss direction (Kernel.cu:96-100). */
+CT_HD void ct_sky_uv(float dx, float dy, float dz, float* u, float* v) {
+    *u = 1.0f + ct_atan2f(dx, -dz) * CT_INV_PI_F * 0.5f;
+    *v = 1.0f - ct_acosf(dy) * CT_INV_PI_F;
+}
+
 /* ------------------------------------------------------- erf / erfinv --- */
 /* Abramowitz & Stegun 7.1.26, in the evaluation order of Microfacet.cuh:69-82. */
 CT_HD float ct_erff(float x) {

@@ -27,7 +27,7 @@
 extern "C" {
 #endif
 
-#define CT_ABI_VERSION 3
+#define CT_ABI_VERSION 4
 
 /* ---------------------------------------------------------------- status */
 enum {
@@ -61,7 +61,11 @@ enum {
 };
 
 /* Field order of struct Material, src/Core/Material.cuh:29-41
- * (albedo doubles as emission for lights). */
+ * (albedo doubles as emission for lights).  diffuse_tex >= 0 replaces the
+ * albedo by the texture colour at the hit's texture coordinates
+ * (Kernel.cu:146); normal_tex >= 0 perturbs the shading normal
+ * (Kernel.cu:109-112).  mask_tex and displace_tex are carried but unused, as
+ * in the reference's shading. */
 typedef struct ct_material {
     float albedo[3];
     float refract_idx;
@@ -92,6 +96,14 @@ typedef struct ct_mbvh_node {
 } ct_mbvh_node;
 
 /* -------------------------------------------------------------- scene */
+/* A texture: core::Surface's float buffer (src/Utils/Surface.cpp:33-100),
+ * width * height RGBA texels, row 0 at the bottom of the image (FreeImage's
+ * scan-line order), texel (x, y) at rgba[4 * (x + y * width)]. */
+typedef struct ct_texture {
+    uint32_t width, height;
+    const float* rgba;
+} ct_texture;
+
 /* Borrowed host arrays in the layout of TriangleList (src/Core/TriangleList.h:59-73)
  * plus the two trees (BVHTree::m_BVHPool / m_PrimitiveIndices and
  * MBVHTree::m_Tree[0..m_FinalPtr)).  The library copies everything it needs
@@ -116,6 +128,12 @@ typedef struct ct_scene_view {
     const ct_mbvh_node* mbvh_nodes;
     int32_t sky_enabled;          /* SceneData::skyboxEnabled               */
     float sky[3];                 /* constant environment radiance (a 1x1 sky texture) */
+    /* TriangleList::createTextureBuffer (TriangleList.cpp:148-178) */
+    uint32_t n_textures;
+    const ct_texture* textures;
+    int32_t sky_texture;          /* SceneData::skyboxTexture: -1 = the constant sky;
+                                     ignored without textures (a zeroed view keeps
+                                     the constant sky)                            */
 } ct_scene_view;
 
 /* Camera state, src/Core/Camera.h:81-92; plane_distance = tan(fov/2). */
@@ -342,7 +360,9 @@ typedef struct ct_host_scene ct_host_scene;
 enum {
     CT_SCENE_C1_BOX_SPHERE = 1,   /* closed box + 20,480-tri icosphere + quad light */
     CT_SCENE_C2_ATRIUM = 2,       /* ~260k-tri procedural atrium                     */
-    CT_SCENE_C3_DISPLACED = 3     /* 5.2M displaced icosphere + 2M noisy ground      */
+    CT_SCENE_C3_DISPLACED = 3,    /* 5.2M displaced icosphere + 2M noisy ground      */
+    CT_SCENE_C2_TEXTURED = 4      /* config 2 with diffuse and normal textures and an
+                                     equirectangular sky texture                     */
 };
 
 int ct_host_scene_create(ct_host_scene** out);
@@ -372,6 +392,20 @@ const char* ct_host_scene_last_error(const ct_host_scene* s);
 int ct_host_scene_build(ct_host_scene* s);
 int ct_host_scene_view(const ct_host_scene* s, ct_scene_view* out);
 int ct_host_scene_set_sky(ct_host_scene* s, int32_t enabled, const float rgb[3]);
+/* TriangleList::loadTexture (TriangleList.cpp:377-391) from memory: a copy of
+ * width * height RGBA floats, row 0 at the bottom.  Returns the texture index
+ * (>= 0) or -status. */
+int ct_host_scene_add_texture(ct_host_scene* s, uint32_t width, uint32_t height, const float* rgba);
+/* TriangleList::loadTexture from a file: binary PPM (P6, maxval <= 255) as
+ * 8-bit colour / 255 with alpha 1 (FreeImage's 32-bit conversion), PFM (PF colour, Pf grey) as floats with
+ * alpha 1; rows stored bottom-up as the reference's FreeImage load does.  A
+ * path loaded before returns its earlier index.  Returns the index or
+ * -status (ct_host_scene_last_error gives the reason).  FreeImage's other
+ * formats (PNG, JPEG, Radiance HDR) are not decoded. */
+int ct_host_scene_load_texture(ct_host_scene* s, const char* path);
+/* Application::loadSkybox (Application.cpp:79-89): texture index (-1 = none;
+ * the constant sky colour of ct_host_scene_set_sky applies). */
+int ct_host_scene_set_sky_texture(ct_host_scene* s, int32_t texture);
 /* The generator's camera for a config scene (Camera ctor, Camera.cpp:9-17). */
 int ct_host_scene_camera(const ct_host_scene* s, int32_t width, int32_t height, ct_camera* cam);
 /* Make a camera from position/look-at/fov (degrees). */

@@ -148,8 +148,16 @@ struct Box {
 
 }  // namespace
 
+struct or_texture {
+    uint32_t w = 0, h = 0;
+    std::vector<float> rgba;  // row 0 at the bottom (Surface.cpp:33-100 via FreeImage)
+};
+
 struct or_scene {
     std::vector<float> vertices, normals, center_normals;
+    std::vector<float> texcoords;  // 2 per vertex (TriangleList::m_TexCoords)
+    std::vector<or_texture> textures;
+    int sky_tex = -1;              // SceneData::skyboxTexture
     std::vector<uint32_t> indices, mat_idx, light_idx;
     std::vector<ct_material> materials;
     int sky_enabled = 0;
@@ -163,6 +171,7 @@ struct or_scene {
     uint32_t ntris() const { return (uint32_t)indices.size() / 3; }
     const float* vtx(uint32_t tri, int k) const { return &vertices[3 * (size_t)indices[3 * (size_t)tri + k]]; }
     const float* nrm(uint32_t tri, int k) const { return &normals[3 * (size_t)indices[3 * (size_t)tri + k]]; }
+    const float* uv(uint32_t tri, int k) const { return &texcoords[2 * (size_t)indices[3 * (size_t)tri + k]]; }
 };
 
 namespace {
@@ -489,6 +498,31 @@ bool trace(const or_scene& s, int accel, V3<T> o, V3<T> d, T* t, int* hit, T eps
     return traverse_brute<T, AnyHit>(s, o, d, t, hit, eps, c);
 }
 
+// ------------------------------------------------------------ textures --
+// SceneData::getTextureColor (SceneData.cuh:81-97): nearest texel after
+// wrapping each coordinate into [0, 1) (ct_texel_index, shared with the
+// kernels).  getTextureNormal (SceneData.cuh:99-114) skips the fmod and
+// reads outside the texture above 1; the decision here wraps it the same way.
+inline const float* texel(const or_scene& s, int t, float u, float v) {
+    const or_texture& x = s.textures[(size_t)t];
+    return &x.rgba[4 * (size_t)ct_texel_index(x.w, x.h, u, v)];
+}
+// convertToLocalSpace + sampleToWorld (Random.cuh:37-53)
+template <class T> V3<T> sample_to_world(V3<T> smp, V3<T> N) {
+    const V3<T> W = m_abs(N.x) > T(0.99f) ? V3<T>{T(0), T(1), T(0)} : V3<T>{T(1), T(0), T(0)};
+    const V3<T> Tv = normalize(cross(N, W));
+    const V3<T> Bv = cross(N, Tv);
+    return smp.x * Tv + smp.y * Bv + smp.z * N;
+}
+// Sky texture coordinates (Kernel.cu:96-100): the shared float functions, or
+// double libm for the FP64 render.
+inline void sky_uv(float dx, float dy, float dz, float* u, float* v) { ct_sky_uv(dx, dy, dz, u, v); }
+inline void sky_uv(double dx, double dy, double dz, float* u, float* v) {
+    const double pi = 3.14159265358979323846;
+    *u = (float)(1.0 + std::atan2(dx, -dz) / pi * 0.5);
+    *v = (float)(1.0 - std::acos(dy < -1.0 ? -1.0 : dy > 1.0 ? 1.0 : dy) / pi);
+}
+
 // triangle helpers, Triangle.cuh:46-96
 template <class T> V3<T> bary_coords(V3<T> p, V3<T> n, V3<T> p0, V3<T> p1, V3<T> p2) {
     const T abc = dot(n, cross(p1 - p0, p2 - p0));
@@ -800,7 +834,12 @@ V3<T> path_sample(const or_scene& s, const Camera3<T>& cb, const ct_frame_opts& 
         st.tcl += cc.tris;
         if (!(t < T(1e34f))) {  // miss (Ray::valid, Ray.cuh:38-41)
             if (s.sky_enabled) {
-                const V3<T> sky{T(s.sky[0]), T(s.sky[1]), T(s.sky[2])};
+                V3<T> sky{T(s.sky[0]), T(s.sky[1]), T(s.sky[2])};
+                if (s.sky_tex >= 0) {  // Kernel.cu:96-100
+                    float u, v;
+                    sky_uv(dir.x, dir.y, dir.z, &u, &v);
+                    sky = ld<T>(texel(s, s.sky_tex, u, v));
+                }
                 L = L + (thr / lastPdf) * sky;
             }
             break;
@@ -811,6 +850,16 @@ V3<T> path_sample(const or_scene& s, const Camera3<T>& cb, const ct_frame_opts& 
         V3<T> N = ld<T>(&s.center_normals[3 * (size_t)prim]);
         const V3<T> bary = bary_coords(hitp, N, p0, p1, p2);
         V3<T> iN = bary.x * ld<T>(s.nrm(prim, 0)) + bary.y * ld<T>(s.nrm(prim, 1)) + bary.z * ld<T>(s.nrm(prim, 2));
+        // textures (Kernel.cu:106-112,146): coordinates at the hit point (Q8)
+        float tu = 0.0f, tv = 0.0f;
+        if (mat.diffuse_tex >= 0 || mat.normal_tex >= 0) {
+            const float* t0 = s.uv(prim, 0);
+            const float* t1 = s.uv(prim, 1);
+            const float* t2 = s.uv(prim, 2);
+            tu = (float)((bary.x * T(t0[0]) + bary.y * T(t1[0])) + bary.z * T(t2[0]));
+            tv = (float)((bary.x * T(t0[1]) + bary.y * T(t1[1])) + bary.z * T(t2[1]));
+        }
+        if (mat.normal_tex >= 0) iN = sample_to_world(normalize(ld<T>(texel(s, mat.normal_tex, tu, tv))), iN);
         if (mat.type == CT_MAT_LIGHT) {
             if (!o.indirect) break;
             const T DdotNL = -dot(dir, iN);
@@ -832,7 +881,7 @@ V3<T> path_sample(const or_scene& s, const Camera3<T>& cb, const ct_frame_opts& 
         seed = ct_shade_seed(pixel, frame, ct_path_length_key(bounce));
         const T r3 = T(ct_rand_float(&seed));
         const T r4 = T(ct_rand_float(&seed));
-        const V3<T> color = ld<T>(mat.albedo);
+        const V3<T> color = mat.diffuse_tex >= 0 ? ld<T>(texel(s, mat.diffuse_tex, tu, tv)) : ld<T>(mat.albedo);
         const V3<T> wiw = -dir;  // direction towards the previous vertex
         // --- BSDF sample (BSDF.h:33-63, extended for the typed materials)
         V3<T> wo{T(0), T(0), T(0)}, f{T(0), T(0), T(0)};
@@ -1035,6 +1084,35 @@ or_scene* or_scene_create(uint32_t n_vertices, const float* vertices, const floa
 }
 
 void or_scene_destroy(or_scene* s) { delete s; }
+
+int or_scene_set_textures(or_scene* s, const float* texcoords, uint32_t n_textures, const ct_texture* textures,
+                          int32_t sky_texture) {
+    if (!s) return 1;
+    const size_t nv = s->vertices.size() / 3;
+    if (texcoords) s->texcoords.assign(texcoords, texcoords + 2 * nv);
+    else s->texcoords.assign(2 * nv, 0.0f);
+    s->textures.clear();
+    for (uint32_t t = 0; t < n_textures; ++t) {
+        or_texture x;
+        x.w = textures[t].width;
+        x.h = textures[t].height;
+        x.rgba.assign(textures[t].rgba, textures[t].rgba + 4 * (size_t)x.w * x.h);
+        s->textures.push_back(std::move(x));
+    }
+    s->sky_tex = n_textures ? sky_texture : -1;
+    for (const ct_material& m : s->materials)
+        if (m.diffuse_tex >= (int)n_textures || m.normal_tex >= (int)n_textures) return 1;
+    return (s->sky_tex >= (int)n_textures) ? 1 : 0;
+}
+
+int or_texel(const or_scene* s, int32_t texture, float u, float v, float* rgba) {
+    if (!s || texture < 0 || (size_t)texture >= s->textures.size() || !rgba) return 1;
+    std::memcpy(rgba, texel(*s, texture, u, v), 16);
+    return 0;
+}
+
+float or_atan2f(float y, float x) { return ct_atan2f(y, x); }
+void or_sky_uv(float dx, float dy, float dz, float* u, float* v) { ct_sky_uv(dx, dy, dz, u, v); }
 
 int or_scene_build(or_scene* s) {
     if (!s || s->ntris() == 0) return 1;

@@ -32,6 +32,14 @@ or_scene* or_scene_create(uint32_t n_vertices, const float* vertices, const floa
                           uint32_t n_lights, const uint32_t* light_idx, uint32_t n_materials,
                           const ct_material* materials, int32_t sky_enabled, const float* sky);
 void or_scene_destroy(or_scene* s);
+/* Texture coordinates (2 per vertex, NULL = zeros), textures (copied) and the
+ * sky texture (-1 = constant sky).  Returns 0, or 1 for an index out of range. */
+int or_scene_set_textures(or_scene* s, const float* texcoords, uint32_t n_textures, const ct_texture* textures,
+                          int32_t sky_texture);
+/* The texel SceneData::getTextureColor returns for (u, v). */
+int or_texel(const or_scene* s, int32_t texture, float u, float v, float* rgba);
+float or_atan2f(float y, float x);
+void or_sky_uv(float dx, float dy, float dz, float* u, float* v);
 
 /* Build the binary SAH tree by the reference's 30-plane rescan and collapse
  * it to the 4-wide tree.  Returns 0 on success. */

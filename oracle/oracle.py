@@ -42,6 +42,10 @@ class RenderStats(C.Structure):
         return {k: int(getattr(self, k)) for k, _ in self._fields_}
 
 
+class Texture(C.Structure):
+    _fields_ = [("width", C.c_uint32), ("height", C.c_uint32), ("rgba", C.POINTER(C.c_float))]
+
+
 if not LIB_PATH.exists():
     raise ImportError(f"{LIB_PATH} missing: run make -C oracle")
 lib = C.CDLL(str(LIB_PATH))
@@ -52,6 +56,10 @@ _sig = {
     "or_scene_create": (_P, [C.c_uint32, _fptr, _fptr, C.c_uint32, _u32p, _fptr, _u32p, C.c_uint32, _u32p,
                              C.c_uint32, C.POINTER(Material), C.c_int32, _fptr]),
     "or_scene_destroy": (None, [_P]),
+    "or_scene_set_textures": (C.c_int, [_P, _fptr, C.c_uint32, C.POINTER(Texture), C.c_int32]),
+    "or_texel": (C.c_int, [_P, C.c_int32, C.c_float, C.c_float, _fptr]),
+    "or_atan2f": (C.c_float, [C.c_float, C.c_float]),
+    "or_sky_uv": (None, [C.c_float, C.c_float, C.c_float, _fptr, _fptr]),
     "or_scene_build": (C.c_int, [_P]),
     "or_scene_tree_sizes": (C.c_int, [_P, _u32p, _u32p, _u32p]),
     "or_scene_tree_export": (C.c_int, [_P, _P, _P, _u32p]),
@@ -124,9 +132,26 @@ class Scene:
                                       _fp(a["center_normals"]), _up(a["material_idx"]), int(li.size),
                                       _up(self._li), len(mats_src), self._mats, int(arrays.get("sky_enabled", 0)),
                                       _fp(self._sky))
+        texs = arrays.get("textures") or []
+        if texs or "texcoords" in arrays:
+            self._tex = [np.ascontiguousarray(t, np.float32) for t in texs]
+            self._tex_views = (Texture * max(len(texs), 1))()
+            for i, t in enumerate(self._tex):
+                self._tex_views[i] = Texture(t.shape[1], t.shape[0], _fp(t))
+            tc = arrays.get("texcoords")
+            self._uv = np.ascontiguousarray(tc, np.float32) if tc is not None and len(tc) else None
+            if lib.or_scene_set_textures(self._h, _fp(self._uv) if self._uv is not None else None, len(texs),
+                                         self._tex_views, int(arrays.get("sky_texture", -1))) != 0:
+                raise ValueError("texture index out of range")
         if build:
             if lib.or_scene_build(self._h) != 0:
                 raise RuntimeError("oracle build failed")
+
+    def texel(self, texture: int, u: float, v: float) -> np.ndarray:
+        out = np.zeros(4, np.float32)
+        if lib.or_texel(self._h, texture, u, v, _fp(out)) != 0:
+            raise ValueError("bad texture index")
+        return out
 
     def trees(self):
         nb, nm, npr = C.c_uint32(), C.c_uint32(), C.c_uint32()

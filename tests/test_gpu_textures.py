@@ -1,0 +1,208 @@
+"""GPU tests of textures, normal maps and the sky texture:
+the textured atrium (config 4) against the oracle (FP32 per pixel, rel-MSE
+against FP64), the invariants of the CPU tests on the GPU (a texture of the
+albedo's colour, a flat normal map and a 1x1 sky texture change no bit),
+material edits that switch textures on, and the upload's index checks."""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+
+def _bits(a):
+    return np.ascontiguousarray(a).view(np.uint32)
+
+
+def _relmse(x, ref):
+    return float(np.mean((x - ref) ** 2 / (ref ** 2 + 1e-2)))
+
+
+class _View:
+    """A ct_scene_view over numpy copies of a scene's arrays, with textures."""
+
+    def __init__(self, ct, arrays, textures=(), mats=None, sky=-1):
+        self.keep = []
+
+        def f(a, dt=np.float32):
+            a = np.ascontiguousarray(a, dt)
+            self.keep.append(a)
+            return a
+
+        v = ct.SceneView()
+        vert = f(arrays["vertices"])
+        v.n_vertices = vert.shape[0]
+        v.vertices = vert.ctypes.data_as(C.POINTER(C.c_float))
+        v.normals = f(arrays["normals"]).ctypes.data_as(C.POINTER(C.c_float))
+        v.texcoords = f(arrays["texcoords"]).ctypes.data_as(C.POINTER(C.c_float))
+        idx = f(arrays["indices"], np.uint32)
+        v.n_triangles = idx.shape[0]
+        v.indices = idx.ctypes.data_as(C.POINTER(C.c_uint32))
+        v.center_normals = f(arrays["center_normals"]).ctypes.data_as(C.POINTER(C.c_float))
+        v.material_idx = f(arrays["material_idx"], np.uint32).ctypes.data_as(C.POINTER(C.c_uint32))
+        li = f(arrays["light_indices"], np.uint32)
+        v.n_lights = li.shape[0]
+        v.light_indices = li.ctypes.data_as(C.POINTER(C.c_uint32))
+        src = mats if mats is not None else arrays["materials"]
+        self.mats = (ct.Material * len(src))(*src)
+        v.n_materials = len(src)
+        v.materials = self.mats
+        v.prim_indices = f(arrays["prim_indices"], np.uint32).ctypes.data_as(C.POINTER(C.c_uint32))
+        bvh = f(arrays["bvh"], np.uint8)
+        v.n_bvh_nodes = bvh.size // 32
+        v.bvh_nodes = C.cast(bvh.ctypes.data, C.POINTER(ct.BVHNode))
+        mbvh = f(arrays["mbvh"], np.uint8)
+        v.n_mbvh_nodes = mbvh.size // 128
+        v.mbvh_nodes = C.cast(mbvh.ctypes.data, C.POINTER(ct.MBVHNode))
+        v.sky_enabled = arrays["sky_enabled"] or sky >= 0
+        v.sky[:] = list(arrays["sky"])
+        self.tex = [f(t) for t in textures]
+        self.tv = (ct.Texture * max(len(self.tex), 1))()
+        for i, t in enumerate(self.tex):
+            self.tv[i] = ct.Texture(t.shape[1], t.shape[0], t.ctypes.data_as(C.POINTER(C.c_float)))
+        v.n_textures = len(self.tex)
+        v.textures = self.tv
+        v.sky_texture = sky
+        self.view = v
+
+
+def _mats(ct, arrays):
+    """Copies of a scene's materials (the cached arrays stay untouched)."""
+    return [ct.Material.from_buffer_copy(m) for m in arrays["materials"]]
+
+
+def _upload(ctx, view: _View):
+    ctx.upload_view(view.view)  # the _View keeps the arrays alive during the call
+
+
+def _frame(ctx, cam, opts, w, h):
+    ctx.resize(w, h)
+    ctx.trace_frame(cam, opts)
+    return ctx.read_accum()[0]
+
+
+def test_textured_atrium_vs_oracle(ct, orc, gpu_ctx):
+    """Config 4 (tiles + normal map on the floors, stone columns, sky
+    texture) at 192x108, 2 spp: every value within 1e-4 of the FP32 oracle,
+    rel-MSE vs FP64 within 1e-3."""
+    scene = ct.HostScene.generate(4, 7)
+    arrays = scene.arrays()
+    osc = orc.Scene(arrays)
+    gpu_ctx.upload(scene)
+    opts, _, _ = scene.frame_opts()
+    w, h = 192, 108
+    opts.spp = 2
+    cam = scene.camera(w, h)
+    acc = _frame(gpu_ctx, cam, opts, w, h)
+    ref, _ = osc.render(cam, opts, w, h, 0, 2, threads=16)
+    diff = np.abs(acc - ref)
+    rel = diff / np.maximum(np.abs(ref), 1e-30)
+    bad = int(np.sum((rel > 1e-4) & (diff > 1e-7)))
+    exact = int(np.sum(_bits(acc) == _bits(ref)))
+    print(f"textured atrium: over 1e-4 rel {bad}, bit-identical {exact}/{acc.size}")
+    assert bad == 0
+    ref64, _ = osc.render(cam, opts, w, h, 0, 2, precision=64, threads=16)
+    r = _relmse(acc[..., :3] / acc[..., 3:], ref64[..., :3] / ref64[..., 3:])
+    print(f"textured atrium rel-MSE vs FP64: {r:.3e}")
+    assert r <= 1e-3
+    # the textures are visible: the untextured atrium differs
+    gpu_ctx.upload(ct.HostScene.generate(2, 7))
+    assert not np.array_equal(_frame(gpu_ctx, cam, opts, w, h), acc)
+
+
+def test_gpu_texture_invariants(ct, orc, gpu_ctx, c1, c2):
+    scene, arrays, _ = c1
+    opts, _, _ = scene.frame_opts()
+    w = h = 64
+    cam = scene.camera(w, h)
+    _upload(gpu_ctx, _View(ct, arrays))
+    base = _frame(gpu_ctx, cam, opts, w, h)
+    mats = _mats(ct, arrays)
+    texs = []
+    for m in mats:
+        if m.type != ct.MAT_LIGHT:
+            t = np.zeros((3, 5, 4), np.float32)
+            t[..., :3] = np.array(m.albedo[:], np.float32)
+            t[..., 3] = 1.0
+            texs.append(t)
+            m.diffuse_tex = len(texs) - 1
+    flat = np.zeros((2, 2, 4), np.float32)
+    flat[..., 2] = 1.0
+    texs.append(flat)
+    for m in mats:
+        if m.type != ct.MAT_LIGHT:
+            m.normal_tex = len(texs) - 1
+    vw = _View(ct, arrays, texs, mats)
+    _upload(gpu_ctx, vw)
+    assert np.array_equal(_bits(_frame(gpu_ctx, cam, opts, w, h)), _bits(base))
+    # the same textured scene against the oracle with a tilted normal map
+    tilted = np.zeros((2, 2, 4), np.float32)
+    tilted[..., 0] = 0.6
+    tilted[..., 2] = 0.8
+    texs2 = texs[:-1] + [tilted]
+    _upload(gpu_ctx, _View(ct, arrays, texs2, mats))
+    acc = _frame(gpu_ctx, cam, opts, w, h)
+    a = dict(arrays)
+    a["materials"] = vw.mats
+    a["textures"] = texs2
+    a["sky_texture"] = -1
+    ref, _ = orc.Scene(a).render(cam, opts, w, h, 0, 1, threads=16)
+    diff = np.abs(acc - ref)
+    assert int(np.sum((diff / np.maximum(np.abs(ref), 1e-30) > 1e-4) & (diff > 1e-7))) == 0
+    assert not np.array_equal(acc, base)
+    # sky: a 1x1 texture of the constant colour
+    scene2, arrays2, _ = c2
+    opts2, _, _ = scene2.frame_opts()
+    cam2 = scene2.camera(96, 54)
+    _upload(gpu_ctx, _View(ct, arrays2))
+    sky0 = _frame(gpu_ctx, cam2, opts2, 96, 54)
+    one = np.zeros((1, 1, 4), np.float32)
+    one[0, 0, :3] = arrays2["sky"]
+    _upload(gpu_ctx, _View(ct, arrays2, [one], sky=0))
+    assert np.array_equal(_bits(_frame(gpu_ctx, cam2, opts2, 96, 54)), _bits(sky0))
+
+
+def test_material_update_switches_texture_on(ct, orc, gpu_ctx):
+    """ct_material_update with a diffuse texture index renders like an
+    upload of the edited scene (oracle of the edited materials)."""
+    scene = ct.HostScene.generate(4, 7)
+    arrays = scene.arrays()
+    gpu_ctx.upload(scene)
+    opts, _, _ = scene.frame_opts()
+    w, h = 96, 54
+    cam = scene.camera(w, h)
+    mats = _mats(ct, arrays)
+    k = next(i for i, m in enumerate(mats) if m.type == ct.MAT_LAMBERTIAN and m.diffuse_tex < 0 and i > 0)
+    m = ct.Material()
+    C.memmove(C.byref(m), C.byref(mats[k]), C.sizeof(ct.Material))
+    m.diffuse_tex = 2  # the stone texture
+    gpu_ctx.material_update(k, m)
+    acc = _frame(gpu_ctx, cam, opts, w, h)
+    mats[k] = m
+    a = dict(arrays)
+    a["materials"] = (ct.Material * len(mats))(*mats)
+    ref, _ = orc.Scene(a).render(cam, opts, w, h, 0, 1, threads=16)
+    diff = np.abs(acc - ref)
+    rel = diff / np.maximum(np.abs(ref), 1e-30)
+    assert int(np.sum((rel > 1e-4) & (diff > 1e-7))) == 0
+    bad = ct.Material()
+    C.memmove(C.byref(bad), C.byref(m), C.sizeof(ct.Material))
+    bad.normal_tex = 99
+    with pytest.raises(ct.CTError):
+        gpu_ctx.material_update(k, bad)
+
+
+def test_texture_index_checks(ct, gpu_ctx, c1):
+    scene, arrays, _ = c1
+    mats = _mats(ct, arrays)
+    mats[1].diffuse_tex = 0  # no textures in the view
+    with pytest.raises(ct.CTError):
+        _upload(gpu_ctx, _View(ct, arrays, (), mats))
+    t = np.ones((2, 2, 4), np.float32)
+    with pytest.raises(ct.CTError):
+        _upload(gpu_ctx, _View(ct, arrays, [t], sky=1))
+    # a zeroed sky_texture without textures keeps the constant sky
+    _upload(gpu_ctx, _View(ct, arrays, (), None, sky=0))
