@@ -57,6 +57,8 @@ WORKLOADS = {
        "typed BSDF (Lambert/GGX/mirror/dielectric) + 4 quad lights + sky",
     3: "config 3: displaced icosphere + noisy ground (%d triangles), 3840x2160, 1 spp, depth 16, Russian roulette "
        "from bounce 3, quad BVH, %s traversal, Beckmann/GGX microfacets + 1 area light",
+    4: "config 2 textured: procedural atrium (%d triangles), 1920x1080, 1 spp, depth 8, MIS+NEE, quad BVH, %s "
+       "traversal, typed BSDF; tiled + normal-mapped floors, stone columns, 1024x512 sky texture",
 }
 HBM_PEAK_GBS = 8000.0
 
@@ -68,8 +70,9 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--seed", type=int, default=7)
     ap.add_argument("--traversal", default="exact", choices=["exact", "fast"])
-    ap.add_argument("--config", type=int, default=2, choices=[2, 3],
-                    help="2: atrium 1080p depth 8 (the headline workload); 3: displaced sphere 4K depth 16, RR")
+    ap.add_argument("--config", type=int, default=2, choices=[2, 3, 4],
+                    help="2: atrium 1080p depth 8 (the headline workload); 3: displaced sphere 4K depth 16, RR; "
+                         "4: the atrium with textures, normal maps and a sky texture")
     ap.add_argument("--cpu-seconds", type=float, default=10.0, help="minimum CPU-baseline work")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
@@ -97,7 +100,7 @@ def main() -> int:
     import numpy as np
     import cudatracer_amd as ct
 
-    scene = ct.HostScene.generate(ct.SCENE_C2 if args.config == 2 else ct.SCENE_C3, args.seed)
+    scene = ct.HostScene.generate({2: ct.SCENE_C2, 3: ct.SCENE_C3, 4: ct.SCENE_C2_TEXTURED}[args.config], args.seed)
     opts, W, H = scene.frame_opts()
     opts.traversal = ct.TRAV_FAST if args.traversal == "fast" else ct.TRAV_EXACT
     if args.sort >= 0:
@@ -211,7 +214,7 @@ def main() -> int:
     }
 
     if not args.no_cpu:
-        out["accuracy"] = accuracy_strip(ctx, scene, cam, opts, W, H, strip=32 if args.config == 2 else 128)
+        out["accuracy"] = accuracy_strip(ctx, scene, cam, opts, W, H, strip=128 if args.config == 3 else 32)
     if not args.no_serve and args.config == 2:
         out["served"] = serve_leg(ct, ctx, scene, opts, args.seed, chunk=args.serve_chunk)
         out["batched"] = batched_leg(ct, ctx, args.seed)
@@ -223,7 +226,7 @@ def main() -> int:
         out["build"]["cpu_baseline"] = cpu_build_baseline(scene)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(scene, cam, opts, W, H, args.cpu_seconds,
-                                           strip=1 if args.config == 2 else 16)
+                                           strip=16 if args.config == 3 else 1)
     print(json.dumps(out))
     return 0
 

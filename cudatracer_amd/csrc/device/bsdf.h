@@ -54,10 +54,24 @@ CT_DEV f3 sample_to_world(f3 s, f3 N) {
     const f3 B = cross(N, T);
     return s.x * T + s.y * B + s.z * N;
 }
-// Nearest texel of texture t at (u, v) (SceneData::getTextureColor, SceneData.cuh:81-97).
-CT_DEV float4 tex_fetch(const float4* __restrict__ texels, const uint4* __restrict__ info, int t, float u, float v) {
+// Nearest texel of texture t at (u, v) (SceneData::getTextureColor,
+// SceneData.cuh:81-97).  info[t] = {first texel, width, height, format}:
+// format 0 texels are float4; format 1 texels are 8-bit RGB (alpha 1) that
+// decode to c / 255.0f, the floats the reference's Surface holds for 8-bit
+// images, in a quarter of the bytes.  Non-temporal loads: texels are
+// fetched at random during shading and should not evict the tree from L2.
+CT_DEV float4 tex_fetch(const float4* __restrict__ texels, const uint32_t* __restrict__ texels8,
+                        const uint4* __restrict__ info, int t, float u, float v) {
     const uint4 ti = info[t];
-    return texels[ti.x + ct_texel_index(ti.y, ti.z, u, v)];
+    const uint32_t i = ti.x + ct_texel_index(ti.y, ti.z, u, v);
+    if (ti.w) {
+        const uint32_t p = __builtin_nontemporal_load(texels8 + i);
+        return make_float4((float)(p & 255u) / 255.0f, (float)((p >> 8) & 255u) / 255.0f,
+                           (float)((p >> 16) & 255u) / 255.0f, 1.0f);
+    }
+    const float* f = reinterpret_cast<const float*>(texels + i);
+    return make_float4(__builtin_nontemporal_load(f), __builtin_nontemporal_load(f + 1),
+                       __builtin_nontemporal_load(f + 2), __builtin_nontemporal_load(f + 3));
 }
 CT_DEV f3 tangent_to_world(f3 V, f3 N) {
     f3 T, B;

@@ -58,8 +58,9 @@ struct DevScene {
     const float4* sh_nrm;   // 4 per primitive: n0, n1, n2, centre normal
     const float4* mats;     // 2 per material: {albedo, roughness}, {type bits, ior, alpha, 0}
     const float4* lights;   // 5 per light: v0, v1, v2, centre normal (w = area), emission
-    const float4* texels;   // all textures' RGBA texels, concatenated
-    const uint4* tex_info;  // per texture: first texel, width, height, 0
+    const float4* texels;     // float textures' RGBA texels, concatenated
+    const uint32_t* texels8;  // 8-bit textures' RGB texels (byte 3 unused), concatenated
+    const uint4* tex_info;    // per texture: first texel (in its buffer), width, height, format (0 float, 1 8-bit)
     const float4* sh_uv;    // 2 per primitive: {uv0, uv1}, {uv2, 0, 0} (only with textures)
     uint32_t n_tris, n_nodes4, n_nodes2, n_fnodes, n_mats, n_lights, n_tex;
     int sky_enabled;

@@ -296,7 +296,7 @@ __global__ void __launch_bounds__(kShadeBlock) k_shade(DevScene sc, ShadeOpts so
                     if (sc.sky_tex >= 0) {  // Kernel.cu:96-100
                         float u, v;
                         ct_sky_uv(dir.x, dir.y, dir.z, &u, &v);
-                        const float4 c = tex_fetch(sc.texels, sc.tex_info, sc.sky_tex, u, v);
+                        const float4 c = tex_fetch(sc.texels, sc.texels8, sc.tex_info, sc.sky_tex, u, v);
                         sky = mk3(c.x, c.y, c.z);
                     }
                     add = (thr / lastPdf) * sky;
@@ -326,7 +326,7 @@ __global__ void __launch_bounds__(kShadeBlock) k_shade(DevScene sc, ShadeOpts so
                     tv = (bary.x * U0.y + bary.y * U0.w) + bary.z * U1.y;
                 }
                 if (ntex >= 0) {
-                    const float4 c = tex_fetch(sc.texels, sc.tex_info, ntex, tu, tv);
+                    const float4 c = tex_fetch(sc.texels, sc.texels8, sc.tex_info, ntex, tu, tv);
                     iN = sample_to_world(normalize(mk3(c.x, c.y, c.z)), iN);
                 }
                 if (mtype == MAT_LIGHT) {
@@ -354,7 +354,7 @@ __global__ void __launch_bounds__(kShadeBlock) k_shade(DevScene sc, ShadeOpts so
                     const float r4 = ct_rand_float(&seed);
                     f3 color = xyz(M0);
                     if (dtex >= 0) {
-                        const float4 c = tex_fetch(sc.texels, sc.tex_info, dtex, tu, tv);
+                        const float4 c = tex_fetch(sc.texels, sc.texels8, sc.tex_info, dtex, tu, tv);
                         color = mk3(c.x, c.y, c.z);
                     }
                     const f3 wiw = -dir;

@@ -206,6 +206,8 @@ void add_grid(TriangleList& L, float y, float ext, int n, float hole, vec3 norma
 }
 
 // Procedural textures of the textured atrium (RGBA, row 0 at the bottom).
+// Colour textures are quantised to 8 bits (c / 255), like image files.
+float q8(float v) { return std::nearbyint(std::fmin(std::fmax(v, 0.0f), 1.0f) * 255.0f) / 255.0f; }
 // Floor tiles: 8 x 8 tiles of two tones with dark grout and value-noise grain.
 int tiles_texture(TriangleList& L, uint64_t seed) {
     const uint32_t n = 1024;
@@ -218,9 +220,9 @@ int tiles_texture(TriangleList& L, uint64_t seed) {
             const float g = 0.06f * vn.eval((float)x / 24.0f, (float)y / 24.0f, 0.5f);
             const vec3 c = grout ? vec3(0.12f) : dark ? vec3(0.32f, 0.30f, 0.28f) : vec3(0.72f, 0.68f, 0.62f);
             float* p = &t[4 * ((size_t)x + (size_t)y * n)];
-            p[0] = c.x + g;
-            p[1] = c.y + g;
-            p[2] = c.z + g;
+            p[0] = q8(c.x + g);
+            p[1] = q8(c.y + g);
+            p[2] = q8(c.z + g);
             p[3] = 1.0f;
         }
     return L.addTexture(n, n, t.data());
@@ -256,16 +258,16 @@ int marble_texture(TriangleList& L, uint64_t seed) {
             const float f = vn.fbm(vec3((float)x / 64.0f, (float)y / 64.0f, 0.25f), 4);
             const float b = 0.5f + 0.5f * std::sin(0.05f * (float)y + 6.0f * f);
             float* p = &t[4 * ((size_t)x + (size_t)y * n)];
-            p[0] = 0.55f + 0.35f * b;
-            p[1] = 0.52f + 0.33f * b;
-            p[2] = 0.50f + 0.30f * b;
+            p[0] = q8(0.55f + 0.35f * b);
+            p[1] = q8(0.52f + 0.33f * b);
+            p[2] = q8(0.50f + 0.30f * b);
             p[3] = 1.0f;
         }
     return L.addTexture(n, n, t.data());
 }
 
 // Equirectangular sky (u = azimuth, v = 1 - polar angle / pi): zenith-to-
-// horizon gradient, dim ground, and a small bright sun.
+// horizon gradient over a dim ground, with faint azimuthal banding, 8-bit.
 int sky_texture(TriangleList& L) {
     const uint32_t w = 1024, h = 512;
     const float pi = 3.14159265358979323846f;
@@ -281,12 +283,11 @@ int sky_texture(TriangleList& L) {
                 const float s = std::sin(elev);
                 c = vec3(0.75f, 0.8f, 0.88f) * (1.0f - s) + vec3(0.22f, 0.36f, 0.66f) * s;
             }
-            const float du = std::fabs(u - 0.3f), dv = std::fabs(v - 0.8f);
-            if (du * du * 4.0f + dv * dv < 0.0004f) c = vec3(30.0f, 28.0f, 24.0f);
+            c = c * (0.92f + 0.08f * std::sin(6.28318530717958647692f * 4.0f * u));
             float* p = &t[4 * ((size_t)x + (size_t)y * w)];
-            p[0] = c.x;
-            p[1] = c.y;
-            p[2] = c.z;
+            p[0] = q8(c.x);
+            p[1] = q8(c.y);
+            p[2] = q8(c.z);
             p[3] = 1.0f;
         }
     return L.addTexture(w, h, t.data());

@@ -201,8 +201,25 @@ int check_geometry(ct_ctx* c, const ct_scene_view* v) {
 struct Shading {
     std::vector<float4> pos, nrm, mats, lights;
     std::vector<float4> texels, uv;  // uv only when the scene has textures
+    std::vector<uint32_t> texels8;
     std::vector<uint4> tex_info;
 };
+
+// An 8-bit texture: alpha 1 and every colour value c / 255.0f for an integer
+// c in [0, 255] (what Surface holds for 8-bit images); stored as bytes.
+bool is_8bit(const ct_texture& x) {
+    const size_t n = (size_t)x.width * x.height;
+    for (size_t i = 0; i < n; ++i) {
+        if (x.rgba[4 * i + 3] != 1.0f) return false;
+        for (int k = 0; k < 3; ++k) {
+            const float v = x.rgba[4 * i + k];
+            if (!(v >= 0.0f && v <= 1.0f)) return false;
+            const float c = std::nearbyint(v * 255.0f);
+            if ((float)(uint32_t)c / 255.0f != v) return false;
+        }
+    }
+    return true;
+}
 
 void stage_shading(const ct_scene_view* v, Shading& sh) {
     const uint32_t N = v->n_triangles;
@@ -223,10 +240,19 @@ void stage_shading(const ct_scene_view* v, Shading& sh) {
     // TriangleList::createTextureBuffer (TriangleList.cpp:148-178)
     for (uint32_t t = 0; t < v->n_textures; ++t) {
         const ct_texture& x = v->textures[t];
-        sh.tex_info.push_back(make_uint4((uint32_t)sh.texels.size(), x.width, x.height, 0u));
         const size_t n = (size_t)x.width * x.height;
-        for (size_t i = 0; i < n; ++i)
-            sh.texels.push_back(make_float4(x.rgba[4 * i], x.rgba[4 * i + 1], x.rgba[4 * i + 2], x.rgba[4 * i + 3]));
+        if (is_8bit(x)) {
+            sh.tex_info.push_back(make_uint4((uint32_t)sh.texels8.size(), x.width, x.height, 1u));
+            for (size_t i = 0; i < n; ++i) {
+                uint32_t p = 0;
+                for (int k = 0; k < 3; ++k) p |= (uint32_t)std::nearbyint(x.rgba[4 * i + k] * 255.0f) << (8 * k);
+                sh.texels8.push_back(p);
+            }
+        } else {
+            sh.tex_info.push_back(make_uint4((uint32_t)sh.texels.size(), x.width, x.height, 0u));
+            for (size_t i = 0; i < n; ++i)
+                sh.texels.push_back(make_float4(x.rgba[4 * i], x.rgba[4 * i + 1], x.rgba[4 * i + 2], x.rgba[4 * i + 3]));
+        }
     }
     if (v->n_textures) {
         sh.uv.resize((size_t)N * 2);
@@ -249,8 +275,11 @@ int upload_shading(ct_ctx* c, const Shading& sh, ctd::DevScene& sc) {
     if (r == CT_OK) r = upload_vec(c, sh.nrm, &sc.sh_nrm, "shading normals");
     if (r == CT_OK) r = upload_vec(c, sh.mats, &sc.mats, "materials");
     if (r == CT_OK) r = upload_vec(c, sh.lights, &sc.lights, "lights");
-    if (r == CT_OK && !sh.texels.empty()) {
+    if (r == CT_OK && !sh.tex_info.empty()) {
         r = upload_vec(c, sh.texels, &sc.texels, "textures");
+        const float4* t8 = nullptr;
+        if (r == CT_OK) r = upload_vec(c, sh.texels8, &t8, "8-bit textures");
+        sc.texels8 = reinterpret_cast<const uint32_t*>(t8);
         const float4* ti = nullptr;
         if (r == CT_OK) r = upload_vec(c, sh.tex_info, &ti, "texture table");
         sc.tex_info = reinterpret_cast<const uint4*>(ti);
