@@ -213,6 +213,7 @@ def main() -> int:
         "roofline": roofline,
     }
 
+    out["display"] = display_leg(ctx, W, H)
     if not args.no_cpu:
         out["accuracy"] = accuracy_strip(ctx, scene, cam, opts, W, H, strip=128 if args.config == 3 else 32)
     if not args.no_serve and args.config == 2:
@@ -456,6 +457,25 @@ def copy_probe(mib: int) -> int:
     torch.cuda.synchronize()
     print(json.dumps({"copy_probe_bytes": n * 4, "dispatches": 5}))
     return 0
+
+
+def display_leg(ctx, W, H, reps: int = 10) -> dict:
+    """draw_framebuffer's replacement for a viewer: tonemap the accumulator
+    and copy the image to (pageable) host memory, as RGBA32F (the reference's
+    surface format) and as RGBA8."""
+    import numpy as np
+    f = np.empty((H, W, 4), np.float32)
+    b = np.empty((H, W, 4), np.uint8)
+    res = {"pixels": W * H}
+    for name, fn, nbytes in (("rgba32f", lambda: ctx.read_image(), f.nbytes), ("rgba8", lambda: ctx.read_image_rgba8(),
+                                                                              b.nbytes)):
+        fn()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            fn()
+        ms = (time.perf_counter() - t0) * 1e3 / reps
+        res[name] = {"ms_per_image": round(ms, 3), "bytes": nbytes, "gb_per_s": round(nbytes / (ms * 1e-3) / 1e9, 2)}
+    return res
 
 
 def build_leg(ctx, scene, wide: bool, reps: int = 3) -> dict:

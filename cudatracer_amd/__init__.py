@@ -176,6 +176,7 @@ _sig = {
     "ct_read_accum": (C.c_int, [_P, C.POINTER(C.c_float), C.c_size_t, C.POINTER(C.c_int32)]),
     "ct_write_accum": (C.c_int, [_P, C.POINTER(C.c_float), C.c_size_t, C.c_int32]),
     "ct_read_image": (C.c_int, [_P, C.POINTER(C.c_float), C.c_size_t]),
+    "ct_read_image_rgba8": (C.c_int, [_P, C.POINTER(C.c_uint8), C.c_size_t]),
     "ct_trace_rays": (C.c_int, [_P, C.POINTER(C.c_float), C.POINTER(C.c_float), C.c_uint32, C.POINTER(FrameOpts),
                                 C.POINTER(C.c_float), C.POINTER(C.c_int32)]),
     "ct_trace_shadow_rays": (C.c_int, [_P, C.POINTER(C.c_float), C.POINTER(C.c_float), C.POINTER(C.c_float),
@@ -491,6 +492,11 @@ class Context:
     def read_image(self):
         out = np.empty((self.height, self.width, 4), np.float32)
         _check(lib.ct_read_image(self._h, _fp(out), out.size), self._h)
+        return out
+
+    def read_image_rgba8(self):
+        out = np.empty((self.height, self.width, 4), np.uint8)
+        _check(lib.ct_read_image_rgba8(self._h, out.ctypes.data_as(C.POINTER(C.c_uint8)), out.size), self._h)
         return out
 
     def trace_rays(self, origins, dirs, opts: FrameOpts):

@@ -650,6 +650,27 @@ hipError_t launch_resolve(const LaunchShape& ls, const ViewDesc* views, const Se
     hipLaunchKernelGGL(k_resolve, dim3(blocks), dim3(256), 0, s, views, segs, hdr, lpath, accum);
     return hipGetLastError();
 }
+// The tonemapped image as 8-bit RGBA (alpha 255): byte = clamp(v, 0, 1) * 255 + 0.5, truncated.
+__global__ void k_tonemap8(const float4* __restrict__ accum, uint32_t n, float samples, uint32_t* out) {
+    const float e = 1.0f / 2.2f;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const float4 a = accum[i];
+        const f3 c = mk3(a.x, a.y, a.z) / samples;
+        const float v[3] = {ct_powf(c.x, e), ct_powf(c.y, e), ct_powf(c.z, e)};
+        uint32_t p = 0xFF000000u;
+        for (int k = 0; k < 3; ++k) {
+            const float q = v[k] > 0.0f ? (v[k] < 1.0f ? v[k] : 1.0f) : 0.0f;
+            p |= (uint32_t)(q * 255.0f + 0.5f) << (8 * k);
+        }
+        out[i] = p;
+    }
+}
+hipError_t launch_tonemap8(const float4* accum, uint32_t n, float samples, uint32_t* out, hipStream_t s) {
+    const uint32_t blocks = std::min<uint32_t>((n + 255) / 256, 65536u);
+    if (blocks == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_tonemap8, dim3(blocks), dim3(256), 0, s, accum, n, samples, out);
+    return hipGetLastError();
+}
 hipError_t launch_tonemap(const float4* accum, uint32_t n, float samples, float4* out, hipStream_t s) {
     const uint32_t blocks = std::min<uint32_t>((n + 255) / 256, 65536u);
     if (blocks == 0) return hipSuccess;

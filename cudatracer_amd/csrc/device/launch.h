@@ -37,6 +37,7 @@ hipError_t launch_resolve(const LaunchShape& ls, const ctd::ViewDesc* views, con
                           const ctd::ChunkHdr* hdr, uint32_t max_tasks, const float4* lpath, float4* accum,
                           hipStream_t s);
 hipError_t launch_tonemap(const float4* accum, uint32_t n, float samples, float4* out, hipStream_t s);
+hipError_t launch_tonemap8(const float4* accum, uint32_t n, float samples, uint32_t* out, hipStream_t s);
 hipError_t launch_shadow_flags(const LaunchShape& ls, const ctd::DevScene& sc, const ctd::TraceOpts& to,
                                const ctd::Queues& q, uint32_t n, uint8_t* out, hipStream_t s);
 

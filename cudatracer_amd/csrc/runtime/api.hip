@@ -618,19 +618,47 @@ int ct_write_accum(ct_ctx* c, const float* rgba, size_t n_floats, int32_t sample
     return CT_OK;
 }
 
+// Device buffer for the tonemapped image, grown on demand and kept.
+static int image_buffer(ct_ctx* c, size_t bytes) {
+    if (c->image_bytes >= bytes) return CT_OK;
+    dfree(c, c->image);
+    c->image = nullptr;
+    c->image_bytes = 0;
+    const int r = dalloc(c, &c->image, bytes, "image");
+    if (r == CT_OK) c->image_bytes = bytes;
+    return r;
+}
+
 int ct_read_image(ct_ctx* c, float* rgba, size_t n_floats) {
     if (!c || !rgba) return CT_ERR_INVALID_ARG;
     if (!c->accum) return fail(c, CT_ERR_INVALID_ARG, "no frame buffer");
     const size_t npix = (size_t)c->width * c->height;
     if (n_floats < npix * 4) return fail(c, CT_ERR_INVALID_ARG, "output buffer too small");
     (void)hipSetDevice(c->device);
-    void* tmp = nullptr;
-    int r = dalloc(c, &tmp, npix * sizeof(float4), "image");
+    sync_lanes(c);
+    int r = image_buffer(c, npix * sizeof(float4));
     if (r != CT_OK) return r;
-    hipError_t e = ctk::launch_tonemap(c->accum, (uint32_t)npix, (float)std::max(c->samples, 1), (float4*)tmp, c->ln[0].stream);
-    if (e == hipSuccess) e = hipMemcpyAsync(rgba, tmp, npix * sizeof(float4), hipMemcpyDeviceToHost, c->ln[0].stream);
+    hipError_t e = ctk::launch_tonemap(c->accum, (uint32_t)npix, (float)std::max(c->samples, 1), (float4*)c->image,
+                                       c->ln[0].stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(rgba, c->image, npix * sizeof(float4), hipMemcpyDeviceToHost, c->ln[0].stream);
     if (e == hipSuccess) e = hipStreamSynchronize(c->ln[0].stream);
-    dfree(c, tmp);
+    if (e != hipSuccess) return hip_fail(c, e, "tonemap");
+    return CT_OK;
+}
+
+int ct_read_image_rgba8(ct_ctx* c, uint8_t* rgba, size_t n_bytes) {
+    if (!c || !rgba) return CT_ERR_INVALID_ARG;
+    if (!c->accum) return fail(c, CT_ERR_INVALID_ARG, "no frame buffer");
+    const size_t npix = (size_t)c->width * c->height;
+    if (n_bytes < npix * 4) return fail(c, CT_ERR_INVALID_ARG, "output buffer too small");
+    (void)hipSetDevice(c->device);
+    sync_lanes(c);
+    int r = image_buffer(c, npix * 4);
+    if (r != CT_OK) return r;
+    hipError_t e = ctk::launch_tonemap8(c->accum, (uint32_t)npix, (float)std::max(c->samples, 1), (uint32_t*)c->image,
+                                        c->ln[0].stream);
+    if (e == hipSuccess) e = hipMemcpyAsync(rgba, c->image, npix * 4, hipMemcpyDeviceToHost, c->ln[0].stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(c->ln[0].stream);
     if (e != hipSuccess) return hip_fail(c, e, "tonemap");
     return CT_OK;
 }

@@ -85,6 +85,8 @@ struct ct_ctx {
     int width = 0, height = 0;
     int32_t samples = 0;
     float4* accum = nullptr;
+    void* image = nullptr;  // tonemapped image of ct_read_image[_rgba8], kept between calls
+    size_t image_bytes = 0;
 
     // path state: buffers of 2 * lane_cap slots, lane k uses [k * lane_cap, (k + 1) * lane_cap)
     uint32_t lane_cap = 0;

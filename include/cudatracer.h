@@ -261,6 +261,10 @@ int ct_read_accum(ct_ctx* ctx, float* rgba, size_t n_floats, int32_t* samples);
 int ct_write_accum(ct_ctx* ctx, const float* rgba, size_t n_floats, int32_t samples);
 /* draw_framebuffer (Kernel.cu:258-270): rgb / samples, then x^(1/2.2). */
 int ct_read_image(ct_ctx* ctx, float* rgba, size_t n_floats);
+/* The same image as 8-bit RGBA for a viewer's texture upload (a quarter of
+ * the PCIe bytes of the RGBA32F surface the reference writes): each channel
+ * clamp(v, 0, 1) * 255 + 0.5 truncated, alpha 255.  n_bytes >= 4 * pixels. */
+int ct_read_image_rgba8(ct_ctx* ctx, uint8_t* rgba, size_t n_bytes);
 
 /* Ray-level entry points (the extend / connect stages on their own).
  * origins, dirs: 3 floats per ray (host).  Closest hit writes t (1e34 on a

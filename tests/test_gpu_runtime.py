@@ -159,6 +159,15 @@ def test_read_image_tonemap(ct, gpu_ctx, c1):
     img = gpu_ctx.read_image()
     ref = np.power(acc[..., :3].astype(np.float64) / n, 1.0 / 2.2)
     assert np.allclose(img[..., :3], ref, rtol=2e-6, atol=1e-7)
+    # the 8-bit image: the same values, clamped, * 255 + 0.5, truncated (float32 arithmetic)
+    img8 = gpu_ctx.read_image_rgba8()
+    q = np.clip(img[..., :3], np.float32(0), np.float32(1)).astype(np.float32)
+    want = (q * np.float32(255.0) + np.float32(0.5)).astype(np.uint8)
+    assert np.array_equal(img8[..., :3], want) and np.all(img8[..., 3] == 255)
+    assert img8[..., :3].max() > 0
+    # repeated reads reuse the image buffer and give the same bytes
+    assert np.array_equal(gpu_ctx.read_image_rgba8(), img8)
+    assert np.array_equal(gpu_ctx.read_image(), img)
 
 
 def test_error_returns(ct, c1):
