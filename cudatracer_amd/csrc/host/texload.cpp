@@ -1,5 +1,6 @@
 // TriangleList::loadTexture / core::Surface (TriangleList.cpp:377-391,
-// src/Utils/Surface.cpp:33-100) without FreeImage: binary PPM (P6) and PFM.
+// src/Utils/Surface.cpp:33-100) without FreeImage: TGA (the format of the
+// reference's Models/sponza textures), binary PPM (P6) and PFM.
 //
 // The reference's Surface keeps a float RGBA copy of the image in FreeImage's
 // scan-line order, whose row 0 is the bottom of the picture: 8-bit images
@@ -12,6 +13,7 @@
 #include <cstdlib>
 #include <cstring>
 #include <fstream>
+#include <iterator>
 #include <sstream>
 
 #include "scene.h"
@@ -127,6 +129,102 @@ bool load_pfm(std::istream& in, bool colour, Texture& t, std::string& why) {
     return true;
 }
 
+// Truevision TGA: types 2 (true colour), 3 (grey) and their run-length
+// encoded forms 10 and 11; 8, 24 or 32 bits per pixel stored B, G, R(, A).
+// Rows run bottom-up unless bit 5 of the descriptor (top-left origin) is
+// set.  Colour becomes c / 255 with alpha 1, the floats Surface keeps for
+// 8-bit images.
+bool load_tga(const std::vector<unsigned char>& f, Texture& t, std::string& why) {
+    if (f.size() < 18) {
+        why = "truncated TGA header";
+        return false;
+    }
+    const uint32_t idlen = f[0], cmtype = f[1], type = f[2];
+    const uint32_t w = f[12] | (uint32_t)f[13] << 8, h = f[14] | (uint32_t)f[15] << 8;
+    const uint32_t bpp = f[16], desc = f[17];
+    const bool rle = type == 10 || type == 11, grey = type == 3 || type == 11;
+    if (cmtype != 0 || !(type == 2 || type == 3 || type == 10 || type == 11)) {
+        why = "unsupported TGA type (true colour or grey, no colour map)";
+        return false;
+    }
+    if ((grey && bpp != 8) || (!grey && bpp != 24 && bpp != 32) || (desc & 0x10)) {
+        why = "unsupported TGA pixel format";
+        return false;
+    }
+    if (w == 0 || h == 0) {
+        why = "bad TGA size";
+        return false;
+    }
+    const size_t px = bpp / 8, n = (size_t)w * h;
+    std::vector<unsigned char> raw(n * px);
+    size_t pos = 18 + idlen;
+    if (!rle) {
+        if (f.size() < pos + raw.size()) {
+            why = "truncated TGA data";
+            return false;
+        }
+        std::memcpy(raw.data(), f.data() + pos, raw.size());
+    } else {
+        size_t out = 0;
+        while (out < n) {
+            if (pos >= f.size()) {
+                why = "truncated TGA data";
+                return false;
+            }
+            const uint32_t hdr = f[pos++];
+            const size_t cnt = (hdr & 0x7F) + 1;
+            if (out + cnt > n) {
+                why = "bad TGA run";
+                return false;
+            }
+            if (hdr & 0x80) {  // run of one pixel value
+                if (pos + px > f.size()) {
+                    why = "truncated TGA data";
+                    return false;
+                }
+                for (size_t i = 0; i < cnt; ++i) std::memcpy(&raw[(out + i) * px], &f[pos], px);
+                pos += px;
+            } else {  // raw packet
+                if (pos + cnt * px > f.size()) {
+                    why = "truncated TGA data";
+                    return false;
+                }
+                std::memcpy(&raw[out * px], &f[pos], cnt * px);
+                pos += cnt * px;
+            }
+            out += cnt;
+        }
+    }
+    const bool top_down = (desc & 0x20) != 0;
+    t.width = w;
+    t.height = h;
+    t.rgba.resize(n * 4);
+    for (uint32_t y = 0; y < h; ++y) {
+        const uint32_t src_row = top_down ? h - 1 - y : y;
+        for (uint32_t x = 0; x < w; ++x) {
+            const unsigned char* p = &raw[((size_t)src_row * w + x) * px];
+            float* q = &t.rgba[4 * ((size_t)x + (size_t)y * w)];
+            if (grey) {
+                q[0] = q[1] = q[2] = (float)p[0] / 255.0f;
+            } else {
+                q[0] = (float)p[2] / 255.0f;
+                q[1] = (float)p[1] / 255.0f;
+                q[2] = (float)p[0] / 255.0f;
+            }
+            q[3] = 1.0f;
+        }
+    }
+    return true;
+}
+
+bool has_suffix(const std::string& s, const char* suf) {
+    const size_t n = std::strlen(suf);
+    if (s.size() < n) return false;
+    for (size_t i = 0; i < n; ++i)
+        if (std::tolower((unsigned char)s[s.size() - n + i]) != suf[i]) return false;
+    return true;
+}
+
 }  // namespace
 
 int TriangleList::addTexture(uint32_t width, uint32_t height, const float* rgba) {
@@ -148,12 +246,15 @@ int TriangleList::loadTexture(const std::string& path, std::string* error) {
     bool ok = false;
     if (!in) {
         why = "cannot open " + path;
+    } else if (has_suffix(path, ".tga")) {  // TGA has no magic number
+        std::vector<unsigned char> f((std::istreambuf_iterator<char>(in)), std::istreambuf_iterator<char>());
+        ok = load_tga(f, t, why);
     } else {
         std::string magic;
         next_token(in, magic);
         if (magic == "P6") ok = load_ppm(in, t, why);
         else if (magic == "PF" || magic == "Pf") ok = load_pfm(in, magic == "PF", t, why);
-        else why = "unsupported image format (binary PPM P6 or PFM): " + path;
+        else why = "unsupported image format (TGA, binary PPM P6 or PFM): " + path;
     }
     if (!ok) {
         if (error) *error = why;

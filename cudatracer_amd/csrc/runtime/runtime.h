@@ -20,7 +20,14 @@ struct ChunkStats {
     int chunks = 0;
 };
 
-// One of the two pipelines the chunk driver alternates between.  Each lane
+#ifndef CT_LANES
+#define CT_LANES 2
+#endif
+// Pipelines the chunk driver rotates through.
+constexpr int kLanes = CT_LANES;
+static_assert(kLanes >= 2, "at least two lanes");
+
+// One of the pipelines the chunk driver rotates through.  Each lane
 // has its own queue views (a half of the context's path buffers), counters,
 // chunk tables, streams, events and captured bounce loop, so the chunks of
 // the two lanes run concurrently and fill each other's launch tails.
@@ -61,7 +68,7 @@ constexpr int kReadWords = kReadWork + 8;
 
 struct ct_ctx {
     int device = 0;
-    Lane ln[2];
+    Lane ln[kLanes];
     std::string err;
     std::string name;
     int num_cus = 0;
@@ -88,7 +95,7 @@ struct ct_ctx {
     void* image = nullptr;  // tonemapped image of ct_read_image[_rgba8], kept between calls
     size_t image_bytes = 0;
 
-    // path state: buffers of 2 * lane_cap slots, lane k uses [k * lane_cap, (k + 1) * lane_cap)
+    // path state: buffers of kLanes * lane_cap slots, lane k uses [k * lane_cap, (k + 1) * lane_cap)
     uint32_t lane_cap = 0;
     std::vector<void*> path_bufs;
     uint32_t spill_threads = 0;

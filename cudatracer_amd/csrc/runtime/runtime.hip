@@ -75,7 +75,7 @@ static void drop_lane_graph(Lane& L) {
     L.graph_key.clear();
 }
 
-// Callers have both lanes idle.
+// Callers have all lanes idle.
 void drop_graphs(ct_ctx* c) {
     for (Lane& L : c->ln) drop_lane_graph(L);
 }
@@ -137,7 +137,7 @@ int ensure_paths(ct_ctx* c, uint32_t cap) {
         return r;
     };
     // path buffers shared by the lanes (lane k at offset k * cap)
-    const size_t slots = 2 * (size_t)cap;
+    const size_t slots = kLanes * (size_t)cap;
     const size_t f4 = slots * sizeof(float4);
     ctd::Queues q{};
     int r = CT_OK;
@@ -152,7 +152,7 @@ int ensure_paths(ct_ctx* c, uint32_t cap) {
     if (r == CT_OK) r = A((void**)&q.sh_d, f4, "shadow directions");
     if (r == CT_OK) r = A((void**)&q.sh_c, f4, "shadow colours");
     if (r == CT_OK) r = A((void**)&q.lpath, f4, "path radiance");
-    for (int l = 0; l < 2 && r == CT_OK; ++l) {
+    for (int l = 0; l < kLanes && r == CT_OK; ++l) {
         Lane& L = c->ln[l];
         const size_t o = (size_t)l * cap;
         L.q = q;
@@ -468,7 +468,7 @@ static int finish_chunk(ct_ctx* c, Lane& L, ChunkStats* st) {
     return CT_OK;
 }
 
-// Paths of a workload below which it is not split over the two lanes.
+// Paths of a workload below which it is not split over the lanes.
 static constexpr uint64_t kSplitMin = 1u << 18;
 
 int run_paths(ct_ctx* c, const std::vector<ViewDesc>& views, const std::vector<Run>& runs, uint32_t cap,
@@ -476,9 +476,9 @@ int run_paths(ct_ctx* c, const std::vector<ViewDesc>& views, const std::vector<R
     uint64_t total = 0;
     for (const Run& run : runs) total += run.n;
     if (total >= kSplitMin && cap > 0) {
-        // an even number of equal chunks, so both lanes get the same work
+        // equal chunks, a multiple of the lane count, so every lane gets the same work
         uint64_t nch = (total + cap - 1) / cap;
-        nch = std::max<uint64_t>(2, nch + (nch & 1));
+        nch = std::max<uint64_t>(kLanes, (nch + kLanes - 1) / kLanes * kLanes);
         cap = (uint32_t)((total + nch - 1) / nch);
     }
     int r = ensure_paths(c, cap);
@@ -486,7 +486,7 @@ int run_paths(ct_ctx* c, const std::vector<ViewDesc>& views, const std::vector<R
     cap = std::min(cap, c->lane_cap);
     // work queued on lane 0 by the caller (accumulator clears) comes first
     HIPCHK(hipEventRecord(c->ln[0].resolved, c->ln[0].stream), "event");
-    HIPCHK(hipStreamWaitEvent(c->ln[1].stream, c->ln[0].resolved, 0), "order lanes");
+    for (int k = 1; k < kLanes; ++k) HIPCHK(hipStreamWaitEvent(c->ln[k].stream, c->ln[0].resolved, 0), "order lanes");
     hipEvent_t prev = nullptr;
     int lane = 0;
     auto flush = [&](std::vector<Segment>& segs, uint32_t fill, uint32_t tasks) -> int {
@@ -496,7 +496,7 @@ int run_paths(ct_ctx* c, const std::vector<ViewDesc>& views, const std::vector<R
         rr = launch_chunk(c, L, views, segs, fill, tasks, accum, o, timed, count, use_graph, prev);
         if (rr != CT_OK) return rr;
         prev = L.resolved;
-        lane ^= 1;
+        lane = (lane + 1) % kLanes;
         return CT_OK;
     };
     std::vector<Segment> segs;
@@ -530,12 +530,12 @@ int run_paths(ct_ctx* c, const std::vector<ViewDesc>& views, const std::vector<R
         r = flush(segs, fill, tasks);
         if (r != CT_OK) { sync_lanes(c); return r; }
     }
-    for (int k = 0; k < 2; ++k) {
+    for (int k = 0; k < kLanes; ++k) {
         r = finish_chunk(c, c->ln[lane], st);
-        lane ^= 1;
+        lane = (lane + 1) % kLanes;
         if (r != CT_OK) { sync_lanes(c); return r; }
     }
-    // lane 1's work is done: later work on lane 0 needs no further ordering
+    // the other lanes' work is done: later work on lane 0 needs no further ordering
     return CT_OK;
 }
 

@@ -385,7 +385,10 @@ int ct_host_scene_add_triangle(ct_host_scene* s, const float p[9], const float n
  * normalize != 0 first fits the model into [-1, 1] (centred, divided by half
  * its largest extent).  Polygons become triangle fans with reversed winding,
  * identical corners are shared, missing normals are generated smooth.
- * material < 0 uses material 0: MTL materials and textures are not read.
+ * material < 0: the faces take their MTL materials (mtllib / usemtl), made
+ * as TriangleList::processMesh makes them (TriangleList.cpp:227-345) with
+ * the OBJ importer's defaults (diffuse 0.6) for faces without one; map_Kd,
+ * norm, map_d and map_Disp textures load through ct_host_scene_load_texture.
  * Triangles of a Light material become lights.  CT_ERR_INVALID_ARG if the
  * file cannot be opened, CT_ERR_INVALID_SCENE if it does not parse;
  * ct_host_scene_last_error gives the reason. */

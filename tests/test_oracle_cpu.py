@@ -640,7 +640,10 @@ def test_obj_import_normalize_and_smooth_normals(ct, tmp_path):
     s.load_model(f, normalize=True)
     s.build()
     V, N, I, _, M, L = _host_list(ct, s)
-    assert I.shape == (8, 3) and V.shape == (6, 3) and L.size == 0 and np.all(M == 0)
+    # without an MTL library the faces get the importer's default material (Lambert 0.6)
+    assert I.shape == (8, 3) and V.shape == (6, 3) and L.size == 0 and np.all(M == 1)
+    m1 = s.arrays()["materials"][1]
+    assert m1.type == ct.MAT_LAMBERTIAN and np.allclose(m1.albedo[:], 0.6)
     assert np.allclose(V.min(0), (-1, -1, -1)) and np.allclose(V.max(0), (1, 1, 1))
     c = np.array([0.0, 1.0, 1.0])  # centre of the original box, in original units
     expect = (np.array(v, float) - c) / 3.0

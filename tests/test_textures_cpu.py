@@ -1,7 +1,7 @@
 """CPU tests of the texture path: the shared atan2 against
 libm, the sky mapping and nearest-texel addressing of SceneData::getTextureColor
 against their closed forms (SceneData.cuh:81-97, Kernel.cu:96-100), the
-PPM/PFM loaders' texel order, and render invariants in the oracle that any
+TGA/PPM/PFM loaders' texel order, OBJ + MTL materials, and render invariants in the oracle that any
 slip in the shading integration breaks (a texture of the albedo's colour, a
 flat normal map and a 1x1 sky texture must leave the image bit-identical;
 a non-trivial map must change it)."""
@@ -238,3 +238,112 @@ def test_textured_config_scene(ct, orc):
     assert sum(m.diffuse_tex >= 0 for m in mats) >= 3 and sum(m.normal_tex >= 0 for m in mats) == 2
     assert a4["texcoords"].shape == (a4["vertices"].shape[0], 2)
     assert all(m.diffuse_tex == -1 and m.normal_tex == -1 for m in a2["materials"])
+
+
+def _tga(path, img, rle=False, top_left=False, alpha=False):
+    """Write an 8-bit TGA; img rows top-down (as seen), RGB(A) uint8."""
+    h, w = img.shape[:2]
+    px = img[..., [2, 1, 0, 3]] if alpha else img[..., [2, 1, 0]]
+    rows = px if top_left else px[::-1]  # bottom-up unless the top-left flag is set
+    bpp = 32 if alpha else 24
+    desc = (0x20 if top_left else 0) | (8 if alpha else 0)
+    hdr = struct.pack("<BBBHHBHHHHBB", 0, 0, 10 if rle else 2, 0, 0, 0, 0, 0, w, h, bpp, desc)
+    data = rows.reshape(-1, px.shape[-1])
+    if not rle:
+        body = data.tobytes()
+    else:  # runs of equal pixels and raw packets, at most 128 each
+        out = bytearray()
+        i, n = 0, len(data)
+        while i < n:
+            j = i + 1
+            while j < n and j - i < 128 and np.array_equal(data[j], data[i]):
+                j += 1
+            if j - i > 1:
+                out += bytes([0x80 | (j - i - 1)]) + data[i].tobytes()
+                i = j
+            else:
+                k = i + 1
+                while k < n and k - i < 128 and not (k + 1 < n and np.array_equal(data[k], data[k + 1])):
+                    k += 1
+                out += bytes([k - i - 1]) + data[i:k].tobytes()
+                i = k
+        body = bytes(out)
+    path.write_bytes(hdr + body)
+
+
+def test_tga_texel_order(ct, tmp_path):
+    """TGA as FreeImage loads it: rows bottom-up, B G R(A) bytes, colour / 255,
+    alpha 1 (Surface.cpp:77-92 keeps no alpha for 8-bit images)."""
+    rng = np.random.default_rng(8)
+    img = rng.integers(0, 256, (6, 9, 4), dtype=np.uint8)
+    img[2:4, :, :] = img[2, 0]  # runs for the RLE packets
+    s = ct.HostScene.empty()
+    paths = []
+    for i, kw in enumerate([{}, {"rle": True}, {"top_left": True}, {"alpha": True}, {"rle": True, "alpha": True}]):
+        p = tmp_path / f"t{i}.tga"
+        _tga(p, img if kw.get("alpha") else img[..., :3], **kw)
+        paths.append(s.load_texture(p))
+    s.add_triangle(np.eye(3), np.tile([0, 0, 1.0], 3), 0)
+    t = s.arrays()["textures"]
+    want = np.concatenate([img[::-1, :, :3].astype(np.float32) / np.float32(255.0), np.ones((6, 9, 1), np.float32)], 2)
+    for k in paths:
+        assert np.array_equal(t[k], want), k
+
+
+@pytest.mark.skipif(not __import__("pathlib").Path("/root/reference/Models/sponza/textures/chain_texture.tga").exists(),
+                    reason="the reference's texture files are only present where the reference is mounted")
+def test_tga_reference_textures(ct):
+    """The reference's own Sponza textures (24- and 32-bit, bottom-up): the
+    texels are the file's pixels in file row order."""
+    d = "/root/reference/Models/sponza/textures/"
+    s = ct.HostScene.empty()
+    names = ["chain_texture.tga", "sponza_thorn_ddn.tga"]
+    idx = [s.load_texture(d + n) for n in names]
+    s.add_triangle(np.eye(3), np.tile([0, 0, 1.0], 3), 0)
+    t = s.arrays()["textures"]
+    for i, n in zip(idx, names):
+        raw = open(d + n, "rb").read()
+        w, h = struct.unpack("<HH", raw[12:16])
+        bpp = raw[16] // 8
+        px = np.frombuffer(raw[18 + raw[0]:18 + raw[0] + w * h * bpp], np.uint8).reshape(h, w, bpp)
+        assert t[i].shape == (h, w, 4)
+        assert np.array_equal(t[i][..., 0], px[..., 2].astype(np.float32) / np.float32(255.0))
+        assert np.array_equal(t[i][..., 2], px[..., 0].astype(np.float32) / np.float32(255.0))
+
+
+def test_obj_mtl_materials(ct, tmp_path):
+    """OBJ + MTL: each usemtl group gets TriangleList::processMesh's material
+    (TriangleList.cpp:227-345); faces before any usemtl get the importer's
+    default; emissive faces become lights; map_Kd loads the texture."""
+    img = np.full((2, 2, 3), 200, np.uint8)
+    _tga(tmp_path / "wood.tga", img)
+    (tmp_path / "m.mtl").write_text(
+        "newmtl diffuse\nKd 0.5 0.4 0.3\nKs 0.1 0.1 0.1\n"
+        "newmtl shiny\nKd 0.2 0.2 0.2\nKs 0.9 0.8 0.7\nNs 50\n"
+        "newmtl lamp\nKe 4 4 3\nKd 0 0 0\n"
+        "newmtl textured\nmap_Kd -bm 1.0 wood.tga\nNs 1.5\n"
+        "newmtl mirror\nKd 0 0 0\nKs 0.95 0.95 0.95\n"
+        "newmtl glass\nKd 0 0 0\nKs 0 0 0\nTf 0.9 0.95 1\nNi 1.45\n"
+        "newmtl broken\nmap_Kd missing.tga\nKd 0.3 0.3 0.3\n")
+    faces = ["f 1 2 3"]
+    for name in ["diffuse", "shiny", "lamp", "textured", "mirror", "glass", "broken", "unknown"]:
+        faces += [f"usemtl {name}", "f 1 2 3"]
+    (tmp_path / "m.obj").write_text("mtllib m.mtl\nv 0 0 0\nv 1 0 0\nv 0 1 0\nvt 0 0\n" + "\n".join(faces) + "\n")
+    s = ct.HostScene.empty()
+    s.load_model(tmp_path / "m.obj", normalize=False)
+    a = s.arrays()
+    mats, mi = a["materials"], a["material_idx"]
+    m = [mats[int(k)] for k in mi]
+    assert len(mi) == 9
+    assert m[0].type == ct.MAT_LAMBERTIAN and np.allclose(m[0].albedo[:], 0.6)       # default
+    assert m[1].type == ct.MAT_LAMBERTIAN and np.allclose(m[1].albedo[:], (0.5, 0.4, 0.3))
+    assert m[2].type == ct.MAT_GGX and np.allclose(m[2].albedo[:], (0.9, 0.8, 0.7))  # Ks brighter than Kd
+    assert m[2].roughness == 1.0  # addMaterial forces roughness 1
+    assert m[3].type == ct.MAT_LIGHT and np.allclose(m[3].albedo[:], (4, 4, 3))
+    assert list(a["light_indices"]) == [3]
+    assert m[4].type == ct.MAT_LAMBERTIAN and m[4].diffuse_tex == 0 and np.allclose(m[4].albedo[:], 1.0)
+    assert np.allclose(a["textures"][0][..., :3], 200 / 255.0)
+    assert m[5].type == ct.MAT_SPECULAR and np.allclose(m[5].albedo[:], 0.95)
+    assert m[6].type == ct.MAT_FRESNEL and abs(m[6].refract_idx - 1.45) < 1e-6
+    assert m[7].diffuse_tex == -1 and np.allclose(m[7].albedo[:], 0.3)              # texture failed to load
+    assert int(mi[8]) == int(mi[0])                                                   # unknown name: default
