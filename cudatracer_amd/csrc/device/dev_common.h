@@ -66,7 +66,12 @@ struct DevScene {
     int sky_enabled;
     int sky_tex;            // -1: the constant sky colour
     float sky_r, sky_g, sky_b;
+    uint32_t plane_flags;   // kPlaneNegZero | kPlaneNaN over the quad tree's box planes
 };
+
+// Properties of the quad tree's box planes, found at upload (k_plane_flags).
+constexpr uint32_t kPlaneNegZero = 1u;  // some plane is -0.0
+constexpr uint32_t kPlaneNaN = 2u;      // some plane is NaN
 
 // One camera view (plain frame, batch member or served request).
 struct ViewDesc {

@@ -113,13 +113,16 @@ constexpr int kBufferWord3 = 0x00020000;
 
 CT_DEV LaneStack make_stack(const Queues& q, int lds_depth) {
     LaneStack st;
+    // the block size comes from a 16-bit vector load of the dispatch packet:
+    // made scalar here, so the stride and the spill resource stay in SGPRs
+    const uint32_t bdim = __builtin_amdgcn_readfirstlane(blockDim.x);
     st.base = threadIdx.x;
-    st.stride = blockDim.x;
+    st.stride = bdim;
     st.depth = lds_depth;
     const uint32_t per_thread = (uint32_t)(kMaxStack - lds_depth);
-    const uint32_t threads = gridDim.x * blockDim.x;
+    const uint32_t threads = gridDim.x * bdim;
     st.spill = __builtin_amdgcn_make_buffer_rsrc(q.spill, 0, (int)(threads * per_thread * 4u), kBufferWord3);
-    st.spill_off = (blockIdx.x * blockDim.x + threadIdx.x) * per_thread * 4u;
+    st.spill_off = (blockIdx.x * bdim + threadIdx.x) * per_thread * 4u;
     st.sp = 0;
     st.overflow = false;
     return st;
@@ -669,6 +672,29 @@ hipError_t launch_tonemap8(const float4* accum, uint32_t n, float samples, uint3
     const uint32_t blocks = std::min<uint32_t>((n + 255) / 256, 65536u);
     if (blocks == 0) return hipSuccess;
     hipLaunchKernelGGL(k_tonemap8, dim3(blocks), dim3(256), 0, s, accum, n, samples, out);
+    return hipGetLastError();
+}
+// kPlaneNegZero / kPlaneNaN over the six plane rows of every quad node
+// (out cleared by the caller); one atomic per wave that found something.
+__global__ void k_plane_flags(const float4* __restrict__ nodes4, uint32_t n4, uint32_t* out) {
+    const uint32_t rows = 6u * n4;
+    uint32_t f = 0;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < rows; i += gridDim.x * blockDim.x) {
+        const float4 v = nodes4[8 * (size_t)(i / 6u) + i % 6u];
+        const float c[4] = {v.x, v.y, v.z, v.w};
+        for (int k = 0; k < 4; ++k) {
+            if (__float_as_uint(c[k]) == 0x80000000u) f |= ctd::kPlaneNegZero;
+            if (isnan(c[k])) f |= ctd::kPlaneNaN;
+        }
+    }
+    const uint32_t w = (__ballot(f & ctd::kPlaneNegZero) ? ctd::kPlaneNegZero : 0u) |
+                       (__ballot(f & ctd::kPlaneNaN) ? ctd::kPlaneNaN : 0u);
+    if (__lane_id() == 0 && w) atomicOr(out, w);
+}
+hipError_t launch_plane_flags(const float4* nodes4, uint32_t n4, uint32_t* out, hipStream_t s) {
+    const uint32_t blocks = std::min<uint32_t>((6u * n4 + 255) / 256, 4096u);
+    if (blocks == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_plane_flags, dim3(blocks), dim3(256), 0, s, nodes4, n4, out);
     return hipGetLastError();
 }
 hipError_t launch_tonemap(const float4* accum, uint32_t n, float samples, float4* out, hipStream_t s) {

@@ -177,6 +177,112 @@ CT_DEV void node4_step(const float4* __restrict__ nodes4, uint32_t e, f3 o, f3 i
     push_children<SORT>(hm, m0, m1, m2, m3, e0, e1, e2, e3, st);
 }
 
+// Fast quad node test: the same tagged entry distances, hit flags and push
+// order as node4_step, from the hardware min/max instructions (one v_min or
+// v_max per slab pair, v_min3/v_max3 across axes) instead of the
+// compare-and-select that (y < x) ? y : x costs.  That is exact under
+// conditions that fast_ray() checks once per ray and this function once per
+// node:
+//  - inv is finite with |inv| > 0.5 and o is finite, so no slab value
+//    (plane - o) * inv is NaN (planes are finite, see plane_flags), and a
+//    slab value is zero only when plane - o is: a non-zero difference times
+//    |inv| > 0.5 stays above half the smallest denormal;
+//  - plane - o is never -0 (o has no +0 component, or no plane is -0), so
+//    every zero slab value is +0 * inv, of the sign of inv on its axis;
+//  - inside a slab pair (one axis) equal values therefore have equal bits,
+//    and v_min / v_max return what (y < x) ? y : x and (x < y) ? y : x
+//    return;
+//  - across axes only zeros of opposite sign can tie.  For the exit distance
+//    that does not matter (a zero fails x > 0 in either sign); where an
+//    entry distance is zero (the origin on a face of the box) the four entry
+//    distances are recomputed with the reference's fold, which keeps the
+//    zero of the first axis (exact slab pairs, then (x < y) ? y : x);
+//  - no key is NaN, so the reference network's last comparator (2,3) never
+//    swaps: after (0,2) and (1,3), s2 = max(s0, s2) <= max(s1, s3) = s3.  The
+//    entries ride through the first four comparators instead of being looked
+//    up from the sorted keys' slot tags.
+constexpr uint32_t kNoEntry = 0xFFFFFFFFu;  // a culled child in the sort
+
+CT_DEV bool fast_ray(f3 o, f3 inv, uint32_t plane_flags) {
+    constexpr float kMax = 3.40282347e38f;
+    const bool inv_ok = fabsf(inv.x) > 0.5f && fabsf(inv.x) <= kMax && fabsf(inv.y) > 0.5f &&
+                        fabsf(inv.y) <= kMax && fabsf(inv.z) > 0.5f && fabsf(inv.z) <= kMax;
+    const bool o_ok = fabsf(o.x) <= kMax && fabsf(o.y) <= kMax && fabsf(o.z) <= kMax;
+    const bool z_ok = !(plane_flags & kPlaneNegZero) || (fbits(o.x) != 0u && fbits(o.y) != 0u && fbits(o.z) != 0u);
+    return !(plane_flags & kPlaneNaN) && inv_ok && o_ok && z_ok;
+}
+
+template <bool SORT>
+CT_DEV void node4_fast(const float4* __restrict__ nodes4, uint32_t e, f3 o, f3 inv, float t, LaneStack& st) {
+    const float4* n = nodes4 + 8 * (size_t)e;
+    const float4 bminx = n[0], bmaxx = n[1], bminy = n[2], bmaxy = n[3], bminz = n[4], bmaxz = n[5];
+    const float4 ch = n[6], cn = n[7];
+    float m0, m1, m2, m3, x0, x1, x2, x3;
+#define CT_SLABF(MN, MX, lo)                                                      \
+    {                                                                             \
+        const float ax = (bminx.lo - o.x) * inv.x, bx = (bmaxx.lo - o.x) * inv.x; \
+        const float ay = (bminy.lo - o.y) * inv.y, by = (bmaxy.lo - o.y) * inv.y; \
+        const float az = (bminz.lo - o.z) * inv.z, bz = (bmaxz.lo - o.z) * inv.z; \
+        MN = fmaxf(fmaxf(fminf(ax, bx), fminf(ay, by)), fminf(az, bz));           \
+        MX = fminf(fminf(fmaxf(ax, bx), fmaxf(ay, by)), fmaxf(az, bz));           \
+    }
+    CT_SLABF(m0, x0, x)
+    CT_SLABF(m1, x1, y)
+    CT_SLABF(m2, x2, z)
+    CT_SLABF(m3, x3, w)
+#undef CT_SLABF
+    // keep the exit distances in this block: sunk past the branch below they
+    // would hold all 24 slab values live (and be re-canonicalised there)
+    __asm__ volatile("" ::"v"(x0), "v"(x1), "v"(x2), "v"(x3));
+    if (fminf(fminf(fabsf(m0), fabsf(m1)), fminf(fabsf(m2), fabsf(m3))) == 0.0f) {
+        // rare: the planes are reloaded (an opaque index) one child at a time
+        // rather than held live beside everything else
+        uint32_t r = e;
+        __asm__ volatile("" : "+v"(r));
+        const float* p = reinterpret_cast<const float*>(nodes4 + 8 * (size_t)r);
+#pragma unroll 1
+        for (uint32_t i = 0; i < 4; ++i) {
+            const float v = gmax(gmax(fminf((p[i] - o.x) * inv.x, (p[4 + i] - o.x) * inv.x),
+                                      fminf((p[8 + i] - o.y) * inv.y, (p[12 + i] - o.y) * inv.y)),
+                                 fminf((p[16 + i] - o.z) * inv.z, (p[20 + i] - o.z) * inv.z));
+            m0 = i == 0 ? v : m0;
+            m1 = i == 1 ? v : m1;
+            m2 = i == 2 ? v : m2;
+            m3 = i == 3 ? v : m3;
+        }
+    }
+    m0 = tag(m0, 0u);
+    m1 = tag(m1, 1u);
+    m2 = tag(m2, 2u);
+    m3 = tag(m3, 3u);
+    const int c0 = (int)fbits(cn.x), c1 = (int)fbits(cn.y), c2 = (int)fbits(cn.z), c3 = (int)fbits(cn.w);
+    const bool h0 = x0 > 0.0f && m0 <= x0 && m0 < t && c0 != 0;
+    const bool h1 = x1 > 0.0f && m1 <= x1 && m1 < t && c1 != 0;
+    const bool h2 = x2 > 0.0f && m2 <= x2 && m2 < t && c2 != 0;
+    const bool h3 = x3 > 0.0f && m3 <= x3 && m3 < t && c3 != 0;
+    if (!(h0 || h1 || h2 || h3)) return;
+    const uint32_t e0 = c0 == -1 ? fbits(ch.x) : (kLeafBit | fbits(ch.x));
+    const uint32_t e1 = c1 == -1 ? fbits(ch.y) : (kLeafBit | fbits(ch.y));
+    const uint32_t e2 = c2 == -1 ? fbits(ch.z) : (kLeafBit | fbits(ch.z));
+    const uint32_t e3 = c3 == -1 ? fbits(ch.w) : (kLeafBit | fbits(ch.w));
+    if (!SORT) {
+        st.push4(e0, h0, e1, h1, e2, h2, e3, h3);
+        return;
+    }
+    // comparators (0,1) and (2,3) on keys and entries, then (0,2) and (1,3)
+    // on entries (their keys are not compared again)
+    uint32_t q0 = h0 ? e0 : kNoEntry, q1 = h1 ? e1 : kNoEntry, q2 = h2 ? e2 : kNoEntry, q3 = h3 ? e3 : kNoEntry;
+    const bool s01 = m0 > m1, s23 = m2 > m3;
+    const float a = s01 ? m1 : m0, b = s01 ? m0 : m1, c = s23 ? m3 : m2, d = s23 ? m2 : m3;
+    const uint32_t ea = s01 ? q1 : q0, eb = s01 ? q0 : q1, ec = s23 ? q3 : q2, ed = s23 ? q2 : q3;
+    const bool s02 = a > c, s13 = b > d;
+    q0 = s02 ? ec : ea;
+    q2 = s02 ? ea : ec;
+    q1 = s13 ? ed : eb;
+    q3 = s13 ? eb : ed;
+    st.push4(q3, q3 != kNoEntry, q2, q2 != kNoEntry, q1, q1 != kNoEntry, q0, q0 != kNoEntry);
+}
+
 // Wide compressed node (128 B, up to 8 children, layout in host/cnode.h).
 // The ray is moved into the node's grid once per node, t = q * (s / d) +
 // (p - o) / d, so each child plane costs one conversion and one fma (the
@@ -321,7 +427,7 @@ struct Fetcher {
 template <int ACCEL, bool AnyHit, bool COUNT, class Source, class Sink>
 CT_DEV void trace_persistent(const DevScene& sc, float eps, uint32_t n, uint32_t* fetch, LaneStack& st,
                              TravCount& cnt, uint32_t* err, Source src, Sink sink) {
-    bool alive = false, exhausted = false;
+    bool alive = false, exhausted = false, fast = false;
     uint32_t ray = 0, k = kNoLeaf;
     f3 o = mk3(0, 0, 0), d = mk3(0, 0, 0), inv = mk3(0, 0, 0);
     float t = 0.0f;
@@ -346,6 +452,7 @@ CT_DEV void trace_persistent(const DevScene& sc, float eps, uint32_t n, uint32_t
                     t = r.tmax;
                     prim = -1;
                     inv = mk3(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
+                    fast = ACCEL == 4 && fast_ray(o, inv, sc.plane_flags);
                     st.sp = 0;
                     st.overflow = false;
                     st.push4(0u, true, 0u, false, 0u, false, 0u, false);
@@ -367,8 +474,12 @@ CT_DEV void trace_persistent(const DevScene& sc, float eps, uint32_t n, uint32_t
                     k = e & ~kLeafBit;
                 } else {
                     if (COUNT) ++cnt.nodes;
-                    if (ACCEL == 4) node4_step<!AnyHit>(sc.nodes4, e, o, inv, t, st);
-                    else fnode8_step<!AnyHit>(sc.fnodes, e, o, inv, t, st);
+                    if (ACCEL == 4) {
+                        if (fast) node4_fast<!AnyHit>(sc.nodes4, e, o, inv, t, st);
+                        else node4_step<!AnyHit>(sc.nodes4, e, o, inv, t, st);
+                    } else {
+                        fnode8_step<!AnyHit>(sc.fnodes, e, o, inv, t, st);
+                    }
                 }
             } else {
                 const float4 lo = sc.nodes2[2 * (size_t)e];

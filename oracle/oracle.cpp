@@ -1125,6 +1125,18 @@ int or_scene_build(or_scene* s) {
     return 0;
 }
 
+int or_scene_set_trees(or_scene* s, uint32_t n_bvh, const ct_bvh_node* bvh, uint32_t n_mbvh, const ct_mbvh_node* mbvh,
+                       const uint32_t* prim_indices) {
+    if (!s || !prim_indices || (n_bvh && !bvh) || (n_mbvh && !mbvh)) return 1;
+    for (uint32_t i = 0; i < s->ntris(); ++i)
+        if (prim_indices[i] >= s->ntris()) return 1;
+    s->bvh.assign(bvh, bvh + n_bvh);
+    s->mbvh.assign(mbvh, mbvh + n_mbvh);
+    s->prims.assign(prim_indices, prim_indices + s->ntris());
+    s->built = true;
+    return 0;
+}
+
 int or_scene_tree_sizes(const or_scene* s, uint32_t* n_bvh, uint32_t* n_mbvh, uint32_t* n_prims) {
     if (!s) return 1;
     if (n_bvh) *n_bvh = (uint32_t)s->bvh.size();

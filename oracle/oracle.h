@@ -44,6 +44,11 @@ void or_sky_uv(float dx, float dy, float dz, float* u, float* v);
 /* Build the binary SAH tree by the reference's 30-plane rescan and collapse
  * it to the 4-wide tree.  Returns 0 on success. */
 int or_scene_build(or_scene* s);
+/* Use the given trees instead of building (the trees of an upload, which
+ * may have been edited; traversed as given).  prim_indices: n_triangles
+ * entries.  Returns 0, or 1 for a missing array or an index out of range. */
+int or_scene_set_trees(or_scene* s, uint32_t n_bvh, const ct_bvh_node* bvh, uint32_t n_mbvh, const ct_mbvh_node* mbvh,
+                       const uint32_t* prim_indices);
 /* Copy the trees out (sizes first with NULL buffers). */
 int or_scene_tree_sizes(const or_scene* s, uint32_t* n_bvh, uint32_t* n_mbvh, uint32_t* n_prims);
 int or_scene_tree_export(const or_scene* s, ct_bvh_node* bvh, ct_mbvh_node* mbvh, uint32_t* prim_indices);

@@ -63,6 +63,7 @@ _sig = {
     "or_scene_build": (C.c_int, [_P]),
     "or_scene_tree_sizes": (C.c_int, [_P, _u32p, _u32p, _u32p]),
     "or_scene_tree_export": (C.c_int, [_P, _P, _P, _u32p]),
+    "or_scene_set_trees": (C.c_int, [_P, C.c_uint32, _P, C.c_uint32, _P, _u32p]),
     "or_trace": (C.c_int, [_P, C.c_int, C.c_int, C.c_uint32, _fptr, _fptr, C.c_float, C.POINTER(C.c_double),
                            C.POINTER(C.c_int32), _u32p, C.c_int]),
     "or_trace_shadow": (C.c_int, [_P, C.c_int, C.c_int, C.c_uint32, _fptr, _fptr, _fptr, C.c_float,
@@ -152,6 +153,16 @@ class Scene:
         if lib.or_texel(self._h, texture, u, v, _fp(out)) != 0:
             raise ValueError("bad texture index")
         return out
+
+    def set_trees(self, bvh, mbvh, prim_indices) -> None:
+        """Traverse the given trees (an upload's, possibly edited) instead of
+        the built ones; bvh / mbvh as the bytes of the scene arrays."""
+        b = np.ascontiguousarray(bvh, np.uint8)
+        m = np.ascontiguousarray(mbvh, np.uint8)
+        pr = np.ascontiguousarray(prim_indices, np.uint32)
+        if lib.or_scene_set_trees(self._h, b.size // 32, b.ctypes.data_as(_P), m.size // 128, m.ctypes.data_as(_P),
+                                  _up(pr)) != 0:
+            raise ValueError("tree arrays do not fit the scene")
 
     def trees(self):
         nb, nm, npr = C.c_uint32(), C.c_uint32(), C.c_uint32()

@@ -10,6 +10,8 @@ import ctypes as C
 import numpy as np
 import pytest
 
+from conftest import ArrayView as _View
+
 pytestmark = pytest.mark.gpu
 
 
@@ -19,54 +21,6 @@ def _bits(a):
 
 def _relmse(x, ref):
     return float(np.mean((x - ref) ** 2 / (ref ** 2 + 1e-2)))
-
-
-class _View:
-    """A ct_scene_view over numpy copies of a scene's arrays, with textures."""
-
-    def __init__(self, ct, arrays, textures=(), mats=None, sky=-1):
-        self.keep = []
-
-        def f(a, dt=np.float32):
-            a = np.ascontiguousarray(a, dt)
-            self.keep.append(a)
-            return a
-
-        v = ct.SceneView()
-        vert = f(arrays["vertices"])
-        v.n_vertices = vert.shape[0]
-        v.vertices = vert.ctypes.data_as(C.POINTER(C.c_float))
-        v.normals = f(arrays["normals"]).ctypes.data_as(C.POINTER(C.c_float))
-        v.texcoords = f(arrays["texcoords"]).ctypes.data_as(C.POINTER(C.c_float))
-        idx = f(arrays["indices"], np.uint32)
-        v.n_triangles = idx.shape[0]
-        v.indices = idx.ctypes.data_as(C.POINTER(C.c_uint32))
-        v.center_normals = f(arrays["center_normals"]).ctypes.data_as(C.POINTER(C.c_float))
-        v.material_idx = f(arrays["material_idx"], np.uint32).ctypes.data_as(C.POINTER(C.c_uint32))
-        li = f(arrays["light_indices"], np.uint32)
-        v.n_lights = li.shape[0]
-        v.light_indices = li.ctypes.data_as(C.POINTER(C.c_uint32))
-        src = mats if mats is not None else arrays["materials"]
-        self.mats = (ct.Material * len(src))(*src)
-        v.n_materials = len(src)
-        v.materials = self.mats
-        v.prim_indices = f(arrays["prim_indices"], np.uint32).ctypes.data_as(C.POINTER(C.c_uint32))
-        bvh = f(arrays["bvh"], np.uint8)
-        v.n_bvh_nodes = bvh.size // 32
-        v.bvh_nodes = C.cast(bvh.ctypes.data, C.POINTER(ct.BVHNode))
-        mbvh = f(arrays["mbvh"], np.uint8)
-        v.n_mbvh_nodes = mbvh.size // 128
-        v.mbvh_nodes = C.cast(mbvh.ctypes.data, C.POINTER(ct.MBVHNode))
-        v.sky_enabled = arrays["sky_enabled"] or sky >= 0
-        v.sky[:] = list(arrays["sky"])
-        self.tex = [f(t) for t in textures]
-        self.tv = (ct.Texture * max(len(self.tex), 1))()
-        for i, t in enumerate(self.tex):
-            self.tv[i] = ct.Texture(t.shape[1], t.shape[0], t.ctypes.data_as(C.POINTER(C.c_float)))
-        v.n_textures = len(self.tex)
-        v.textures = self.tv
-        v.sky_texture = sky
-        self.view = v
 
 
 def _mats(ct, arrays):
