@@ -281,58 +281,88 @@ static ctd::ShadeOpts shade_opts(const ct_frame_opts& o) {
 }
 
 // The kernel sequence of one chunk on a lane, up to the resolve
-// (setGlobals -> generate -> depth x (extend, shade, connect)).  timed
-// records an event pair around every kernel.
-static hipError_t enqueue_bounces(ct_ctx* c, Lane& L, const ct_frame_opts& o, bool count, bool timed) {
-    const ctd::TraceOpts to = trace_opts(o);
-    const ctd::ShadeOpts so = shade_opts(o);
-    hipStream_t s = L.stream;
-    unsigned long long* ws = count ? L.wstats : nullptr;
-    int e = 0;
-    auto rec = [&]() {
-        if (timed) (void)hipEventRecord(L.ev[e++], s);
-    };
-    hipError_t r;
-    rec();
-    if ((r = ctk::launch_setup(L.q.counters, L.d_hdr, s)) != hipSuccess) return r;
-    if ((r = ctk::launch_generate(c->shape, L.d_views, L.d_segs, L.d_hdr, c->lane_cap, L.q, s)) != hipSuccess)
-        return r;
-    rec();
-    // connect(b) on the side stream overlaps extend(b + 1): both read queues
-    // shade(b) wrote, and only connect and shade add to a path's radiance, so
-    // shade(b + 1) and resolve wait for connect(b) (the additions keep their
-    // order).  Its traversal stacks spill to their own area.
-    const bool nee = o.shadow && c->sc.n_lights > 0;
-    ctd::Queues qside = L.q;
-    qside.spill = L.spill_side;
-    int sel = 0;
-    for (int b = 0; b < o.max_depth; ++b) {
+// (setGlobals -> generate -> depth x (extend, shade, connect)), in steps:
+// step 0 is the prologue, step 1 + b is bounce b, then finish().  Chunks of
+// several lanes are enqueued step by step in turn, so every lane starts at
+// once instead of after the host has enqueued the whole of the previous
+// lanes' chunks.  timed records an event pair around every kernel.
+struct BounceSeq {
+    ct_ctx* c = nullptr;
+    Lane* L = nullptr;
+    ct_frame_opts o{};
+    ctd::TraceOpts to{};
+    ctd::ShadeOpts so{};
+    unsigned long long* ws = nullptr;
+    bool timed = false, nee = false;
+    ctd::Queues qside{};
+    int e = 0, sel = 0;
+
+    BounceSeq() = default;
+    BounceSeq(ct_ctx* c_, Lane* L_, const ct_frame_opts& o_, bool count, bool timed_)
+        : c(c_), L(L_), o(o_), to(trace_opts(o_)), so(shade_opts(o_)), ws(count ? L_->wstats : nullptr),
+          timed(timed_), nee(o_.shadow && c_->sc.n_lights > 0), qside(L_->q) {
+        qside.spill = L_->spill_side;
+    }
+    int steps() const { return 1 + o.max_depth; }
+    void rec() {
+        if (timed) (void)hipEventRecord(L->ev[e++], L->stream);
+    }
+    hipError_t step(int k) {
+        hipStream_t s = L->stream;
+        hipError_t r;
+        if (k == 0) {
+            rec();
+            if ((r = ctk::launch_setup(L->q.counters, L->d_hdr, s)) != hipSuccess) return r;
+            if ((r = ctk::launch_generate(c->shape, L->d_views, L->d_segs, L->d_hdr, c->lane_cap, L->q, s)) !=
+                hipSuccess)
+                return r;
+            rec();
+            return hipSuccess;
+        }
+        // connect(b) on the side stream overlaps extend(b + 1): both read
+        // queues shade(b) wrote, and only connect and shade add to a path's
+        // radiance, so shade(b + 1) and resolve wait for connect(b) (the
+        // additions keep their order).  Its traversal stacks spill to their
+        // own area.
+        const int b = k - 1;
         rec();
-        if ((r = ctk::launch_extend(c->shape, c->sc, to, L.q, sel, b, ws, s)) != hipSuccess) return r;
+        if ((r = ctk::launch_extend(c->shape, c->sc, to, L->q, sel, b, ws, s)) != hipSuccess) return r;
         rec();
-        if (nee && b > 0 && (r = hipStreamWaitEvent(s, L.join[b - 1], 0)) != hipSuccess) return r;
+        if (nee && b > 0 && (r = hipStreamWaitEvent(s, L->join[b - 1], 0)) != hipSuccess) return r;
         rec();
-        if ((r = ctk::launch_shade(c->shape, c->sc, so, L.d_views, L.d_segs, L.d_hdr, L.q, sel, b, s)) != hipSuccess)
+        if ((r = ctk::launch_shade(c->shape, c->sc, so, L->d_views, L->d_segs, L->d_hdr, L->q, sel, b, s)) !=
+            hipSuccess)
             return r;
         rec();
         if (nee) {
-            if ((r = hipEventRecord(L.fork[b], s)) != hipSuccess) return r;
-            if ((r = hipStreamWaitEvent(L.side, L.fork[b], 0)) != hipSuccess) return r;
-            if (timed) (void)hipEventRecord(L.ev[e], L.side);
+            if ((r = hipEventRecord(L->fork[b], s)) != hipSuccess) return r;
+            if ((r = hipStreamWaitEvent(L->side, L->fork[b], 0)) != hipSuccess) return r;
+            if (timed) (void)hipEventRecord(L->ev[e], L->side);
             ++e;
-            if ((r = ctk::launch_connect(c->shape, c->sc, to, qside, b, ws, L.side)) != hipSuccess) return r;
-            if (timed) (void)hipEventRecord(L.ev[e], L.side);
+            if ((r = ctk::launch_connect(c->shape, c->sc, to, qside, b, ws, L->side)) != hipSuccess) return r;
+            if (timed) (void)hipEventRecord(L->ev[e], L->side);
             ++e;
-            if ((r = hipEventRecord(L.join[b], L.side)) != hipSuccess) return r;
+            if ((r = hipEventRecord(L->join[b], L->side)) != hipSuccess) return r;
         } else {
             rec();
             rec();
         }
         sel ^= 1;
+        return hipSuccess;
     }
-    if (nee && (r = hipStreamWaitEvent(s, L.join[o.max_depth - 1], 0)) != hipSuccess) return r;
-    L.n_events = e;
-    return hipSuccess;
+    hipError_t finish() {
+        hipError_t r;
+        if (nee && (r = hipStreamWaitEvent(L->stream, L->join[o.max_depth - 1], 0)) != hipSuccess) return r;
+        L->n_events = e;
+        return hipSuccess;
+    }
+};
+
+static hipError_t enqueue_bounces(ct_ctx* c, Lane& L, const ct_frame_opts& o, bool count, bool timed) {
+    BounceSeq q(c, &L, o, count, timed);
+    hipError_t r = hipSuccess;
+    for (int k = 0; k < q.steps() && r == hipSuccess; ++k) r = q.step(k);
+    return r == hipSuccess ? q.finish() : r;
 }
 
 static std::string graph_key(const ct_ctx* c, const Lane& L, const ct_frame_opts& o) {
@@ -349,12 +379,12 @@ static std::string graph_key(const ct_ctx* c, const Lane& L, const ct_frame_opts
 static constexpr int kEvTotal0 = 2 * (3 * ctd::kMaxBounce + 8) - 2;  // chunk start / end events
 static constexpr int kEvResolve = 2 * (3 * ctd::kMaxBounce + 8) - 4;  // resolve pair (timed)
 
-// Queue one chunk on lane L: upload its tables, the bounce loop (replayed from
-// the lane's captured graph when use_graph), then the resolve after the
-// previous chunk's resolve (prev_resolved, may be null), then the read-back.
-static int launch_chunk(ct_ctx* c, Lane& L, const std::vector<ViewDesc>& views, const std::vector<Segment>& segs,
-                        uint32_t n, uint32_t n_tasks, float4* accum, const ct_frame_opts& o, bool timed, bool count,
-                        bool use_graph, hipEvent_t prev_resolved) {
+// Chunk launch in three parts.  stage_chunk uploads the chunk's tables;
+// then either graph_chunk replays the lane's captured bounce loop or a
+// BounceSeq enqueues it; close_chunk queues the resolve after the previous
+// chunk's resolve (prev_resolved, may be null) and the read-back.
+static int stage_chunk(ct_ctx* c, Lane& L, const std::vector<ViewDesc>& views, const std::vector<Segment>& segs,
+                       uint32_t n, uint32_t n_tasks, bool count) {
     int r = ensure_tables(c, L, (uint32_t)views.size(), (uint32_t)segs.size());
     if (r != CT_OK) return r;
     hipStream_t s = L.stream;
@@ -382,27 +412,36 @@ static int launch_chunk(ct_ctx* c, Lane& L, const std::vector<ViewDesc>& views, 
     HIPCHK(hipMemcpyAsync(L.d_hdr, L.h_stage, sizeof(h), hipMemcpyHostToDevice, s), "upload chunk header");
     if (count) HIPCHK(hipMemsetAsync(L.wstats, 0, 32, s), "clear work counters");
     HIPCHK(hipEventRecord(L.ev[kEvTotal0], s), "event");
-    if (use_graph && !count && !timed) {
-        const std::string key = graph_key(c, L, o);
-        if (L.graph_key != key || !L.graph_exec) {
-            if (L.graph_exec) (void)hipGraphExecDestroy(L.graph_exec);
-            L.graph_exec = nullptr;
-            L.graph_key.clear();
-            hipGraph_t g = nullptr;
-            HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal), "begin capture");
-            const hipError_t ce = enqueue_bounces(c, L, o, false, false);
-            const hipError_t ee = hipStreamEndCapture(s, &g);
-            if (ce != hipSuccess) { if (g) (void)hipGraphDestroy(g); return hip_fail(c, ce, "capture bounce loop"); }
-            if (ee != hipSuccess) return hip_fail(c, ee, "end capture");
-            const hipError_t ie = hipGraphInstantiate(&L.graph_exec, g, nullptr, nullptr, 0);
-            (void)hipGraphDestroy(g);
-            if (ie != hipSuccess) { L.graph_exec = nullptr; return hip_fail(c, ie, "instantiate graph"); }
-            L.graph_key = key;
-        }
-        HIPCHK(hipGraphLaunch(L.graph_exec, s), "launch graph");
-    } else {
-        HIPCHK(enqueue_bounces(c, L, o, count, timed), "launch chunk");
+    return CT_OK;
+}
+
+// The bounce loop from the lane's graph, captured again when the options,
+// launch shape or buffers changed.
+static int graph_chunk(ct_ctx* c, Lane& L, const ct_frame_opts& o) {
+    hipStream_t s = L.stream;
+    const std::string key = graph_key(c, L, o);
+    if (L.graph_key != key || !L.graph_exec) {
+        if (L.graph_exec) (void)hipGraphExecDestroy(L.graph_exec);
+        L.graph_exec = nullptr;
+        L.graph_key.clear();
+        hipGraph_t g = nullptr;
+        HIPCHK(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal), "begin capture");
+        const hipError_t ce = enqueue_bounces(c, L, o, false, false);
+        const hipError_t ee = hipStreamEndCapture(s, &g);
+        if (ce != hipSuccess) { if (g) (void)hipGraphDestroy(g); return hip_fail(c, ce, "capture bounce loop"); }
+        if (ee != hipSuccess) return hip_fail(c, ee, "end capture");
+        const hipError_t ie = hipGraphInstantiate(&L.graph_exec, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        if (ie != hipSuccess) { L.graph_exec = nullptr; return hip_fail(c, ie, "instantiate graph"); }
+        L.graph_key = key;
     }
+    HIPCHK(hipGraphLaunch(L.graph_exec, s), "launch graph");
+    return CT_OK;
+}
+
+static int close_chunk(ct_ctx* c, Lane& L, uint32_t n, float4* accum, const ct_frame_opts& o, bool timed,
+                       bool count, hipEvent_t prev_resolved) {
+    hipStream_t s = L.stream;
     // accumulator updates in chunk order
     if (prev_resolved) HIPCHK(hipStreamWaitEvent(s, prev_resolved, 0), "order resolves");
     if (timed) HIPCHK(hipEventRecord(L.ev[kEvResolve], s), "event");
@@ -489,15 +528,48 @@ int run_paths(ct_ctx* c, const std::vector<ViewDesc>& views, const std::vector<R
     for (int k = 1; k < kLanes; ++k) HIPCHK(hipStreamWaitEvent(c->ln[k].stream, c->ln[0].resolved, 0), "order lanes");
     hipEvent_t prev = nullptr;
     int lane = 0;
-    auto flush = [&](std::vector<Segment>& segs, uint32_t fill, uint32_t tasks) -> int {
-        Lane& L = c->ln[lane];
-        int rr = finish_chunk(c, L, st);  // the lane's previous chunk
-        if (rr != CT_OK) return rr;
-        rr = launch_chunk(c, L, views, segs, fill, tasks, accum, o, timed, count, use_graph, prev);
-        if (rr != CT_OK) return rr;
-        prev = L.resolved;
-        lane = (lane + 1) % kLanes;
+    // The first kLanes chunks (one per lane) are launched together, their
+    // bounce loops enqueued step by step in turn, so no lane waits for the
+    // host to enqueue another lane's whole chunk.  Later chunks start alone,
+    // each as soon as its lane's previous chunk has finished.
+    int launched = 0;
+    struct Staged {
+        std::vector<Segment> segs;
+        uint32_t fill, tasks;
+    };
+    std::vector<Staged> group;
+    auto launch_group = [&]() -> int {
+        const int g = (int)group.size();
+        int rr;
+        for (int i = 0; i < g; ++i) {
+            Lane& L = c->ln[(lane + i) % kLanes];
+            if ((rr = finish_chunk(c, L, st)) != CT_OK) return rr;  // the lane's previous chunk
+            if ((rr = stage_chunk(c, L, views, group[i].segs, group[i].fill, group[i].tasks, count)) != CT_OK)
+                return rr;
+        }
+        if (use_graph && !count && !timed) {
+            for (int i = 0; i < g; ++i)
+                if ((rr = graph_chunk(c, c->ln[(lane + i) % kLanes], o)) != CT_OK) return rr;
+        } else {
+            BounceSeq seq[kLanes];
+            for (int i = 0; i < g; ++i) seq[i] = BounceSeq(c, &c->ln[(lane + i) % kLanes], o, count, timed);
+            for (int k = 0; k < seq[0].steps(); ++k)
+                for (int i = 0; i < g; ++i) HIPCHK(seq[i].step(k), "launch chunk");
+            for (int i = 0; i < g; ++i) HIPCHK(seq[i].finish(), "launch chunk");
+        }
+        for (int i = 0; i < g; ++i) {
+            Lane& L = c->ln[(lane + i) % kLanes];
+            if ((rr = close_chunk(c, L, group[i].fill, accum, o, timed, count, prev)) != CT_OK) return rr;
+            prev = L.resolved;
+        }
+        lane = (lane + g) % kLanes;
+        launched += g;
+        group.clear();
         return CT_OK;
+    };
+    auto flush = [&](std::vector<Segment>& chunk_segs, uint32_t chunk_fill, uint32_t chunk_tasks) -> int {
+        group.push_back(Staged{chunk_segs, chunk_fill, chunk_tasks});
+        return launched + (int)group.size() < kLanes ? CT_OK : launch_group();
     };
     std::vector<Segment> segs;
     uint32_t fill = 0, tasks = 0;
@@ -530,6 +602,7 @@ int run_paths(ct_ctx* c, const std::vector<ViewDesc>& views, const std::vector<R
         r = flush(segs, fill, tasks);
         if (r != CT_OK) { sync_lanes(c); return r; }
     }
+    if (!group.empty() && (r = launch_group()) != CT_OK) { sync_lanes(c); return r; }
     for (int k = 0; k < kLanes; ++k) {
         r = finish_chunk(c, c->ln[lane], st);
         lane = (lane + 1) % kLanes;
