@@ -66,12 +66,13 @@ struct DevScene {
     int sky_enabled;
     int sky_tex;            // -1: the constant sky colour
     float sky_r, sky_g, sky_b;
-    uint32_t plane_flags;   // kPlaneNegZero | kPlaneNaN over the quad tree's box planes
+    uint32_t plane_flags;   // kPlaneNegZero | kPlaneNaN | kEmptyEntry over the quad tree
 };
 
 // Properties of the quad tree's box planes, found at upload (k_plane_flags).
 constexpr uint32_t kPlaneNegZero = 1u;  // some plane is -0.0
 constexpr uint32_t kPlaneNaN = 2u;      // some plane is NaN
+constexpr uint32_t kEmptyEntry = 4u;    // some empty slot (count 0) has a child other than 0xFFFFFFFF
 
 // One camera view (plain frame, batch member or served request).
 struct ViewDesc {

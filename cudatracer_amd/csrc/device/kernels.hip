@@ -116,22 +116,23 @@ CT_DEV LaneStack make_stack(const Queues& q, int lds_depth) {
     // the block size comes from a 16-bit vector load of the dispatch packet:
     // made scalar here, so the stride and the spill resource stay in SGPRs
     const uint32_t bdim = __builtin_amdgcn_readfirstlane(blockDim.x);
-    st.base = threadIdx.x;
-    st.stride = bdim;
-    st.depth = lds_depth;
-    const uint32_t per_thread = (uint32_t)(kMaxStack - lds_depth);
-    const uint32_t threads = gridDim.x * bdim;
-    st.spill = __builtin_amdgcn_make_buffer_rsrc(q.spill, 0, (int)(threads * per_thread * 4u), kBufferWord3);
-    st.spill_off = (blockIdx.x * bdim + threadIdx.x) * per_thread * 4u;
-    st.sp = 0;
-    st.overflow = false;
+    st.stride4 = 4u * bdim;
+    st.lds_end = (uint32_t)lds_depth * st.stride4;
+    st.max_end = (uint32_t)kMaxStack * st.stride4;
+    // the block's spill region: positions [lds_depth, kMaxStack) of its threads
+    const uint32_t region = (uint32_t)(kMaxStack - lds_depth) * st.stride4;
+    char* base = reinterpret_cast<char*>(q.spill) + (size_t)blockIdx.x * region;
+    st.spill = __builtin_amdgcn_make_buffer_rsrc(base, 0, (int)region, kBufferWord3);
+    st.reset();
     return st;
 }
 
 // One kernel per tree and counting mode, so each gets its own register
-// budget (the wide tree's step needs more than the exact ones).
+// budget.  Eight waves per SIMD (64 VGPRs) is the occupancy the traversal
+// needs to hide its memory latency; asked for explicitly, the scheduler
+// reaches it without spills (the counting copies spill a register).
 template <int TREE, bool COUNT>
-__global__ void k_extend(DevScene sc, TraceOpts to, Queues q, int sel, int bounce, int lds_depth,
+__global__ void __attribute__((amdgpu_waves_per_eu(8))) k_extend(DevScene sc, TraceOpts to, Queues q, int sel, int bounce, int lds_depth,
                          unsigned long long* wstats) {
     LaneStack st = make_stack(q, lds_depth);
     TravCount tc{0u, 0u};
@@ -168,7 +169,7 @@ CT_DEV void connect_impl(const DevScene& sc, const TraceOpts& to, const Queues& 
 }
 
 template <int TREE, bool COUNT>
-__global__ void k_connect(DevScene sc, TraceOpts to, Queues q, int bounce, int lds_depth,
+__global__ void __attribute__((amdgpu_waves_per_eu(8))) k_connect(DevScene sc, TraceOpts to, Queues q, int bounce, int lds_depth,
                           unsigned long long* wstats) {
     LaneStack st = make_stack(q, lds_depth);
     TravCount tc{0u, 0u};
@@ -674,25 +675,35 @@ hipError_t launch_tonemap8(const float4* accum, uint32_t n, float samples, uint3
     hipLaunchKernelGGL(k_tonemap8, dim3(blocks), dim3(256), 0, s, accum, n, samples, out);
     return hipGetLastError();
 }
-// kPlaneNegZero / kPlaneNaN over the six plane rows of every quad node
-// (out cleared by the caller); one atomic per wave that found something.
+// kPlaneNegZero / kPlaneNaN over the six plane rows of every quad node and
+// kEmptyEntry over its slots (out cleared by the caller); one atomic per wave
+// that found something.
 __global__ void k_plane_flags(const float4* __restrict__ nodes4, uint32_t n4, uint32_t* out) {
-    const uint32_t rows = 6u * n4;
+    const uint32_t rows = 7u * n4;
     uint32_t f = 0;
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < rows; i += gridDim.x * blockDim.x) {
-        const float4 v = nodes4[8 * (size_t)(i / 6u) + i % 6u];
-        const float c[4] = {v.x, v.y, v.z, v.w};
-        for (int k = 0; k < 4; ++k) {
-            if (__float_as_uint(c[k]) == 0x80000000u) f |= ctd::kPlaneNegZero;
-            if (isnan(c[k])) f |= ctd::kPlaneNaN;
+        if (i < 6u * n4) {
+            const float4 v = nodes4[8 * (size_t)(i / 6u) + i % 6u];
+            const float c[4] = {v.x, v.y, v.z, v.w};
+            for (int k = 0; k < 4; ++k) {
+                if (__float_as_uint(c[k]) == 0x80000000u) f |= ctd::kPlaneNegZero;
+                if (isnan(c[k])) f |= ctd::kPlaneNaN;
+            }
+        } else {
+            const size_t nd = i - 6u * n4;
+            const float4 ch = nodes4[8 * nd + 6], cn = nodes4[8 * nd + 7];
+            const float a[4] = {ch.x, ch.y, ch.z, ch.w}, b[4] = {cn.x, cn.y, cn.z, cn.w};
+            for (int k = 0; k < 4; ++k)
+                if (__float_as_uint(b[k]) == 0u && __float_as_uint(a[k]) != 0xFFFFFFFFu) f |= ctd::kEmptyEntry;
         }
     }
-    const uint32_t w = (__ballot(f & ctd::kPlaneNegZero) ? ctd::kPlaneNegZero : 0u) |
-                       (__ballot(f & ctd::kPlaneNaN) ? ctd::kPlaneNaN : 0u);
+    uint32_t w = 0;
+    for (uint32_t bit : {ctd::kPlaneNegZero, ctd::kPlaneNaN, ctd::kEmptyEntry})
+        if (__ballot(f & bit)) w |= bit;
     if (__lane_id() == 0 && w) atomicOr(out, w);
 }
 hipError_t launch_plane_flags(const float4* nodes4, uint32_t n4, uint32_t* out, hipStream_t s) {
-    const uint32_t blocks = std::min<uint32_t>((6u * n4 + 255) / 256, 4096u);
+    const uint32_t blocks = std::min<uint32_t>((7u * n4 + 255) / 256, 4096u);
     if (blocks == 0) return hipSuccess;
     hipLaunchKernelGGL(k_plane_flags, dim3(blocks), dim3(256), 0, s, nodes4, n4, out);
     return hipGetLastError();

@@ -60,53 +60,66 @@ CT_DEV bool tri_test(f3 o, f3 d, float& t, f3 p0, f3 edge1, f3 edge2, float eps)
 
 CT_DEV float tag(float v, uint32_t slot) { return __uint_as_float((fbits(v) & 0xFFFFFFFCu) | slot); }
 
-// Per-lane stack: LDS entries [0, depth), global spill [depth, kMaxStack).
-// The spill area is addressed through a raw buffer resource (vector buffer
+// Per-lane stack: LDS positions [0, depth), global spill [depth, kMaxStack).
+// Both are laid out [position][thread]: position p of thread i is at byte
+// 4 * (p * blockDim + i) of the LDS stack or, spilled, at byte
+// 4 * ((p - depth) * blockDim + i) of the block's spill region.  The stack is
+// held as `top`, the LDS byte offset of the next free position, so a push or
+// a pop is an add and the depth checks compare with uniform limits.  The
+// spill region is addressed through a raw buffer resource (vector buffer
 // loads/stores, bounds-checked by the hardware), which also keeps the
 // compiler from folding LDS and spill reads into one flat access.
 struct LaneStack {
-    uint32_t base;      // threadIdx.x
-    uint32_t stride;    // blockDim.x
-    int depth;          // LDS entries per lane
-    __amdgpu_buffer_rsrc_t spill;  // whole spill area
-    uint32_t spill_off; // byte offset of this thread's spill slots
-    int sp;
+    uint32_t top;      // p * stride4 + 4 * threadIdx.x for the next free position p
+    uint32_t stride4;  // 4 * blockDim.x
+    uint32_t lds_end;  // depth * stride4: positions from here on are spilled
+    uint32_t max_end;  // kMaxStack * stride4
+    __amdgpu_buffer_rsrc_t spill;  // this block's spill region
     bool overflow;
 
-    CT_DEV void put(int pos, uint32_t v) {
-        if (pos < depth) ct_lds_stack[(uint32_t)pos * stride + base] = v;
-        else if (pos < kMaxStack)
-            __builtin_amdgcn_raw_buffer_store_b32(v, spill, spill_off + 4u * (uint32_t)(pos - depth), 0, 0);
-        else overflow = true;
+    CT_DEV static uint32_t& lds(uint32_t byte) {
+        return *reinterpret_cast<uint32_t*>(reinterpret_cast<char*>(ct_lds_stack) + byte);
+    }
+    CT_DEV void reset() {
+        top = 4u * threadIdx.x;
+        overflow = false;
+    }
+    CT_DEV bool empty() const { return top < stride4; }
+    // Store at byte a: LDS, spill region, or dropped beyond kMaxStack (push4
+    // flags the overflow).
+    CT_DEV void put(uint32_t a, uint32_t v) {
+        if (a < lds_end) lds(a) = v;
+        else if (a < max_end) __builtin_amdgcn_raw_buffer_store_b32(v, spill, a - lds_end, 0, 0);
     }
     CT_DEV uint32_t top_pop() {
-        --sp;
-        // LDS read at a clamped index (always in bounds), buffer read only for
-        // spilled entries
-        const int li = sp < depth ? sp : depth - 1;
-        uint32_t v = ct_lds_stack[(uint32_t)li * stride + base];
-        if (sp >= depth) v = __builtin_amdgcn_raw_buffer_load_b32(spill, spill_off + 4u * (uint32_t)(sp - depth), 0, 0);
+        top -= stride4;
+        // LDS read at a clamped offset (always in bounds), buffer read only
+        // for spilled entries
+        uint32_t v = lds(top < lds_end ? top : lds_end - 4u);
+        if (top >= lds_end) v = __builtin_amdgcn_raw_buffer_load_b32(spill, top - lds_end, 0, 0);
         return v;
     }
-    // Push v0..v3 where k0..k3 (in that order); branch-free when all four
-    // candidate positions are in LDS.
+    // Push v0..v3 where k0..k3 (in that order).  Every value is stored at the
+    // next free position; a culled one is overwritten by the next store or
+    // left above the new top.  Branch-free when all four positions are in LDS.
     CT_DEV void push4(uint32_t v0, bool k0, uint32_t v1, bool k1, uint32_t v2, bool k2, uint32_t v3, bool k3) {
-        if (sp + 4 <= depth) {
-            uint32_t p = (uint32_t)sp * stride + base;
-            ct_lds_stack[p] = v0;
-            p += k0 ? stride : 0u;
-            ct_lds_stack[p] = v1;
-            p += k1 ? stride : 0u;
-            ct_lds_stack[p] = v2;
-            p += k2 ? stride : 0u;
-            ct_lds_stack[p] = v3;
-            sp += (int)k0 + (int)k1 + (int)k2 + (int)k3;
+        const uint32_t a0 = top, a1 = a0 + (k0 ? stride4 : 0u), a2 = a1 + (k1 ? stride4 : 0u),
+                       a3 = a2 + (k2 ? stride4 : 0u);
+        top = a3 + (k3 ? stride4 : 0u);
+        if (a3 < lds_end) {
+            lds(a0) = v0;
+            lds(a1) = v1;
+            lds(a2) = v2;
+            lds(a3) = v3;
         } else {
-            if (k0) put(sp++, v0);
-            if (k1) put(sp++, v1);
-            if (k2) put(sp++, v2);
-            if (k3) put(sp++, v3);
-            if (sp > kMaxStack) sp = kMaxStack;
+            put(a0, v0);
+            put(a1, v1);
+            put(a2, v2);
+            put(a3, v3);
+            if (top >= max_end + stride4) {  // a position at or beyond kMaxStack was needed
+                overflow = true;
+                top = max_end + 4u * threadIdx.x;
+            }
         }
     }
 };
@@ -201,7 +214,10 @@ CT_DEV void node4_step(const float4* __restrict__ nodes4, uint32_t e, f3 o, f3 i
 //    swaps: after (0,2) and (1,3), s2 = max(s0, s2) <= max(s1, s3) = s3.  The
 //    entries ride through the first four comparators instead of being looked
 //    up from the sorted keys' slot tags.
-constexpr uint32_t kNoEntry = 0xFFFFFFFFu;  // a culled child in the sort
+// A culled child in the sort; also the entry of an empty slot, whose child
+// is 0xFFFFFFFF in the builders' trees (kLeafBit | 0xFFFFFFFF), which the
+// upload checks (kEmptyEntry) because the sorted path relies on it.
+constexpr uint32_t kNoEntry = 0xFFFFFFFFu;
 
 CT_DEV bool fast_ray(f3 o, f3 inv, uint32_t plane_flags) {
     constexpr float kMax = 3.40282347e38f;
@@ -209,7 +225,7 @@ CT_DEV bool fast_ray(f3 o, f3 inv, uint32_t plane_flags) {
                         fabsf(inv.y) <= kMax && fabsf(inv.z) > 0.5f && fabsf(inv.z) <= kMax;
     const bool o_ok = fabsf(o.x) <= kMax && fabsf(o.y) <= kMax && fabsf(o.z) <= kMax;
     const bool z_ok = !(plane_flags & kPlaneNegZero) || (fbits(o.x) != 0u && fbits(o.y) != 0u && fbits(o.z) != 0u);
-    return !(plane_flags & kPlaneNaN) && inv_ok && o_ok && z_ok;
+    return !(plane_flags & (kPlaneNaN | kEmptyEntry)) && inv_ok && o_ok && z_ok;
 }
 
 template <bool SORT>
@@ -256,10 +272,12 @@ CT_DEV void node4_fast(const float4* __restrict__ nodes4, uint32_t e, f3 o, f3 i
     m2 = tag(m2, 2u);
     m3 = tag(m3, 3u);
     const int c0 = (int)fbits(cn.x), c1 = (int)fbits(cn.y), c2 = (int)fbits(cn.z), c3 = (int)fbits(cn.w);
-    const bool h0 = x0 > 0.0f && m0 <= x0 && m0 < t && c0 != 0;
-    const bool h1 = x1 > 0.0f && m1 <= x1 && m1 < t && c1 != 0;
-    const bool h2 = x2 > 0.0f && m2 <= x2 && m2 < t && c2 != 0;
-    const bool h3 = x3 > 0.0f && m3 <= x3 && m3 < t && c3 != 0;
+    // sorted: no count test, an empty slot's entry is kNoEntry already (its
+    // child is 0xFFFFFFFF, no kEmptyEntry), so it is never pushed
+    const bool h0 = x0 > 0.0f && m0 <= x0 && m0 < t && (SORT || c0 != 0);
+    const bool h1 = x1 > 0.0f && m1 <= x1 && m1 < t && (SORT || c1 != 0);
+    const bool h2 = x2 > 0.0f && m2 <= x2 && m2 < t && (SORT || c2 != 0);
+    const bool h3 = x3 > 0.0f && m3 <= x3 && m3 < t && (SORT || c3 != 0);
     if (!(h0 || h1 || h2 || h3)) return;
     const uint32_t e0 = c0 == -1 ? fbits(ch.x) : (kLeafBit | fbits(ch.x));
     const uint32_t e1 = c1 == -1 ? fbits(ch.y) : (kLeafBit | fbits(ch.y));
@@ -453,8 +471,7 @@ CT_DEV void trace_persistent(const DevScene& sc, float eps, uint32_t n, uint32_t
                     prim = -1;
                     inv = mk3(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
                     fast = ACCEL == 4 && fast_ray(o, inv, sc.plane_flags);
-                    st.sp = 0;
-                    st.overflow = false;
+                    st.reset();
                     st.push4(0u, true, 0u, false, 0u, false, 0u, false);
                     k = kNoLeaf;
                     alive = true;
@@ -511,7 +528,7 @@ CT_DEV void trace_persistent(const DevScene& sc, float eps, uint32_t n, uint32_t
             done = true;
             prim = -2;
         }
-        if (done || (k == kNoLeaf && st.sp == 0)) {
+        if (done || (k == kNoLeaf && st.empty())) {
             sink(ray, t, prim);
             alive = false;
         }
