@@ -52,6 +52,7 @@ constexpr int kMaxStack = 128;  // >= 3*32+1 (quad tree of a depth-64 binary tre
 struct DevScene {
     const float4* tris;     // 3 per triangle (leaf order): {p0, id}, {p1-p0}, {p2-p0}
     const float4* nodes4;   // 8 per quad node: bminx,bmaxx,bminy,bmaxy,bminz,bmaxz,childs,counts
+    const float4* nodes4f;  // the same nodes for the fast node test: six plane rows, four stack entries, zeros
     const float4* nodes2;   // 2 per binary node: {min, leftFirst}, {max, count}
     const uint4* fnodes;    // 8 per wide compressed node (128 B), layout in host/cnode.h
     const float4* sh_pos;   // 3 per primitive: v0 (w = material index bits), v1, v2

@@ -675,37 +675,47 @@ hipError_t launch_tonemap8(const float4* accum, uint32_t n, float samples, uint3
     hipLaunchKernelGGL(k_tonemap8, dim3(blocks), dim3(256), 0, s, accum, n, samples, out);
     return hipGetLastError();
 }
-// kPlaneNegZero / kPlaneNaN over the six plane rows of every quad node and
-// kEmptyEntry over its slots (out cleared by the caller); one atomic per wave
-// that found something.
-__global__ void k_plane_flags(const float4* __restrict__ nodes4, uint32_t n4, uint32_t* out) {
-    const uint32_t rows = 7u * n4;
+// Per quad node: the fast node test's layout (the six plane rows, the four
+// stack entries kLeafBit | first triangle record or the child node, a zero
+// row) and the tree's kPlaneNegZero / kPlaneNaN / kEmptyEntry flags (flags
+// cleared by the caller; one atomic per wave that found something).
+__global__ void k_prepare_quad(const float4* __restrict__ nodes4, uint32_t n4, float4* __restrict__ fast,
+                               uint32_t* flags) {
     uint32_t f = 0;
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < rows; i += gridDim.x * blockDim.x) {
-        if (i < 6u * n4) {
-            const float4 v = nodes4[8 * (size_t)(i / 6u) + i % 6u];
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n4; i += gridDim.x * blockDim.x) {
+        const float4* n = nodes4 + 8 * (size_t)i;
+        float4* o = fast + 8 * (size_t)i;
+        for (int r = 0; r < 6; ++r) {
+            const float4 v = n[r];
+            o[r] = v;
             const float c[4] = {v.x, v.y, v.z, v.w};
             for (int k = 0; k < 4; ++k) {
                 if (__float_as_uint(c[k]) == 0x80000000u) f |= ctd::kPlaneNegZero;
                 if (isnan(c[k])) f |= ctd::kPlaneNaN;
             }
-        } else {
-            const size_t nd = i - 6u * n4;
-            const float4 ch = nodes4[8 * nd + 6], cn = nodes4[8 * nd + 7];
-            const float a[4] = {ch.x, ch.y, ch.z, ch.w}, b[4] = {cn.x, cn.y, cn.z, cn.w};
-            for (int k = 0; k < 4; ++k)
-                if (__float_as_uint(b[k]) == 0u && __float_as_uint(a[k]) != 0xFFFFFFFFu) f |= ctd::kEmptyEntry;
         }
+        const float4 ch = n[6], cn = n[7];
+        const uint32_t child[4] = {__float_as_uint(ch.x), __float_as_uint(ch.y), __float_as_uint(ch.z),
+                                   __float_as_uint(ch.w)};
+        const uint32_t count[4] = {__float_as_uint(cn.x), __float_as_uint(cn.y), __float_as_uint(cn.z),
+                                   __float_as_uint(cn.w)};
+        uint32_t e[4];
+        for (int k = 0; k < 4; ++k) {
+            e[k] = count[k] == 0xFFFFFFFFu ? child[k] : (ctd::kLeafBit | child[k]);
+            if (count[k] == 0u && child[k] != 0xFFFFFFFFu) f |= ctd::kEmptyEntry;
+        }
+        o[6] = make_float4(__uint_as_float(e[0]), __uint_as_float(e[1]), __uint_as_float(e[2]), __uint_as_float(e[3]));
+        o[7] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     }
     uint32_t w = 0;
     for (uint32_t bit : {ctd::kPlaneNegZero, ctd::kPlaneNaN, ctd::kEmptyEntry})
         if (__ballot(f & bit)) w |= bit;
-    if (__lane_id() == 0 && w) atomicOr(out, w);
+    if (__lane_id() == 0 && w) atomicOr(flags, w);
 }
-hipError_t launch_plane_flags(const float4* nodes4, uint32_t n4, uint32_t* out, hipStream_t s) {
-    const uint32_t blocks = std::min<uint32_t>((7u * n4 + 255) / 256, 4096u);
+hipError_t launch_prepare_quad(const float4* nodes4, uint32_t n4, float4* fast, uint32_t* flags, hipStream_t s) {
+    const uint32_t blocks = std::min<uint32_t>((n4 + 255) / 256, 4096u);
     if (blocks == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_plane_flags, dim3(blocks), dim3(256), 0, s, nodes4, n4, out);
+    hipLaunchKernelGGL(k_prepare_quad, dim3(blocks), dim3(256), 0, s, nodes4, n4, fast, flags);
     return hipGetLastError();
 }
 hipError_t launch_tonemap(const float4* accum, uint32_t n, float samples, float4* out, hipStream_t s) {

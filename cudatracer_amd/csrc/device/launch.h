@@ -38,8 +38,9 @@ hipError_t launch_resolve(const LaunchShape& ls, const ctd::ViewDesc* views, con
                           hipStream_t s);
 hipError_t launch_tonemap(const float4* accum, uint32_t n, float samples, float4* out, hipStream_t s);
 hipError_t launch_tonemap8(const float4* accum, uint32_t n, float samples, uint32_t* out, hipStream_t s);
-// Plane properties of a quad tree for the fast node test (*out cleared first).
-hipError_t launch_plane_flags(const float4* nodes4, uint32_t n4, uint32_t* out, hipStream_t s);
+// The fast node test's copy of a quad tree (8 float4 per node) and the
+// tree's plane / slot flags (*flags cleared first).
+hipError_t launch_prepare_quad(const float4* nodes4, uint32_t n4, float4* fast, uint32_t* flags, hipStream_t s);
 hipError_t launch_shadow_flags(const LaunchShape& ls, const ctd::DevScene& sc, const ctd::TraceOpts& to,
                                const ctd::Queues& q, uint32_t n, uint8_t* out, hipStream_t s);
 

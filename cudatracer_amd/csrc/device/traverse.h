@@ -228,11 +228,13 @@ CT_DEV bool fast_ray(f3 o, f3 inv, uint32_t plane_flags) {
     return !(plane_flags & (kPlaneNaN | kEmptyEntry)) && inv_ok && o_ok && z_ok;
 }
 
+// nodes4f: the tree in the fast layout (k_prepare_quad): six plane rows and
+// the four stack entries, seven loads per node.
 template <bool SORT>
-CT_DEV void node4_fast(const float4* __restrict__ nodes4, uint32_t e, f3 o, f3 inv, float t, LaneStack& st) {
-    const float4* n = nodes4 + 8 * (size_t)e;
+CT_DEV void node4_fast(const float4* __restrict__ nodes4f, uint32_t e, f3 o, f3 inv, float t, LaneStack& st) {
+    const float4* n = nodes4f + 8 * (size_t)e;
     const float4 bminx = n[0], bmaxx = n[1], bminy = n[2], bmaxy = n[3], bminz = n[4], bmaxz = n[5];
-    const float4 ch = n[6], cn = n[7];
+    const float4 ent = n[6];
     float m0, m1, m2, m3, x0, x1, x2, x3;
 #define CT_SLABF(MN, MX, lo)                                                      \
     {                                                                             \
@@ -255,7 +257,7 @@ CT_DEV void node4_fast(const float4* __restrict__ nodes4, uint32_t e, f3 o, f3 i
         // rather than held live beside everything else
         uint32_t r = e;
         __asm__ volatile("" : "+v"(r));
-        const float* p = reinterpret_cast<const float*>(nodes4 + 8 * (size_t)r);
+        const float* p = reinterpret_cast<const float*>(nodes4f + 8 * (size_t)r);
 #pragma unroll 1
         for (uint32_t i = 0; i < 4; ++i) {
             const float v = gmax(gmax(fminf((p[i] - o.x) * inv.x, (p[4 + i] - o.x) * inv.x),
@@ -271,18 +273,15 @@ CT_DEV void node4_fast(const float4* __restrict__ nodes4, uint32_t e, f3 o, f3 i
     m1 = tag(m1, 1u);
     m2 = tag(m2, 2u);
     m3 = tag(m3, 3u);
-    const int c0 = (int)fbits(cn.x), c1 = (int)fbits(cn.y), c2 = (int)fbits(cn.z), c3 = (int)fbits(cn.w);
-    // sorted: no count test, an empty slot's entry is kNoEntry already (its
-    // child is 0xFFFFFFFF, no kEmptyEntry), so it is never pushed
-    const bool h0 = x0 > 0.0f && m0 <= x0 && m0 < t && (SORT || c0 != 0);
-    const bool h1 = x1 > 0.0f && m1 <= x1 && m1 < t && (SORT || c1 != 0);
-    const bool h2 = x2 > 0.0f && m2 <= x2 && m2 < t && (SORT || c2 != 0);
-    const bool h3 = x3 > 0.0f && m3 <= x3 && m3 < t && (SORT || c3 != 0);
+    // An empty slot's entry is kNoEntry (its child is 0xFFFFFFFF: no
+    // kEmptyEntry), so the count test is the entry test: the sorted path
+    // carries kNoEntry through the network and never pushes it.
+    const uint32_t e0 = fbits(ent.x), e1 = fbits(ent.y), e2 = fbits(ent.z), e3 = fbits(ent.w);
+    const bool h0 = x0 > 0.0f && m0 <= x0 && m0 < t && (SORT || e0 != kNoEntry);
+    const bool h1 = x1 > 0.0f && m1 <= x1 && m1 < t && (SORT || e1 != kNoEntry);
+    const bool h2 = x2 > 0.0f && m2 <= x2 && m2 < t && (SORT || e2 != kNoEntry);
+    const bool h3 = x3 > 0.0f && m3 <= x3 && m3 < t && (SORT || e3 != kNoEntry);
     if (!(h0 || h1 || h2 || h3)) return;
-    const uint32_t e0 = c0 == -1 ? fbits(ch.x) : (kLeafBit | fbits(ch.x));
-    const uint32_t e1 = c1 == -1 ? fbits(ch.y) : (kLeafBit | fbits(ch.y));
-    const uint32_t e2 = c2 == -1 ? fbits(ch.z) : (kLeafBit | fbits(ch.z));
-    const uint32_t e3 = c3 == -1 ? fbits(ch.w) : (kLeafBit | fbits(ch.w));
     if (!SORT) {
         st.push4(e0, h0, e1, h1, e2, h2, e3, h3);
         return;
@@ -492,7 +491,7 @@ CT_DEV void trace_persistent(const DevScene& sc, float eps, uint32_t n, uint32_t
                 } else {
                     if (COUNT) ++cnt.nodes;
                     if (ACCEL == 4) {
-                        if (fast) node4_fast<!AnyHit>(sc.nodes4, e, o, inv, t, st);
+                        if (fast) node4_fast<!AnyHit>(sc.nodes4f, e, o, inv, t, st);
                         else node4_step<!AnyHit>(sc.nodes4, e, o, inv, t, st);
                     } else {
                         fnode8_step<!AnyHit>(sc.fnodes, e, o, inv, t, st);

@@ -288,22 +288,29 @@ int upload_shading(ct_ctx* c, const Shading& sh, ctd::DevScene& sc) {
     return r;
 }
 
-// sc.plane_flags of the uploaded quad tree (the traversal's fast node test
-// needs planes that are not NaN, and origins without +0 components only
-// where some plane is -0).
-int scan_planes(ct_ctx* c, ctd::DevScene& sc) {
+// The fast node test's copy of the uploaded quad tree (sc.nodes4f) and the
+// tree's flags (sc.plane_flags): the fast test needs planes that are not NaN,
+// empty slots whose child is 0xFFFFFFFF, and origins without +0 components
+// only where some plane is -0.
+int prepare_quad(ct_ctx* c, ctd::DevScene& sc) {
     sc.plane_flags = 0;
+    sc.nodes4f = nullptr;
     if (!sc.nodes4 || sc.n_nodes4 == 0) return CT_OK;
+    void* fast = nullptr;
+    int r = dalloc(c, &fast, (size_t)sc.n_nodes4 * sizeof(ct_mbvh_node), "fast quad nodes");
+    if (r != CT_OK) return r;
+    c->scene_bufs.push_back(fast);
     uint32_t* d = nullptr;
-    HIPCHK(hipMalloc(&d, sizeof(uint32_t)), "plane scan");
+    HIPCHK(hipMalloc(&d, sizeof(uint32_t)), "quad tree flags");
     const hipStream_t s = c->ln[0].stream;
     uint32_t h = 0;
     hipError_t e = hipMemsetAsync(d, 0, sizeof(uint32_t), s);
-    if (e == hipSuccess) e = ctk::launch_plane_flags(sc.nodes4, sc.n_nodes4, d, s);
+    if (e == hipSuccess) e = ctk::launch_prepare_quad(sc.nodes4, sc.n_nodes4, reinterpret_cast<float4*>(fast), d, s);
     if (e == hipSuccess) e = hipMemcpyAsync(&h, d, sizeof(uint32_t), hipMemcpyDeviceToHost, s);
     if (e == hipSuccess) e = hipStreamSynchronize(s);
     (void)hipFree(d);
-    if (e != hipSuccess) return ctr::hip_fail(c, e, "plane scan");
+    if (e != hipSuccess) return ctr::hip_fail(c, e, "quad tree flags");
+    sc.nodes4f = reinterpret_cast<const float4*>(fast);
     sc.plane_flags = h;
     return CT_OK;
 }
@@ -406,7 +413,7 @@ int ct_scene_upload(ct_ctx* c, const ct_scene_view* v) {
     sc.n_nodes4 = v->n_mbvh_nodes;
     sc.n_nodes2 = v->n_bvh_nodes;
     sc.n_fnodes = (uint32_t)(fn.size() / ct::kWideNodeWords);
-    if ((r = scan_planes(c, sc)) != CT_OK) { free_scene(c); return r; }
+    if ((r = prepare_quad(c, sc)) != CT_OK) { free_scene(c); return r; }
     install_scene(c, v, sc, sh, wide_bound);
     return CT_OK;
 }
@@ -456,7 +463,7 @@ int ct_scene_upload_build(ct_ctx* c, const ct_scene_view* v, uint32_t flags, ct_
     }
     if (r != CT_OK) { free_scene(c); return r; }
     HIPCHK(hipStreamSynchronize(c->ln[0].stream), "scene upload");
-    if ((r = scan_planes(c, sc)) != CT_OK) { free_scene(c); return r; }
+    if ((r = prepare_quad(c, sc)) != CT_OK) { free_scene(c); return r; }
     install_scene(c, v, sc, sh, wide_bound);
     bs.total_ms = (float)std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     if (st) *st = bs;
