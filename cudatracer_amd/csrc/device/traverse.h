@@ -515,6 +515,11 @@ CT_DEV void trace_persistent(const DevScene& sc, float eps, uint32_t n, uint32_t
             const float4 a = sc.tris[3 * (size_t)k];
             const float4 b = sc.tris[3 * (size_t)k + 1];
             const float4 c = sc.tris[3 * (size_t)k + 2];
+            // p0 is needed only after the determinant test: left to the
+            // compiler its load follows that test, a second memory round trip
+            // on most triangle tests.  Held here, all three rows are in flight
+            // together.
+            __asm__ volatile("" ::"v"(a.x), "v"(a.y), "v"(a.z));
             const uint32_t w = fbits(a.w);
             if (tri_test(o, d, t, xyz(a), xyz(b), xyz(c), eps)) {
                 prim = (int)(w & ~kLastInLeaf);
