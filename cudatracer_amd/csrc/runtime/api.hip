@@ -881,51 +881,84 @@ int ct_autotune(ct_ctx* c, const ct_camera* cam, const ct_frame_opts* opts, int3
     if (r != CT_OK) return r;
     (void)hipSetDevice(c->device);
     const uint64_t npix = (uint64_t)probe_width * probe_height;
+    // the probe view is sampled several times so that every timed run has at
+    // least 2M paths (shape) or 8M (chunk size): on a few hundred thousand
+    // paths launch latency and tails decide, and the choice was a slow shape
+    const uint64_t spp_shape = std::max<uint64_t>(1, ((1u << 21) + npix - 1) / npix);
+    const uint64_t spp_chunk = std::max<uint64_t>(1, ((1u << 23) + npix - 1) / npix);
+    if (npix * spp_chunk >= (1ull << 32)) return fail(c, CT_ERR_INVALID_ARG, "autotune probe too large");
     float4* acc = nullptr;
     r = dalloc(c, (void**)&acc, npix * sizeof(float4), "autotune accumulator");
     if (r != CT_OK) return r;
     const ctk::LaunchShape saved = c->shape;
     const uint32_t saved_chunk = c->chunk_default;
-    std::vector<ViewDesc> views{make_view(*cam, probe_width, probe_height, 1, 0u, 0)};
-    std::vector<Run> runs{{0u, 0u, (uint32_t)npix}};
-    double best = 1e30;
-    int best_block = saved.block, best_lds = saved.lds_depth, trials = 0;
+    std::vector<ViewDesc> views{make_view(*cam, probe_width, probe_height, (int)spp_shape, 0u, 0)};
+    std::vector<Run> runs{{0u, 0u, (uint32_t)(npix * spp_shape)}};
+    int trials = 0;
+    // time of one shape: two runs after a warm-up run
+    auto time_shape = [&](int b, int l, double* t) -> int {
+        ctk::LaunchShape trial = saved;
+        trial.block = b;
+        trial.lds_depth = l;
+        if (!ctk::set_trace_grids(trial, c->num_cus)) return CT_ERR_INVALID_ARG;
+        c->shape = trial;
+        *t = 0;
+        for (int rep = 0; rep < 3; ++rep) {
+            ChunkStats st;
+            const int rr = run_paths(c, views, runs, (uint32_t)(npix * spp_shape), acc, o, false, false, false, &st);
+            if (rr != CT_OK) return rr;
+            if (rep > 0) *t += st.total_ms;
+        }
+        ++trials;
+        return CT_OK;
+    };
+    double t_saved = 0;
+    if ((r = time_shape(saved.block, saved.lds_depth, &t_saved)) != CT_OK) {
+        c->shape = saved;
+        dfree(c, acc);
+        return r;
+    }
+    double best = t_saved;
+    int best_block = saved.block, best_lds = saved.lds_depth;
     const int blocks[] = {64, 128, 256};
     const int ldss[] = {8, 16, 24, 32};
     for (int b : blocks)
         for (int l : ldss) {
-            ctk::LaunchShape trial = c->shape;
-            trial.block = b;
-            trial.lds_depth = l;
-            if (!ctk::set_trace_grids(trial, c->num_cus)) continue;
-            c->shape = trial;
+            if (b == saved.block && l == saved.lds_depth) continue;
             double t = 0;
-            bool ok = true;
-            for (int rep = 0; rep < 3 && ok; ++rep) {
-                ChunkStats st;
-                r = run_paths(c, views, runs, (uint32_t)npix, acc, o, false, false, false, &st);
-                if (r != CT_OK) { ok = false; break; }
-                if (rep > 0) t += st.total_ms;  // first run warms up
-            }
-            if (!ok) { c->shape = saved; dfree(c, acc); return r; }
-            ++trials;
+            r = time_shape(b, l, &t);
+            if (r == CT_ERR_INVALID_ARG) continue;  // does not fit a CU
+            if (r != CT_OK) { c->shape = saved; dfree(c, acc); return r; }
             if (t < best) { best = t; best_block = b; best_lds = l; }
         }
+    // a new shape only when it is at least 2% faster than the current one
+    if (best > 0.98 * t_saved) { best = t_saved; best_block = saved.block; best_lds = saved.lds_depth; }
     ctr_apply_shape(c, best_block, best_lds);
-    // chunk size: paths per wavefront chunk for multi-sample work
-    const uint32_t chunks[] = {1u << 20, 1u << 21, 1u << 22};
-    const int spp_probe = (int)std::max<uint64_t>(1, (1u << 22) / npix);
-    std::vector<ViewDesc> v2{make_view(*cam, probe_width, probe_height, spp_probe, 0u, 0)};
-    std::vector<Run> r2{{0u, 0u, (uint32_t)(npix * spp_probe)}};
-    double best_c = 1e30;
-    uint32_t best_chunk = saved_chunk;
-    for (uint32_t ch : chunks) {
+    // chunk size: paths per wavefront chunk for multi-sample work; again the
+    // current one unless another is at least 2% faster
+    std::vector<ViewDesc> v2{make_view(*cam, probe_width, probe_height, (int)spp_chunk, 0u, 0)};
+    std::vector<Run> r2{{0u, 0u, (uint32_t)(npix * spp_chunk)}};
+    auto time_chunk = [&](uint32_t ch, double* t) -> int {
         ChunkStats st;
-        r = run_paths(c, v2, r2, ch, acc, o, false, false, false, &st);
+        const int rr = run_paths(c, v2, r2, ch, acc, o, false, false, false, &st);
+        if (rr == CT_OK) { *t = st.total_ms; ++trials; }
+        return rr;
+    };
+    double best_c = 0;
+    uint32_t best_chunk = saved_chunk;
+    r = time_chunk(saved_chunk, &best_c);
+    const double t_chunk_saved = best_c;
+    const uint32_t chunks[] = {1u << 21, 1u << 22, 1u << 23};
+    for (uint32_t ch : chunks) {
         if (r != CT_OK) break;
-        ++trials;
-        if (st.total_ms < best_c) { best_c = st.total_ms; best_chunk = ch; }
+        if (ch == saved_chunk) continue;
+        double t = 0;
+        const int rr = time_chunk(ch, &t);
+        if (rr == CT_ERR_OUT_OF_MEMORY) continue;  // path buffers beyond the memory budget
+        if ((r = rr) != CT_OK) break;
+        if (t < best_c) { best_c = t; best_chunk = ch; }
     }
+    if (best_c > 0.98 * t_chunk_saved) best_chunk = saved_chunk;
     dfree(c, acc);
     if (r != CT_OK) { c->shape = saved; c->chunk_default = saved_chunk; return r; }
     c->chunk_default = best_chunk;
