@@ -175,3 +175,44 @@ def test_edge_rays_any_hit(ct, orc, gpu_ctx, c2):
     occ_gpu = gpu_ctx.trace_shadow_rays(o, d, tmax, opts)
     occ_ref = osc.trace_shadow(o, d, tmax, 1, 32, opts.triangle_epsilon, threads=16)
     assert np.array_equal(occ_gpu, occ_ref)
+
+
+def _edit_empty_slots(arrays, what, seed):
+    """Copies of the arrays whose quad-tree empty slots (count 0) get a child
+    other than 0xFFFFFFFF ("child") or a NaN plane ("nan"); neither changes
+    what a traversal finds, but both turn off the fast node test (the upload
+    flags kEmptyEntry and kPlaneNaN), so rays take the exact test."""
+    rng = np.random.default_rng(seed)
+    a = dict(arrays)
+    m = arrays["mbvh"].copy()
+    u = m.view(np.uint32).reshape(-1, 32)
+    f = m.view(np.float32).reshape(-1, 32)
+    empty = np.argwhere(u[:, 28:32] == 0)
+    assert empty.shape[0] > 0, "the tree should have empty slots"
+    pick = empty[rng.choice(empty.shape[0], min(20, empty.shape[0]), replace=False)]
+    for node, slot in pick:
+        if what == "child":
+            u[node, 24 + slot] = 7
+        else:
+            f[node, 8 * int(rng.integers(0, 3)) + slot] = np.nan
+    a["mbvh"] = m
+    return a
+
+
+@pytest.mark.parametrize("what", ["child", "nan"])
+def test_user_tree_flags_take_exact_test(ct, orc, gpu_ctx, c2, what):
+    from conftest import ArrayView, ray_set
+    scene, arrays, _ = c2
+    edited = _edit_empty_slots(arrays, what, 5)
+    view = ArrayView(ct, edited)
+    gpu_ctx.upload_view(view.view)
+    osc = orc.Scene(edited, build=False)
+    osc.set_trees(edited["bvh"], edited["mbvh"], edited["prim_indices"])
+    o, d = ray_set(scene, edited, 1 << 17, seed=95)
+    _check_closest(ct, orc, gpu_ctx, scene, edited, o, d, ct.ACCEL_BVH4, f"empty slots ({what})", osc)
+    opts, _, _ = scene.frame_opts()
+    opts.accel = ct.ACCEL_BVH4
+    tmax = np.random.default_rng(96).uniform(0.0, 80.0, o.shape[0]).astype(np.float32)
+    occ_gpu = gpu_ctx.trace_shadow_rays(o, d, tmax, opts)
+    occ_ref = osc.trace_shadow(o, d, tmax, 1, 32, opts.triangle_epsilon, threads=16)
+    assert np.array_equal(occ_gpu, occ_ref)
