@@ -100,7 +100,7 @@ struct ChunkHdr {
     uint32_t n;        // path slots in the chunk
     uint32_t n_tasks;  // resolve tasks (pixels touched)
     uint32_t nseg;     // segments
-    uint32_t pad;
+    uint32_t gen;      // the chunk's generation (>= 1) in the stamps of the radiance terms
 };
 
 struct TraceOpts {
@@ -126,10 +126,14 @@ struct Queues {
     float4* thr[2];     // throughput.xyz, lastBsdfPdf
     uint32_t* path[2];  // slot | lastBounceType << 30
     float2* hit;        // t, primitive id bits
-    float4* sh_o;       // shadow origin.xyz, tmax
-    float4* sh_d;       // shadow direction.xyz, slot bits
-    float4* sh_c;       // shadow colour.xyz
-    float4* lpath;      // per-slot radiance (xyz)
+    // shadow queue of bounce b at [b % 2]: connect(b) still reads it while
+    // shade(b + 1) writes the next one
+    float4* sh_o[2];    // shadow origin.xyz, tmax
+    float4* sh_d[2];    // shadow direction.xyz, slot bits
+    float4* sh_c[2];    // shadow colour.xyz
+    float4* lpath;      // per-slot radiance (xyz), .w = bounces whose terms are folded in (bits)
+    float4* ebuf[2];    // shade's radiance term of bounce b at [b % 2] (xyz), .w = stamp (bits)
+    float4* nbuf[2];    // connect's radiance term of bounce b at [b % 2] (xyz), .w = stamp (bits)
     uint32_t* counters; // see CounterIndex
     uint32_t* spill;    // traversal stack overflow area
     uint32_t* err;      // error flags
@@ -137,6 +141,21 @@ struct Queues {
 
 // counters[] layout: for bounce b in [0, kMaxBounce]
 constexpr int kMaxBounce = 64;
+
+// Radiance terms.  A path's radiance is the sum, in this order, of shade's
+// term (emission or sky) and connect's term (unoccluded light sample) of
+// bounce 0, 1, 2, ...; float addition is not associative, so the order is
+// the reference's.  The terms of bounce b are written to ebuf / nbuf[b % 2]
+// stamped (gen << 6) | b; shade(b + 2) folds bounce b's pair into lpath and
+// k_resolve folds the pairs still open, so shade(b + 1) need not wait for
+// connect(b).
+CT_DEV uint32_t term_stamp(uint32_t gen, int b) { return (gen << 6) | (uint32_t)b; }
+CT_DEV float4 term(f3 v, uint32_t stamp) { return make_float4(v.x, v.y, v.z, __uint_as_float(stamp)); }
+CT_DEV void add_term(float4& L, float4 t) {
+    L.x = L.x + t.x;
+    L.y = L.y + t.y;
+    L.z = L.z + t.z;
+}
 CT_DEV uint32_t* cnt_active(uint32_t* c, int b) { return c + b; }
 CT_DEV uint32_t* cnt_shadow(uint32_t* c, int b) { return c + (kMaxBounce + 1) + b; }
 // Work-fetch counters, [queue][range][bounce]: each queue range owns a

@@ -144,36 +144,35 @@ __global__ void __attribute__((amdgpu_waves_per_eu(8))) k_extend(DevScene sc, Tr
 }
 
 // -------------------------------------------------------------- connect --
-// Persistent any-hit over the bounce's shadow queue; unoccluded rays add
-// their colour to the path's radiance (Kernel.cu:247-256).
+// Persistent any-hit over the bounce's shadow queue; an unoccluded ray's
+// colour is its path's radiance term of this bounce (Kernel.cu:247-256),
+// stored for shade(bounce + 2) or k_resolve to add in order.
 template <int ACCEL, bool COUNT>
-CT_DEV void connect_impl(const DevScene& sc, const TraceOpts& to, const Queues& q, int bounce, LaneStack& st,
-                         TravCount& tc) {
+CT_DEV void connect_impl(const DevScene& sc, const TraceOpts& to, const Queues& q, const ChunkHdr* hdr, int bounce,
+                         LaneStack& st, TravCount& tc) {
     const uint32_t n = *cnt_shadow(q.counters, bounce);
+    const uint32_t stamp = term_stamp(hdr->gen, bounce);
     trace_persistent<ACCEL, true, COUNT>(
         sc, to.teps, n, cnt_fetch_sh(q.counters, bounce), st, tc, q.err,
         [&](uint32_t i) {
-            const float4 o4 = q.sh_o[i], d4 = q.sh_d[i];
+            const float4 o4 = q.sh_o[bounce & 1][i], d4 = q.sh_d[bounce & 1][i];
             return RayIn{xyz(o4), xyz(d4), o4.w};
         },
         [&](uint32_t i, float, int prim) {
             if (prim != -1) return;  // occluded (or stack overflow, flagged)
-            const uint32_t slot = __float_as_uint(q.sh_d[i].w);
-            const float4 c = q.sh_c[i];
-            float4 L = q.lpath[slot];
-            L.x = L.x + c.x;
-            L.y = L.y + c.y;
-            L.z = L.z + c.z;
-            q.lpath[slot] = L;
+            const uint32_t slot = __float_as_uint(q.sh_d[bounce & 1][i].w);
+            const float4 c = q.sh_c[bounce & 1][i];
+            q.nbuf[bounce & 1][slot] = make_float4(c.x, c.y, c.z, __uint_as_float(stamp));
         });
 }
 
 template <int TREE, bool COUNT>
-__global__ void __attribute__((amdgpu_waves_per_eu(8))) k_connect(DevScene sc, TraceOpts to, Queues q, int bounce, int lds_depth,
-                          unsigned long long* wstats) {
+__global__ void __attribute__((amdgpu_waves_per_eu(8)))
+k_connect(DevScene sc, TraceOpts to, Queues q, const ChunkHdr* hdr, int bounce, int lds_depth,
+          unsigned long long* wstats) {
     LaneStack st = make_stack(q, lds_depth);
     TravCount tc{0u, 0u};
-    connect_impl<TREE, COUNT>(sc, to, q, bounce, st, tc);
+    connect_impl<TREE, COUNT>(sc, to, q, hdr, bounce, st, tc);
     if (COUNT) {
         atomicAdd(&wstats[2], (unsigned long long)tc.nodes);
         atomicAdd(&wstats[3], (unsigned long long)tc.tris);
@@ -271,6 +270,7 @@ k_shade(DevScene sc, ShadeOpts so, const ViewDesc* __restrict__ views, const Seg
     __shared__ uint32_t s_perm[SORT ? kShadeBlock : 1];
     const uint32_t n = *cnt_active(q.counters, bounce);
     const int nseg = (int)hdr->nseg;
+    const uint32_t gen = hdr->gen;
     const int nsel = sel ^ 1;
     const uint32_t stride = gridDim.x * kShadeBlock;
     // block-uniform loop: all threads of a block iterate together
@@ -491,13 +491,21 @@ k_shade(DevScene sc, ShadeOpts so, const ViewDesc* __restrict__ views, const Seg
                     }
                 }
             }
-            if (contrib) {
-                float4 L = q.lpath[slot];
-                L.x = L.x + add.x;
-                L.y = L.y + add.y;
-                L.z = L.z + add.z;
-                q.lpath[slot] = L;
+            // bounce - 2's pair of terms into the radiance (shade's, then
+            // connect's), then this bounce's term for shade(bounce + 2)
+            if (bounce >= 2) {
+                const uint32_t st = term_stamp(gen, bounce - 2);
+                const float4 e = q.ebuf[bounce & 1][slot], nn = q.nbuf[bounce & 1][slot];
+                const bool he = __float_as_uint(e.w) == st, hn = __float_as_uint(nn.w) == st;
+                if (he || hn) {
+                    float4 L = q.lpath[slot];
+                    if (he) add_term(L, e);
+                    if (hn) add_term(L, nn);
+                    L.w = __uint_as_float((uint32_t)bounce - 1u);
+                    q.lpath[slot] = L;
+                }
             }
+            if (contrib) q.ebuf[bounce & 1][slot] = term(add, term_stamp(gen, bounce));
         }
         // block-aggregated queue appends (every thread of the block reaches here)
         uint32_t je = 0, js = 0;
@@ -510,19 +518,45 @@ k_shade(DevScene sc, ShadeOpts so, const ViewDesc* __restrict__ views, const Seg
             q.path[nsel][je] = slot | (e_type << 30);
         }
         if (push_sh) {
-            q.sh_o[js] = make_float4(s_o.x, s_o.y, s_o.z, s_tmax);
-            q.sh_d[js] = make_float4(s_d.x, s_d.y, s_d.z, __uint_as_float(slot));
-            q.sh_c[js] = make_float4(s_c.x, s_c.y, s_c.z, 0.0f);
+            q.sh_o[bounce & 1][js] = make_float4(s_o.x, s_o.y, s_o.z, s_tmax);
+            q.sh_d[bounce & 1][js] = make_float4(s_d.x, s_d.y, s_d.z, __uint_as_float(slot));
+            q.sh_c[bounce & 1][js] = make_float4(s_c.x, s_c.y, s_c.z, 0.0f);
         }
     }
 }
 
 // -------------------------------------------------------------- resolve --
+// The radiance terms a path's shades have not folded in: stamped with this
+// chunk and a bounce >= the folded count, which leaves at most the path's
+// last two bounces (one per parity).  Added in bounce order, shade's term
+// before connect's.
+CT_DEV void fold_open_terms(float4& L, const Queues& q, uint32_t s, uint32_t gen) {
+    const uint32_t kf = __float_as_uint(L.w);
+    const float4 e0 = q.ebuf[0][s], n0 = q.nbuf[0][s], e1 = q.ebuf[1][s], n1 = q.nbuf[1][s];
+    auto open = [&](float4 v) -> int {
+        const uint32_t st = __float_as_uint(v.w);
+        return ((st >> 6) == gen && (st & 63u) >= kf) ? (int)(st & 63u) : -1;
+    };
+    const int ke0 = open(e0), kn0 = open(n0), ke1 = open(e1), kn1 = open(n1);
+    const int k0 = ke0 > kn0 ? ke0 : kn0, k1 = ke1 > kn1 ? ke1 : kn1;
+    const bool one_first = k1 >= 0 && (k0 < 0 || k1 < k0);
+    for (int pass = 0; pass < 2; ++pass) {
+        if ((pass == 0) != one_first) {
+            if (ke0 >= 0) add_term(L, e0);
+            if (kn0 >= 0) add_term(L, n0);
+        } else {
+            if (ke1 >= 0) add_term(L, e1);
+            if (kn1 >= 0) add_term(L, n1);
+        }
+    }
+}
+
 // accum[pixel] += per-sample radiance, samples in increasing order.
 __global__ void k_resolve(const ViewDesc* __restrict__ views, const Segment* __restrict__ segs,
-                          const ChunkHdr* hdr, const float4* __restrict__ lpath, float4* accum) {
+                          const ChunkHdr* hdr, Queues q, float4* accum) {
     const uint32_t n_tasks = hdr->n_tasks;
     const int nseg = (int)hdr->nseg;
+    const uint32_t gen = hdr->gen;
     for (uint32_t j = blockIdx.x * blockDim.x + threadIdx.x; j < n_tasks; j += gridDim.x * blockDim.x) {
         int lo = 0, hi = nseg - 1;
         while (lo < hi) {
@@ -539,7 +573,9 @@ __global__ void k_resolve(const ViewDesc* __restrict__ views, const Segment* __r
         float4* a = accum + v.accum_offset + p;
         float4 acc = *a;
         for (uint32_t g = g0; g < g1; ++g) {
-            const float4 L = lpath[sg.chunk_offset + (g - sg.first_path)];
+            const uint32_t s = sg.chunk_offset + (g - sg.first_path);
+            float4 L = q.lpath[s];
+            fold_open_terms(L, q, s, gen);
             acc.x = acc.x + L.x;
             acc.y = acc.y + L.y;
             acc.z = acc.z + L.z;
@@ -567,7 +603,7 @@ CT_DEV void flags_impl(const DevScene& sc, const TraceOpts& to, const Queues& q,
     trace_persistent<ACCEL, true, false>(
         sc, to.teps, n, cnt_fetch_sh(q.counters, 0), st, tc, q.err,
         [&](uint32_t i) {
-            const float4 o4 = q.sh_o[i], d4 = q.sh_d[i];
+            const float4 o4 = q.sh_o[0][i], d4 = q.sh_d[0][i];
             return RayIn{xyz(o4), xyz(d4), o4.w};
         },
         [&](uint32_t i, float, int prim) { out[i] = prim >= 0 ? 1 : 0; });
@@ -610,14 +646,15 @@ void extend_t(const LaunchShape& ls, const DevScene& sc, const TraceOpts& to, co
         hipLaunchKernelGGL((k_extend<TREE, false>), grid, block, lds, s, sc, to, q, sel, bounce, ls.lds_depth, wstats);
 }
 template <int TREE>
-void connect_t(const LaunchShape& ls, const DevScene& sc, const TraceOpts& to, const Queues& q, int bounce,
-               unsigned long long* wstats, hipStream_t s) {
+void connect_t(const LaunchShape& ls, const DevScene& sc, const TraceOpts& to, const Queues& q, const ChunkHdr* hdr,
+               int bounce, unsigned long long* wstats, hipStream_t s) {
     const size_t lds = (size_t)ls.lds_depth * ls.block * 4;
     const dim3 grid(grid_of(ls, TREE)), block(ls.block);
     if (wstats)
-        hipLaunchKernelGGL((k_connect<TREE, true>), grid, block, lds, s, sc, to, q, bounce, ls.lds_depth, wstats);
+        hipLaunchKernelGGL((k_connect<TREE, true>), grid, block, lds, s, sc, to, q, hdr, bounce, ls.lds_depth, wstats);
     else
-        hipLaunchKernelGGL((k_connect<TREE, false>), grid, block, lds, s, sc, to, q, bounce, ls.lds_depth, wstats);
+        hipLaunchKernelGGL((k_connect<TREE, false>), grid, block, lds, s, sc, to, q, hdr, bounce, ls.lds_depth,
+                           wstats);
 }
 
 hipError_t launch_extend(const LaunchShape& ls, const DevScene& sc, const TraceOpts& to, const Queues& q, int sel,
@@ -630,11 +667,11 @@ hipError_t launch_extend(const LaunchShape& ls, const DevScene& sc, const TraceO
     return hipGetLastError();
 }
 hipError_t launch_connect(const LaunchShape& ls, const DevScene& sc, const TraceOpts& to, const Queues& q,
-                          int bounce, unsigned long long* wstats, hipStream_t s) {
+                          const ChunkHdr* hdr, int bounce, unsigned long long* wstats, hipStream_t s) {
     switch (tree_of(to)) {
-        case 2: connect_t<2>(ls, sc, to, q, bounce, wstats, s); break;
-        case 4: connect_t<4>(ls, sc, to, q, bounce, wstats, s); break;
-        default: connect_t<8>(ls, sc, to, q, bounce, wstats, s); break;
+        case 2: connect_t<2>(ls, sc, to, q, hdr, bounce, wstats, s); break;
+        case 4: connect_t<4>(ls, sc, to, q, hdr, bounce, wstats, s); break;
+        default: connect_t<8>(ls, sc, to, q, hdr, bounce, wstats, s); break;
     }
     return hipGetLastError();
 }
@@ -650,10 +687,10 @@ hipError_t launch_shade(const LaunchShape& ls, const DevScene& sc, const ShadeOp
     return hipGetLastError();
 }
 hipError_t launch_resolve(const LaunchShape& ls, const ViewDesc* views, const Segment* segs, const ChunkHdr* hdr,
-                          uint32_t max_tasks, const float4* lpath, float4* accum, hipStream_t s) {
+                          uint32_t max_tasks, const Queues& q, float4* accum, hipStream_t s) {
     const uint32_t blocks =
         std::max<uint32_t>(1u, std::min<uint32_t>((max_tasks + 255) / 256, (uint32_t)ls.max_blocks));
-    hipLaunchKernelGGL(k_resolve, dim3(blocks), dim3(256), 0, s, views, segs, hdr, lpath, accum);
+    hipLaunchKernelGGL(k_resolve, dim3(blocks), dim3(256), 0, s, views, segs, hdr, q, accum);
     return hipGetLastError();
 }
 // The tonemapped image as 8-bit RGBA (alpha 255): byte = clamp(v, 0, 1) * 255 + 0.5, truncated.

@@ -29,12 +29,13 @@ hipError_t launch_generate(const LaunchShape& ls, const ctd::ViewDesc* views, co
 hipError_t launch_extend(const LaunchShape& ls, const ctd::DevScene& sc, const ctd::TraceOpts& to,
                          const ctd::Queues& q, int sel, int bounce, unsigned long long* wstats, hipStream_t s);
 hipError_t launch_connect(const LaunchShape& ls, const ctd::DevScene& sc, const ctd::TraceOpts& to,
-                          const ctd::Queues& q, int bounce, unsigned long long* wstats, hipStream_t s);
+                          const ctd::Queues& q, const ctd::ChunkHdr* hdr, int bounce, unsigned long long* wstats,
+                          hipStream_t s);
 hipError_t launch_shade(const LaunchShape& ls, const ctd::DevScene& sc, const ctd::ShadeOpts& so,
                         const ctd::ViewDesc* views, const ctd::Segment* segs, const ctd::ChunkHdr* hdr,
                         const ctd::Queues& q, int sel, int bounce, hipStream_t s);
 hipError_t launch_resolve(const LaunchShape& ls, const ctd::ViewDesc* views, const ctd::Segment* segs,
-                          const ctd::ChunkHdr* hdr, uint32_t max_tasks, const float4* lpath, float4* accum,
+                          const ctd::ChunkHdr* hdr, uint32_t max_tasks, const ctd::Queues& q, float4* accum,
                           hipStream_t s);
 hipError_t launch_tonemap(const float4* accum, uint32_t n, float samples, float4* out, hipStream_t s);
 hipError_t launch_tonemap8(const float4* accum, uint32_t n, float samples, uint32_t* out, hipStream_t s);

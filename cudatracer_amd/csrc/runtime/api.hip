@@ -769,8 +769,8 @@ int ct_trace_shadow_rays(ct_ctx* c, const float* origins, const float* dirs, con
             sd[i] = f4(dirs + 3 * (size_t)(base + i), 0.0f);
         }
         const ctd::ChunkHdr h{m, 0u, 0u, 0u};
-        e = hipMemcpyAsync(c->ln[0].q.sh_o, so.data(), m * sizeof(float4), hipMemcpyHostToDevice, c->ln[0].stream);
-        if (e == hipSuccess) e = hipMemcpyAsync(c->ln[0].q.sh_d, sd.data(), m * sizeof(float4), hipMemcpyHostToDevice, c->ln[0].stream);
+        e = hipMemcpyAsync(c->ln[0].q.sh_o[0], so.data(), m * sizeof(float4), hipMemcpyHostToDevice, c->ln[0].stream);
+        if (e == hipSuccess) e = hipMemcpyAsync(c->ln[0].q.sh_d[0], sd.data(), m * sizeof(float4), hipMemcpyHostToDevice, c->ln[0].stream);
         if (e == hipSuccess) e = hipMemcpyAsync(c->ln[0].d_hdr, &h, sizeof(h), hipMemcpyHostToDevice, c->ln[0].stream);
         if (e == hipSuccess) e = ctk::launch_setup(c->ln[0].q.counters, c->ln[0].d_hdr, c->ln[0].stream);
         if (e == hipSuccess) e = ctk::launch_shadow_flags(c->shape, c->sc, to, c->ln[0].q, m, (uint8_t*)dout, c->ln[0].stream);

@@ -54,6 +54,7 @@ struct Lane {
     std::string graph_key;
     hipEvent_t ev[2 * (3 * ctd::kMaxBounce + 8)] = {};
     int n_events = 0;
+    uint32_t gen = 0;  // generation of the lane's latest chunk (ChunkHdr::gen)
     // the chunk in flight (launched, not yet finished)
     bool pending = false;
     uint32_t n_paths = 0;

@@ -148,10 +148,20 @@ int ensure_paths(ct_ctx* c, uint32_t cap) {
         if (r == CT_OK) r = A((void**)&q.path[k], slots * 4, "path words");
     }
     if (r == CT_OK) r = A((void**)&q.hit, slots * sizeof(float2), "hits");
-    if (r == CT_OK) r = A((void**)&q.sh_o, f4, "shadow origins");
-    if (r == CT_OK) r = A((void**)&q.sh_d, f4, "shadow directions");
-    if (r == CT_OK) r = A((void**)&q.sh_c, f4, "shadow colours");
+    for (int k = 0; k < 2 && r == CT_OK; ++k) {
+        r = A((void**)&q.sh_o[k], f4, "shadow origins");
+        if (r == CT_OK) r = A((void**)&q.sh_d[k], f4, "shadow directions");
+        if (r == CT_OK) r = A((void**)&q.sh_c[k], f4, "shadow colours");
+    }
     if (r == CT_OK) r = A((void**)&q.lpath, f4, "path radiance");
+    for (int k = 0; k < 2 && r == CT_OK; ++k) {
+        r = A((void**)&q.ebuf[k], f4, "radiance terms");
+        if (r == CT_OK) r = A((void**)&q.nbuf[k], f4, "radiance terms");
+        // stamp 0 is never a chunk's (generations start at 1)
+        if (r == CT_OK && (hipMemsetAsync(q.ebuf[k], 0, f4, c->ln[0].stream) != hipSuccess ||
+                           hipMemsetAsync(q.nbuf[k], 0, f4, c->ln[0].stream) != hipSuccess))
+            r = fail(c, CT_ERR_DEVICE, "clear radiance terms");
+    }
     for (int l = 0; l < kLanes && r == CT_OK; ++l) {
         Lane& L = c->ln[l];
         const size_t o = (size_t)l * cap;
@@ -163,10 +173,17 @@ int ensure_paths(ct_ctx* c, uint32_t cap) {
             L.q.path[k] = q.path[k] + o;
         }
         L.q.hit = q.hit + o;
-        L.q.sh_o = q.sh_o + o;
-        L.q.sh_d = q.sh_d + o;
-        L.q.sh_c = q.sh_c + o;
+        for (int k = 0; k < 2; ++k) {
+            L.q.sh_o[k] = q.sh_o[k] + o;
+            L.q.sh_d[k] = q.sh_d[k] + o;
+            L.q.sh_c[k] = q.sh_c[k] + o;
+        }
         L.q.lpath = q.lpath + o;
+        for (int k = 0; k < 2; ++k) {
+            L.q.ebuf[k] = q.ebuf[k] + o;
+            L.q.nbuf[k] = q.nbuf[k] + o;
+        }
+        L.gen = 0;
         r = A((void**)&L.q.counters, ctd::kCounterWords * 4, "counters");
         if (r == CT_OK) r = A((void**)&L.q.err, 64, "error flags");
         if (r == CT_OK) r = A((void**)&L.q.spill, (size_t)threads * ctd::kMaxStack * 4, "stack spill");
@@ -180,6 +197,8 @@ int ensure_paths(ct_ctx* c, uint32_t cap) {
         if (r == CT_OK && hipMemsetAsync(L.q.err, 0, 64, L.stream) != hipSuccess)
             r = fail(c, CT_ERR_DEVICE, "clear error flags");
     }
+    if (r == CT_OK && hipStreamSynchronize(c->ln[0].stream) != hipSuccess)
+        r = fail(c, CT_ERR_DEVICE, "clear radiance terms");
     if (r != CT_OK) {
         free_paths(c);
         return r;
@@ -319,16 +338,16 @@ struct BounceSeq {
             rec();
             return hipSuccess;
         }
-        // connect(b) on the side stream overlaps extend(b + 1): both read
-        // queues shade(b) wrote, and only connect and shade add to a path's
-        // radiance, so shade(b + 1) and resolve wait for connect(b) (the
-        // additions keep their order).  Its traversal stacks spill to their
-        // own area.
+        // connect(b) on the side stream overlaps extend(b + 1) and shade(b + 1):
+        // both read queues shade(b) wrote.  Connect's radiance term of bounce
+        // b is added by shade(b + 2) or resolve (the additions keep their
+        // order), so shade(b + 2) and resolve wait for connect(b).  Its
+        // traversal stacks spill to their own area.
         const int b = k - 1;
         rec();
         if ((r = ctk::launch_extend(c->shape, c->sc, to, L->q, sel, b, ws, s)) != hipSuccess) return r;
         rec();
-        if (nee && b > 0 && (r = hipStreamWaitEvent(s, L->join[b - 1], 0)) != hipSuccess) return r;
+        if (nee && b > 1 && (r = hipStreamWaitEvent(s, L->join[b - 2], 0)) != hipSuccess) return r;
         rec();
         if ((r = ctk::launch_shade(c->shape, c->sc, so, L->d_views, L->d_segs, L->d_hdr, L->q, sel, b, s)) !=
             hipSuccess)
@@ -339,7 +358,7 @@ struct BounceSeq {
             if ((r = hipStreamWaitEvent(L->side, L->fork[b], 0)) != hipSuccess) return r;
             if (timed) (void)hipEventRecord(L->ev[e], L->side);
             ++e;
-            if ((r = ctk::launch_connect(c->shape, c->sc, to, qside, b, ws, L->side)) != hipSuccess) return r;
+            if ((r = ctk::launch_connect(c->shape, c->sc, to, qside, L->d_hdr, b, ws, L->side)) != hipSuccess) return r;
             if (timed) (void)hipEventRecord(L->ev[e], L->side);
             ++e;
             if ((r = hipEventRecord(L->join[b], L->side)) != hipSuccess) return r;
@@ -403,7 +422,17 @@ static int stage_chunk(ct_ctx* c, Lane& L, const std::vector<ViewDesc>& views, c
         }
         L.h_stage_bytes = bytes;
     }
-    const ChunkHdr h{n, n_tasks, (uint32_t)segs.size(), 0u};
+    // the chunk's generation: its radiance terms' stamps differ from any
+    // left in the lane's term buffers by earlier chunks (cleared when the
+    // 26-bit counter wraps)
+    if (++L.gen >= (1u << 26)) {
+        L.gen = 1;
+        for (int k = 0; k < 2; ++k) {
+            HIPCHK(hipMemsetAsync(L.q.ebuf[k], 0, c->lane_cap * sizeof(float4), s), "clear radiance terms");
+            HIPCHK(hipMemsetAsync(L.q.nbuf[k], 0, c->lane_cap * sizeof(float4), s), "clear radiance terms");
+        }
+    }
+    const ChunkHdr h{n, n_tasks, (uint32_t)segs.size(), L.gen};
     std::memcpy(L.h_stage, &h, sizeof(h));
     std::memcpy(L.h_stage + 256, views.data(), vb);
     std::memcpy(L.h_stage + 256 + vb, segs.data(), sb);
@@ -445,7 +474,7 @@ static int close_chunk(ct_ctx* c, Lane& L, uint32_t n, float4* accum, const ct_f
     // accumulator updates in chunk order
     if (prev_resolved) HIPCHK(hipStreamWaitEvent(s, prev_resolved, 0), "order resolves");
     if (timed) HIPCHK(hipEventRecord(L.ev[kEvResolve], s), "event");
-    HIPCHK(ctk::launch_resolve(c->shape, L.d_views, L.d_segs, L.d_hdr, c->lane_cap, L.q.lpath, accum, s),
+    HIPCHK(ctk::launch_resolve(c->shape, L.d_views, L.d_segs, L.d_hdr, c->lane_cap, L.q, accum, s),
            "launch resolve");
     if (timed) HIPCHK(hipEventRecord(L.ev[kEvResolve + 1], s), "event");
     HIPCHK(hipEventRecord(L.resolved, s), "event");
