@@ -18,8 +18,8 @@
 //     triangle of the current leaf, so long leaves do not stall node lanes;
 //   - the per-lane stack lives in LDS, laid out [depth][thread]; pushes are
 //     branch-free stores at the stack top (a store for a culled child is
-//     overwritten by the next one); entries beyond the LDS depth spill to a
-//     per-thread global area.  The builder's depth limit (64) bounds the
+//     overwritten by the next one); entries beyond the LDS depth spill to
+//     the block's global region, laid out the same way.  The builder's depth limit (64) bounds the
 //     stack at 97 entries; kMaxStack guards user-supplied trees.
 #pragma once
 
