@@ -686,6 +686,12 @@ int ct_ctx_create(const ct_device_opts* opts, ct_ctx** out) {
     ct_device_opts d;
     std::memset(&d, 0, sizeof(d));
     if (opts) d = *opts;
+    const int block = d.block_size > 0 ? d.block_size : 256;
+    const int lds = d.stack_lds > 0 ? d.stack_lds : 16;
+    if (block % 64 != 0 || block > 1024 || lds > ctd::kMaxStack) {
+        delete c;
+        return CT_ERR_INVALID_ARG;
+    }
     int ndev = 0;
     hipError_t e = hipGetDeviceCount(&ndev);
     if (e != hipSuccess || ndev == 0) {
@@ -723,9 +729,6 @@ int ct_ctx_create(const ct_device_opts* opts, ct_ctx** out) {
             return CT_ERR_DEVICE;
         }
     }
-    const int block = d.block_size > 0 ? d.block_size : 256;
-    const int lds = d.stack_lds > 0 ? d.stack_lds : 16;
-    if (block % 64 != 0 || block > 1024 || lds > ctd::kMaxStack) { delete c; return CT_ERR_INVALID_ARG; }
     ctr_apply_shape(c, block, lds);
     if (d.chunk_paths > 0) c->chunk_default = (uint32_t)d.chunk_paths;
     *out = c;

@@ -555,6 +555,15 @@ def test_cabi_host_errors(ct):
     assert abs(o.triangle_epsilon - 1e-3) < 1e-9 and abs(o.normal_epsilon - 1e-5) < 1e-12
 
 
+def test_ctx_create_rejects_bad_shape_before_device(ct):
+    """A launch shape the kernels cannot take is refused before any device
+    call, so the error is the same with or without a GPU."""
+    for kw in ({"block_size": 100}, {"block_size": 2048}, {"stack_lds": 129}):
+        with pytest.raises(ct.CTError) as e:
+            ct.Context(0, **kw)
+        assert e.value.code == ct.CT_ERR_INVALID_ARG, kw
+
+
 def test_add_material_forces_roughness_like_reference(ct):
     s = ct.HostScene.empty()
     i = s.add_material([0.5, 0.5, 0.5], roughness=0.05, keep_roughness=False)
