@@ -94,6 +94,25 @@ struct Segment {
     uint32_t pad[3];
 };
 
+// Pixel of a view's pixel ordinal (path index / spp).  Ordinals run down
+// strips of kTileRows rows column by column, so the 64 primary rays of a
+// wave cover a kTileRows-high tile and share more of the tree than a run of
+// one row.  The last strip has h % kTileRows rows when h is not a multiple.
+// A bijection of [0, w * h); seeds and the accumulator use the pixel itself,
+// so the order changes no result.
+#ifndef CT_TILE_ROWS
+#define CT_TILE_ROWS 8
+#endif
+constexpr uint32_t kTileRows = CT_TILE_ROWS;
+CT_DEV uint32_t pixel_of(uint32_t ord, uint32_t w, uint32_t h) {
+    const uint32_t strip = ord / (kTileRows * w);
+    const uint32_t r = ord - strip * (kTileRows * w);
+    const uint32_t y0 = strip * kTileRows;
+    const uint32_t rows = min(kTileRows, h - y0);
+    const uint32_t x = r / rows;
+    return (y0 + (r - x * rows)) * w + x;
+}
+
 // Per-chunk sizes, written by the host before the chunk's kernels run (so a
 // captured graph of the bounce loop replays unchanged for every chunk).
 struct ChunkHdr {
