@@ -1,4 +1,4 @@
-"""GPU tests of textures, normal maps and the sky texture (SURVEY row f2):
+"""GPU tests of textures, normal maps and the sky texture:
 the textured atrium (config 4) against the oracle (FP32 per pixel, rel-MSE
 against FP64), the invariants of the CPU tests on the GPU (a texture of the
 albedo's colour, a flat normal map and a 1x1 sky texture change no bit),

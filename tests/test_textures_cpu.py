@@ -1,4 +1,4 @@
-"""CPU tests of the texture path (SURVEY row f2): the shared atan2 against
+"""CPU tests of the texture path: the shared atan2 against
 libm, the sky mapping and nearest-texel addressing of SceneData::getTextureColor
 against their closed forms (SceneData.cuh:81-97, Kernel.cu:96-100), the
 TGA/PPM/PFM loaders' texel order, OBJ + MTL materials, and render invariants in the oracle that any
