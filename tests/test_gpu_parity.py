@@ -236,6 +236,27 @@ def test_batch_chunked_graph_replay(ct, orc, gpu_ctx, c1):
     assert np.array_equal(outs_big[0], ref)
 
 
+def test_odd_sizes_split_pixels_vs_oracle(ct, orc, c1):
+    """Views whose height is not a multiple of the pixel-tile height (a short
+    last strip), one-pixel-wide and three-pixel-high views, and a chunk size
+    that splits a pixel's samples over two chunks: every accumulator equals
+    the oracle's bit for bit (ids, counts and sums)."""
+    scene, arrays, osc = c1
+    opts, _, _ = scene.frame_opts()
+    sizes = [(37, 23, 3), (33, 17, 2), (5, 3, 1), (1, 7, 2), (64, 6, 5)]
+    views = [(scene.camera(w, h), w, h, spp, 3 * i) for i, (w, h, spp) in enumerate(sizes)]
+    ctx = ct.Context(0, chunk_paths=2048)
+    try:
+        ctx.upload(scene)
+        outs, st = ctx.render_batch(views, opts)
+    finally:
+        ctx.close()
+    assert st.chunks >= 2
+    for (cam, w, h, spp, fb), out in zip(views, outs):
+        ref, _ = osc.render(cam, opts, w, h, fb, spp, threads=16)
+        assert np.array_equal(out, ref), f"view {w}x{h} spp {spp}"
+
+
 @pytest.mark.parametrize("config", [1, 2])
 def test_material_sort_same_image(ct, orc, gpu_ctx, config):
     """Shading in block-local material order (sort_by_material) changes which
