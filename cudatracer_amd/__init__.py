@@ -192,6 +192,7 @@ _sig = {
     "ct_server_wait": (C.c_int, [_P, C.c_uint64, C.POINTER(Result), C.POINTER(C.c_float), C.c_size_t]),
     "ct_server_cancel": (C.c_int, [_P, C.c_uint64]),
     "ct_server_clock_ms": (C.c_double, [_P]),
+    "ct_server_last_error": (C.c_char_p, [_P]),
     "ct_server_destroy": (None, [_P]),
     "ct_host_scene_create": (C.c_int, [C.POINTER(_P)]),
     "ct_host_scene_generate": (C.c_int, [C.c_int32, C.c_uint64, C.POINTER(_P)]),
@@ -371,7 +372,10 @@ class HostScene:
         def arr(ptr, n, dt):
             if n == 0 or not ptr:
                 return np.zeros(0, dt)
-            return np.ctypeslib.as_array(ptr, shape=(n,)).copy()
+            # a typed view of the memory (np.ctypeslib.as_array on a pointer
+            # copies at ~130 MB/s, 17 s for the 7.24M-triangle scene)
+            addr = C.cast(ptr, C.c_void_p).value
+            return np.frombuffer((C.c_uint8 * (n * np.dtype(dt).itemsize)).from_address(addr), dt).copy()
 
         out = {
             "vertices": arr(v.vertices, 3 * nv, np.float32).reshape(-1, 3),
@@ -567,7 +571,9 @@ class Server:
         r.width, r.height, r.spp, r.frame_base = w, h, spp, frame_base
         r.opts = opts
         rid = C.c_uint64()
-        _check(lib.ct_server_submit(self._h, C.byref(r), C.byref(rid)), self._ctx.handle)
+        rc = lib.ct_server_submit(self._h, C.byref(r), C.byref(rid))
+        if rc != CT_OK:
+            raise CTError(rc, lib.ct_server_last_error(self._h).decode())
         self._sizes[rid.value] = (w, h)
         return rid.value
 

@@ -372,11 +372,15 @@ k_shade(DevScene sc, ShadeOpts so, const ViewDesc* __restrict__ views, const Seg
                     const bool typed = so.bsdf_mode == CT_BSDF_TYPED;
                     const float roughness = M0.w;
                     const float alpha = gmax(roughness, 1e-3f);
+                    // `type` is the material's lobe, set before sampling:
+                    // next-event estimation runs for a diffuse or glossy
+                    // lobe also when the sample is rejected (the path then
+                    // ends after it), which keeps the MIS sum unbiased
                     if ((!typed && roughness < 0.1f) || (typed && mtype == MAT_SPECULAR)) {
+                        type = LT_SPECULAR;
                         wo = reflect(dir, iN);
                         if (dot(N, wo) > 0.0f) {
                             pdf = 1.0f;
-                            type = LT_SPECULAR;
                             f = color * (1.0f / ct_fabsf(dot(iN, wo)));
                             valid = true;
                         }
@@ -401,6 +405,7 @@ k_shade(DevScene sc, ShadeOpts so, const ViewDesc* __restrict__ views, const Seg
                         }
                         if (valid) f = color * (1.0f / ct_fabsf(dot(iN, wo)));
                     } else if (typed && is_microfacet(mtype)) {
+                        type = LT_GLOSSY;
                         f3 Tv, Bv;
                         tangent_frame(iN, Tv, Bv);
                         const f3 wol = mk3(dot(wiw, Tv), dot(wiw, Bv), dot(wiw, iN));
@@ -411,21 +416,20 @@ k_shade(DevScene sc, ShadeOpts so, const ViewDesc* __restrict__ views, const Seg
                                 wo = normalize(wil.x * Tv + wil.y * Bv + wil.z * iN);
                                 if (dot(N, wo) > 0.0f) {
                                     mf_eval(mtype, alpha, color, wol, wil, &f, &pdf);
-                                    type = LT_GLOSSY;
                                     valid = pdf > 0.0f;
                                 }
                             }
                         }
                     } else {
+                        type = LT_DIFFUSE;
                         wo = normalize(tangent_to_world(cos_weighted(r3, r4), iN));
                         if (dot(N, wo) > 0.0f) {
-                            type = LT_DIFFUSE;
                             pdf = gmax(0.0f, dot(wo, iN)) * CT_INV_PI_F;
                             f = color * CT_INV_PI_F;
                             valid = true;
                         }
                     }
-                    if (valid) {
+                    {
                         // next-event estimation (Kernel.cu:161-195)
                         if (type != LT_SPECULAR && so.shadow && sc.n_lights > 0) {
                             int light = ct_rand_int_max(&seed, (int)sc.n_lights - 1);
@@ -467,8 +471,8 @@ k_shade(DevScene sc, ShadeOpts so, const ViewDesc* __restrict__ views, const Seg
                                 }
                             }
                         }
-                        // continuation (Kernel.cu:197-205)
-                        if (!(pdf < 1e-6f || isnanf_(pdf) || isnanf_(thr.x) || isnanf_(thr.y) || isnanf_(thr.z) ||
+                        // continuation (Kernel.cu:197-205); a rejected sample ends the path
+                        if (valid && !(pdf < 1e-6f || isnanf_(pdf) || isnanf_(thr.x) || isnanf_(thr.y) || isnanf_(thr.z) ||
                               (thr.x <= 0.0f && thr.y <= 0.0f && thr.z <= 0.0f))) {
                             thr = vmax0(thr * f * ct_fabsf(dot(iN, wo)));
                             org = transmit ? org - N * so.neps : org + N * so.neps;

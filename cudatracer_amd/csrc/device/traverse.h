@@ -502,7 +502,9 @@ CT_DEV void trace_persistent(const DevScene& sc, float eps, uint32_t n, uint32_t
                 const float4 hi = sc.nodes2[2 * (size_t)e + 1];
                 const int count = (int)fbits(hi.w);
                 if (count > -1) {
-                    k = fbits(lo.w);  // leaf: triangles until the end-of-leaf flag
+                    // leaf: triangles until the end-of-leaf flag (an empty
+                    // leaf of a user-supplied tree has none)
+                    if (count > 0) k = fbits(lo.w);
                 } else {
                     if (COUNT) cnt.nodes += 2;
                     node2_step(sc.nodes2, fbits(lo.w), o, inv, st);

@@ -549,15 +549,6 @@ int ct_reset(ct_ctx* c) {
     return CT_OK;
 }
 
-static bool camera_ok(const ct_camera* cam) {
-    if (!cam) return false;
-    if (!finite3(cam->position) || !finite3(cam->forward) || !finite3(cam->up) || !std::isfinite(cam->plane_distance))
-        return false;
-    const float l2 = cam->forward[0] * cam->forward[0] + cam->forward[1] * cam->forward[1] +
-                     cam->forward[2] * cam->forward[2];
-    return l2 > 0.0f && cam->plane_distance > 0.0f;
-}
-
 static void fill_stats(const ChunkStats& s, ct_frame_stats* out) {
     if (!out) return;
     out->primary_rays = s.primary;
@@ -597,7 +588,8 @@ int ct_trace_frame(ct_ctx* c, const ct_camera* cam, const ct_frame_opts* opts, c
 }
 
 // Frame with work counters (node / triangle tests per stage), for the
-// benchmark's algorithmic-byte accounting.  Not part of the public header.
+// benchmark's algorithmic-byte accounting (declared with the measurement
+// entry points of the header).
 int ct_trace_frame_counted(ct_ctx* c, const ct_camera* cam, const ct_frame_opts* opts, ct_frame_stats* stats,
                            uint64_t work[4]) {
     if (!c || !work) return CT_ERR_INVALID_ARG;
@@ -808,10 +800,18 @@ int ct_trace_rays_device(ct_ctx* c, const void* d_rays_o, const void* d_rays_d, 
     HIPCHK(hipEventRecord(c->ln[0].ev[0], c->ln[0].stream), "event");
     HIPCHK(ctk::launch_extend(c->shape, c->sc, to, q, 0, 0, nullptr, c->ln[0].stream), "extend");
     HIPCHK(hipEventRecord(c->ln[0].ev[1], c->ln[0].stream), "event");
+    uint32_t errflag = 0;
+    HIPCHK(hipMemcpyAsync(&errflag, c->ln[0].q.err, 4, hipMemcpyDeviceToHost, c->ln[0].stream), "flags");
     HIPCHK(hipStreamSynchronize(c->ln[0].stream), "trace rays");
     float ms = 0;
     HIPCHK(hipEventElapsedTime(&ms, c->ln[0].ev[0], c->ln[0].ev[1]), "elapsed");
     if (kernel_ms) *kernel_ms = ms;
+    if (errflag) {
+        // cleared here, so the next call on the context does not report it
+        HIPCHK(hipMemsetAsync(c->ln[0].q.err, 0, 4, c->ln[0].stream), "clear error flags");
+        HIPCHK(hipStreamSynchronize(c->ln[0].stream), "clear error flags");
+        return fail(c, CT_ERR_STACK_OVERFLOW, "traversal stack bound exceeded (rays reported as misses)");
+    }
     return CT_OK;
 }
 

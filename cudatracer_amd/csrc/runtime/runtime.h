@@ -110,6 +110,17 @@ int dalloc(ct_ctx* c, void** p, size_t bytes, const char* what);
 void dfree(ct_ctx* c, void* p);
 
 int validate_opts(ct_ctx* c, const ct_frame_opts& o);
+// Option checks of validate_opts without touching the context's error string
+// (for callers on other threads than the context's owner); *why gets the
+// message of a failure.
+int check_opts(const ct_ctx* c, const ct_frame_opts& o, const char** why);
+// Finite position, forward and up, non-zero forward, plane distance > 0.
+bool camera_ok(const ct_camera* cam);
+// Device bytes of one path slot of the path-state buffers (every lane holds
+// lane_cap slots): ray origin, direction and throughput (two sets), path
+// words (two), hit, shadow origin, direction and colour (two sets), path
+// radiance and the radiance terms (two sets each of shade's and connect's).
+constexpr size_t kPathSlotBytes = 6 * 16 + 2 * 4 + 8 + 6 * 16 + 16 + 4 * 16;
 // Path buffers for chunks of up to cap paths on each lane.
 int ensure_paths(ct_ctx* c, uint32_t cap);
 int ensure_tables(ct_ctx* c, Lane& L, uint32_t nviews, uint32_t nsegs);

@@ -88,25 +88,42 @@ void sync_lanes(ct_ctx* c) {
     }
 }
 
-int validate_opts(ct_ctx* c, const ct_frame_opts& o) {
-    if (o.max_depth < 1 || o.max_depth > ctd::kMaxBounce)
-        return fail(c, CT_ERR_INVALID_ARG, "max_depth must be in [1, 64]");
-    if (o.spp < 1 || o.spp > (1 << 20)) return fail(c, CT_ERR_INVALID_ARG, "spp must be in [1, 2^20]");
-    if (o.accel != CT_ACCEL_BVH2 && o.accel != CT_ACCEL_BVH4) return fail(c, CT_ERR_INVALID_ARG, "bad accel");
-    if (o.bsdf_mode != CT_BSDF_REFERENCE && o.bsdf_mode != CT_BSDF_TYPED)
-        return fail(c, CT_ERR_INVALID_ARG, "bad bsdf_mode");
-    if (o.traversal != CT_TRAV_EXACT && o.traversal != CT_TRAV_FAST)
-        return fail(c, CT_ERR_INVALID_ARG, "bad traversal");
+int check_opts(const ct_ctx* c, const ct_frame_opts& o, const char** why) {
+    auto bad = [&](int code, const char* msg) {
+        *why = msg;
+        return code;
+    };
+    if (o.max_depth < 1 || o.max_depth > ctd::kMaxBounce) return bad(CT_ERR_INVALID_ARG, "max_depth must be in [1, 64]");
+    if (o.spp < 1 || o.spp > (1 << 20)) return bad(CT_ERR_INVALID_ARG, "spp must be in [1, 2^20]");
+    if (o.accel != CT_ACCEL_BVH2 && o.accel != CT_ACCEL_BVH4) return bad(CT_ERR_INVALID_ARG, "bad accel");
+    if (o.bsdf_mode != CT_BSDF_REFERENCE && o.bsdf_mode != CT_BSDF_TYPED) return bad(CT_ERR_INVALID_ARG, "bad bsdf_mode");
+    if (o.traversal != CT_TRAV_EXACT && o.traversal != CT_TRAV_FAST) return bad(CT_ERR_INVALID_ARG, "bad traversal");
     if (o.light_pdf != CT_LIGHTPDF_REFERENCE && o.light_pdf != CT_LIGHTPDF_UNIFORM)
-        return fail(c, CT_ERR_INVALID_ARG, "bad light_pdf");
+        return bad(CT_ERR_INVALID_ARG, "bad light_pdf");
     if (!(o.triangle_epsilon >= 0.0f) || !(o.normal_epsilon >= 0.0f) || !(o.dist_epsilon >= 0.0f))
-        return fail(c, CT_ERR_INVALID_ARG, "epsilons must be finite and >= 0");
-    if (!c->has_scene) return fail(c, CT_ERR_NO_SCENE, "no scene uploaded");
-    if (o.accel == CT_ACCEL_BVH2 && !c->has_bvh2) return fail(c, CT_ERR_INVALID_ARG, "scene has no binary tree");
-    if (o.accel == CT_ACCEL_BVH4 && !c->has_bvh4) return fail(c, CT_ERR_INVALID_ARG, "scene has no quad tree");
+        return bad(CT_ERR_INVALID_ARG, "epsilons must be finite and >= 0");
+    if (!c->has_scene) return bad(CT_ERR_NO_SCENE, "no scene uploaded");
+    if (o.accel == CT_ACCEL_BVH2 && !c->has_bvh2) return bad(CT_ERR_INVALID_ARG, "scene has no binary tree");
+    if (o.accel == CT_ACCEL_BVH4 && !c->has_bvh4) return bad(CT_ERR_INVALID_ARG, "scene has no quad tree");
     if (o.traversal == CT_TRAV_FAST && (c->sc.n_fnodes == 0 || c->wide_stack_bound > (uint32_t)ctd::kMaxStack))
-        return fail(c, CT_ERR_INVALID_ARG, "fast traversal not available for this scene (wide tree too deep)");
+        return bad(CT_ERR_INVALID_ARG, "fast traversal not available for this scene (wide tree too deep)");
     return CT_OK;
+}
+
+int validate_opts(ct_ctx* c, const ct_frame_opts& o) {
+    const char* why = "";
+    const int r = check_opts(c, o, &why);
+    return r == CT_OK ? CT_OK : fail(c, r, why);
+}
+
+bool camera_ok(const ct_camera* cam) {
+    if (!cam) return false;
+    auto fin3 = [](const float* p) { return std::isfinite(p[0]) && std::isfinite(p[1]) && std::isfinite(p[2]); };
+    if (!fin3(cam->position) || !fin3(cam->forward) || !fin3(cam->up) || !std::isfinite(cam->plane_distance))
+        return false;
+    const float l2 = cam->forward[0] * cam->forward[0] + cam->forward[1] * cam->forward[1] +
+                     cam->forward[2] * cam->forward[2];
+    return l2 > 0.0f && cam->plane_distance > 0.0f;
 }
 
 static void free_paths(ct_ctx* c) {
@@ -536,6 +553,17 @@ static int finish_chunk(ct_ctx* c, Lane& L, ChunkStats* st) {
     return CT_OK;
 }
 
+// After a failed chunk: wait for every lane and clear all lanes' error
+// flags, so a chunk of the failed call that was never finished cannot fail
+// a later call.  The accumulator may hold part of the failed call's samples.
+static void abort_lanes(ct_ctx* c) {
+    sync_lanes(c);
+    for (Lane& L : c->ln) {
+        if (L.q.err) (void)hipMemsetAsync(L.q.err, 0, 4, L.stream);
+        if (L.stream) (void)hipStreamSynchronize(L.stream);
+    }
+}
+
 // Paths of a workload below which it is not split over the lanes.
 static constexpr uint64_t kSplitMin = 1u << 18;
 
@@ -621,7 +649,7 @@ int run_paths(ct_ctx* c, const std::vector<ViewDesc>& views, const std::vector<R
             left -= take;
             if (fill == cap) {
                 r = flush(segs, fill, tasks);
-                if (r != CT_OK) { sync_lanes(c); return r; }
+                if (r != CT_OK) { abort_lanes(c); return r; }
                 segs.clear();
                 fill = tasks = 0;
             }
@@ -629,13 +657,13 @@ int run_paths(ct_ctx* c, const std::vector<ViewDesc>& views, const std::vector<R
     }
     if (fill > 0) {
         r = flush(segs, fill, tasks);
-        if (r != CT_OK) { sync_lanes(c); return r; }
+        if (r != CT_OK) { abort_lanes(c); return r; }
     }
-    if (!group.empty() && (r = launch_group()) != CT_OK) { sync_lanes(c); return r; }
+    if (!group.empty() && (r = launch_group()) != CT_OK) { abort_lanes(c); return r; }
     for (int k = 0; k < kLanes; ++k) {
         r = finish_chunk(c, c->ln[lane], st);
         lane = (lane + 1) % kLanes;
-        if (r != CT_OK) { sync_lanes(c); return r; }
+        if (r != CT_OK) { abort_lanes(c); return r; }
     }
     // the other lanes' work is done: later work on lane 0 needs no further ordering
     return CT_OK;

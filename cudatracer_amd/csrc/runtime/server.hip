@@ -3,18 +3,24 @@
 // The reference renders one view per launchKernels call from its UI loop
 // (Application.cpp:101-125); it has no request queue.  Here a scheduler
 // thread owns the context:
-//   - submit() queues a request (camera, size, spp, frame base); a request
-//     whose footprint can never fit the budget is refused (CT_ERR_ADMISSION);
-//   - requests are admitted in arrival order while their footprint
-//     (accumulator + the path-state queue slots they can occupy) fits the
-//     memory budget, so requests that arrive while others are on the GPU join
-//     the next chunk;
-//   - every chunk packs path samples of the admitted requests into one
-//     wavefront (one generate, depth x (extend, shade, connect), one resolve,
-//     replayed from a captured hipGraph), smallest remaining work first with
-//     aging (remaining paths / (1 + wait / 50 ms)), so a small request that
-//     arrives behind a large one is done in the next chunk and a large one is
-//     not starved;
+//   - submit() checks and queues a request (camera, size, spp, frame base,
+//     frame options); a request whose accumulator can never fit the arena is
+//     refused (CT_ERR_ADMISSION);
+//   - memory: the budget holds the shared path-state queues (kLanes x chunk
+//     path slots of kPathSlotBytes, sized once at creation) and the
+//     accumulator arena (the rest).  A request is admitted, in arrival
+//     order, once its accumulator (width x height x 16 bytes) fits the arena;
+//     its path samples then take queue slots chunk by chunk as slots free up,
+//     so requests that arrive while others are on the GPU join the next chunk;
+//   - every chunk packs path samples of admitted requests into one wavefront
+//     (one generate, depth x (extend, shade, connect), one resolve, replayed
+//     from a captured hipGraph), smallest remaining work first with aging
+//     (remaining paths / (1 + wait / 50 ms)), so a small request that arrives
+//     behind a large one is done in the next chunk and a large one is not
+//     starved.  Every option of a request (depth, RR, BSDF mode, epsilons,
+//     light pdf, ...) is honoured: a chunk takes the options of its first
+//     request and only requests with the same options, others wait for a
+//     later chunk (aging moves them to the front);
 //   - a request is finished when its last path sample has been resolved; a
 //     copier thread moves its accumulator to host memory on its own stream
 //     while the next chunk runs, then releases its memory.
@@ -37,7 +43,6 @@ using namespace ctr;
 
 namespace {
 
-constexpr uint64_t kBytesPerPath = 176;  // path-state queue bytes per in-flight path sample
 // Default paths per chunk (split over the two lanes): small enough that a
 // request arriving while a chunk runs starts within a few milliseconds,
 // large enough that both lanes stay full.
@@ -48,10 +53,11 @@ enum ReqState { PENDING = 0, ACTIVE = 1, COPYING = 2, DONE = 3 };
 struct Req {
     uint64_t id = 0;
     ct_request r{};
+    ct_frame_opts opts{};          // r.opts with spp = 1 (the chunk options)
     int state = PENDING;
     int status = CT_OK;
     uint64_t total = 0, next = 0;  // path samples
-    uint64_t footprint = 0;
+    uint64_t footprint = 0;        // accumulator bytes
     uint64_t acc_off = 0;          // float4 offset in the arena
     double submit_ms = 0, admit_ms = 0, done_ms = 0;
     uint64_t rays = 0;
@@ -91,13 +97,14 @@ struct Arena {
     }
 };
 
+// Requests share a chunk only with equal options (spp normalised to 1).
+bool same_opts(const ct_frame_opts& a, const ct_frame_opts& b) { return std::memcmp(&a, &b, sizeof(a)) == 0; }
+
 }  // namespace
 
 struct ct_server {
     ct_ctx* ctx = nullptr;
     ct_server_opts opts{};
-    ct_frame_opts fopts{};
-    bool have_fopts = false;
     std::mutex mu;
     std::condition_variable cv;
     std::thread worker, copier;
@@ -108,7 +115,8 @@ struct ct_server {
     std::map<uint64_t, Req> reqs;
     std::deque<uint64_t> pending;
     std::vector<uint64_t> active;  // admission order
-    uint64_t in_use = 0;
+    uint64_t in_use = 0;     // accumulator bytes of admitted requests
+    uint64_t queue_bytes = 0;  // the shared path-state queues charged to the budget
     float4* arena_ptr = nullptr;
     uint64_t arena_len = 0;  // float4
     Arena arena;
@@ -116,6 +124,7 @@ struct ct_server {
     std::chrono::steady_clock::time_point t0;
     int fatal = CT_OK;
     std::string fatal_msg;
+    std::string err;  // message of the last failed call on the server (under mu)
 
     double now() const {
         return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
@@ -168,6 +177,7 @@ void ct_server::loop() {
     // out under the lock (submit() may insert into reqs concurrently)
     std::vector<std::pair<uint64_t, uint64_t>> fresh;
     std::vector<std::pair<double, uint64_t>> order;  // (priority, id)
+    ct_frame_opts chunk_opts{};
     while (true) {
         {
             std::unique_lock<std::mutex> lk(mu);
@@ -191,7 +201,6 @@ void ct_server::loop() {
                     pending.pop_front();
                     continue;
                 }
-                if (in_use + q.footprint > opts.mem_budget) break;
                 uint64_t off = 0;
                 if (!arena.alloc((uint64_t)q.r.width * q.r.height, &off)) break;
                 q.acc_off = off;
@@ -211,7 +220,8 @@ void ct_server::loop() {
                 cv.wait(lk);
                 continue;
             }
-            // pack one chunk: smallest remaining work first, with aging
+            // pack one chunk: smallest remaining work first, with aging; the
+            // first request sets the chunk's options
             views.clear();
             runs.clear();
             ids.clear();
@@ -225,12 +235,19 @@ void ct_server::loop() {
             }
             std::sort(order.begin(), order.end());
             uint64_t room = chunk;
+            bool have_opts = false;
             for (const auto& oi : order) {
                 const uint64_t id = oi.second;
                 if (room == 0) break;
                 Req& q = reqs[id];
                 const uint64_t take = std::min<uint64_t>(room, q.total - q.next);
                 if (take == 0) continue;
+                if (!have_opts) {
+                    chunk_opts = q.opts;
+                    have_opts = true;
+                } else if (!same_opts(q.opts, chunk_opts)) {
+                    continue;
+                }
                 views.push_back(make_view(q.r.camera, q.r.width, q.r.height, q.r.spp, q.r.frame_base, q.acc_off));
                 runs.push_back({(uint32_t)(views.size() - 1), (uint32_t)q.next, (uint32_t)take});
                 ids.push_back(id);
@@ -246,7 +263,7 @@ void ct_server::loop() {
                 rc = CT_ERR_DEVICE;
         ChunkStats st;
         if (rc == CT_OK)
-            rc = run_paths(ctx, views, runs, chunk, arena_ptr, fopts, false, false, true, &st);
+            rc = run_paths(ctx, views, runs, chunk, arena_ptr, chunk_opts, false, false, true, &st);
         std::unique_lock<std::mutex> lk(mu);
         if (rc != CT_OK) {
             fatal = rc;
@@ -289,7 +306,14 @@ int ct_server_create(ct_ctx* ctx, const ct_server_opts* opts, ct_server** out) {
     (void)hipSetDevice(ctx->device);
     int r = ensure_paths(ctx, s->chunk);
     if (r == CT_OK) {
-        s->arena_len = s->opts.mem_budget / sizeof(float4);
+        // the shared queues come out of the budget (at the size they hold for
+        // this server's chunks), the accumulator arena is the rest
+        s->queue_bytes = (uint64_t)kLanes * std::min<uint32_t>(ctx->lane_cap, s->chunk) * kPathSlotBytes;
+        if (s->queue_bytes + sizeof(float4) > s->opts.mem_budget)
+            r = fail(ctx, CT_ERR_OUT_OF_MEMORY, "server budget smaller than its shared path-state queues");
+    }
+    if (r == CT_OK) {
+        s->arena_len = (s->opts.mem_budget - s->queue_bytes) / sizeof(float4);
         r = dalloc(ctx, (void**)&s->arena_ptr, s->arena_len * sizeof(float4), "server accumulator arena");
     }
     if (r != CT_OK) {
@@ -314,28 +338,28 @@ int ct_server_create(ct_ctx* ctx, const ct_server_opts* opts, ct_server** out) {
 int ct_server_submit(ct_server* s, const ct_request* req, uint64_t* id) {
     if (!s || !req || !id) return CT_ERR_INVALID_ARG;
     const ct_request& r = *req;
-    if (r.width <= 0 || r.height <= 0 || r.spp <= 0) return CT_ERR_INVALID_ARG;
-    const uint64_t total = (uint64_t)r.width * r.height * (uint64_t)r.spp;
-    if (total >= (1ull << 32)) return CT_ERR_INVALID_ARG;
-    const uint64_t fp = (uint64_t)r.width * r.height * sizeof(float4) + std::min<uint64_t>(total, s->chunk) * kBytesPerPath;
     std::lock_guard<std::mutex> lk(s->mu);
-    if (fp > s->opts.mem_budget) return fail(s->ctx, CT_ERR_ADMISSION, "request can never fit the memory budget");
-    if (s->fatal != CT_OK) return s->fatal;
-    if (!s->have_fopts) {
-        ct_frame_opts o = r.opts;
-        o.spp = 1;
-        const int v = validate_opts(s->ctx, o);
-        if (v != CT_OK) return v;
-        s->fopts = o;
-        s->have_fopts = true;
-    } else if (r.opts.max_depth != s->fopts.max_depth || r.opts.accel != s->fopts.accel ||
-               r.opts.traversal != s->fopts.traversal || r.opts.bsdf_mode != s->fopts.bsdf_mode ||
-               r.opts.rr_start != s->fopts.rr_start || r.opts.shadow != s->fopts.shadow) {
-        return fail(s->ctx, CT_ERR_INVALID_ARG, "request options differ from the server's");
-    }
+    auto refuse = [&](int code, const char* why) {
+        s->err = why;
+        return code;
+    };
+    if (r.width <= 0 || r.height <= 0 || r.spp <= 0) return refuse(CT_ERR_INVALID_ARG, "bad request size or spp");
+    if (!camera_ok(&r.camera)) return refuse(CT_ERR_INVALID_ARG, "camera is not finite or degenerate");
+    const uint64_t total = (uint64_t)r.width * r.height * (uint64_t)r.spp;
+    if (total >= (1ull << 32)) return refuse(CT_ERR_INVALID_ARG, "width*height*spp must be < 2^32");
+    const uint64_t fp = (uint64_t)r.width * r.height * sizeof(float4);
+    if (fp > s->arena_len * sizeof(float4))
+        return refuse(CT_ERR_ADMISSION, "request's accumulator can never fit the server's arena");
+    if (s->fatal != CT_OK) return refuse(s->fatal, s->fatal_msg.c_str());
+    ct_frame_opts o = r.opts;
+    o.spp = 1;
+    const char* why = "";
+    const int v = check_opts(s->ctx, o, &why);  // scene flags only change while no server exists
+    if (v != CT_OK) return refuse(v, why);
     Req q;
     q.id = s->next_id++;
     q.r = r;
+    q.opts = o;
     q.total = total;
     q.footprint = fp;
     q.submit_ms = s->now();
@@ -397,6 +421,12 @@ int ct_server_cancel(ct_server* s, uint64_t id) {
 }
 
 double ct_server_clock_ms(const ct_server* s) { return s ? s->now() : 0.0; }
+
+const char* ct_server_last_error(ct_server* s) {
+    if (!s) return "null server";
+    std::lock_guard<std::mutex> lk(s->mu);
+    return s->err.c_str();
+}
 
 void ct_server_destroy(ct_server* s) {
     if (!s) return;

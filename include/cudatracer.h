@@ -325,17 +325,21 @@ int ct_autotune(ct_ctx* ctx, const ct_camera* cam, const ct_frame_opts* opts,
 typedef struct ct_server ct_server;
 
 typedef struct ct_server_opts {
-    uint64_t mem_budget;      /* device bytes for admitted requests' state   */
-    int32_t chunk_paths;      /* paths per shared wavefront chunk, 0 = ctx   */
-    int32_t max_active;       /* cap on concurrently admitted requests       */
+    uint64_t mem_budget;      /* device bytes of the server's state, 0 = 8 GiB: the
+                                 shared path-state queues (2 x chunk_paths path
+                                 slots of 288 B) and, in the rest, the arena that
+                                 holds admitted requests' accumulators          */
+    int32_t chunk_paths;      /* paths per shared wavefront chunk, 0 = 2^23  */
+    int32_t max_active;       /* cap on concurrently admitted requests, 0 = none */
 } ct_server_opts;
 
 typedef struct ct_request {
     ct_camera camera;
     int32_t width, height, spp;
     uint32_t frame_base;
-    ct_frame_opts opts;       /* per-request depth/RR/...; accel and traversal
-                                 must match the server's first request       */
+    ct_frame_opts opts;       /* every field but spp applies to this request;
+                                 requests share a wavefront chunk only with
+                                 requests of equal options                    */
 } ct_request;
 
 typedef struct ct_result {
@@ -345,9 +349,17 @@ typedef struct ct_result {
     uint64_t rays;            /* rays traced for this request                */
 } ct_result;
 
-/* The server thread owns ctx until ct_server_destroy returns. */
+/* The server thread owns ctx until ct_server_destroy returns.  Admission:
+ * a request is admitted in arrival order once its accumulator (width x height
+ * x 16 bytes) fits the arena; its path samples then take slots of the shared
+ * queues chunk by chunk.  A request whose accumulator is larger than the
+ * whole arena is refused at submit with CT_ERR_ADMISSION; invalid sizes,
+ * cameras or options with CT_ERR_INVALID_ARG (ct_server_last_error gives the
+ * reason). */
 int ct_server_create(ct_ctx* ctx, const ct_server_opts* opts, ct_server** out);
 int ct_server_submit(ct_server* srv, const ct_request* req, uint64_t* id);
+/* Message of the server's last refused call (valid until the next call). */
+const char* ct_server_last_error(ct_server* srv);
 /* Non-blocking: CT_OK with done=1 when finished (accumulator copied to
  * accum_out, 4 floats per pixel, if not NULL), done=0 while pending. */
 int ct_server_poll(ct_server* srv, uint64_t id, int32_t* done, ct_result* res,

@@ -168,6 +168,11 @@ struct or_scene {
     std::vector<uint32_t> prims;
     std::vector<Box> aabbs;
     bool built = false;
+    // any-hit visit order of the quad tree: false = the reference's sorted
+    // order (traverseMBVHShadow calls intersect, MBVHNode.cuh:165-205), true =
+    // slot order, slot 3 popped first (the order the kernels visit any-hit
+    // children in: the result is order-free, the work counts are not)
+    bool anyhit_slot_order = false;
     uint32_t ntris() const { return (uint32_t)indices.size() / 3; }
     const float* vtx(uint32_t tri, int k) const { return &vertices[3 * (size_t)indices[3 * (size_t)tri + k]]; }
     const float* nrm(uint32_t tri, int k) const { return &normals[3 * (size_t)indices[3 * (size_t)tri + k]]; }
@@ -472,6 +477,11 @@ bool traverse_bvh4(const or_scene& s, V3<T> o, V3<T> d, T* t, int* hit, T eps, C
         bool res[4];
         c.nodes++;
         mbvh_intersect(n, o, inv, *t, tmin, res);
+        if (AnyHit && s.anyhit_slot_order) {
+            for (int i = 0; i < 4; ++i)
+                if (res[i]) stack[++sp] = {n.childs[i], n.counts[i]};
+            continue;
+        }
         for (int i = 3; i >= 0; --i) {
             const int idx = tag_of(tmin[i]);
             if (res[idx]) stack[++sp] = {n.childs[idx], n.counts[idx]};
@@ -756,6 +766,16 @@ template <class T> T fresnel_dielectric(T cosThetaI, T eta) {
     return T(0.5) * (Rs * Rs + Rp * Rp);
 }
 
+// glm::refract(I, N, eta), include/glm/detail/func_geometric.inl:112-122:
+// k = 1 - eta^2 (1 - (N.I)^2); the direction eta I - (eta N.I + sqrt(k)) N,
+// valid (no total internal reflection) when k >= 0.
+template <class T> bool refract_dir(V3<T> I, V3<T> N, T eta, V3<T>* out) {
+    const T dv = dot(N, I);
+    const T k = T(1) - eta * eta * (T(1) - dv * dv);
+    *out = eta * I - (eta * dv + m_sqrt(k)) * N;
+    return k >= T(0);
+}
+
 // --------------------------------------------------------------- shading --
 enum { SPECULAR = 0, DIFFUSE = 1, GLOSSY = 2 };
 
@@ -892,11 +912,13 @@ V3<T> path_sample(const or_scene& s, const Camera3<T>& cb, const ct_frame_opts& 
         const bool typed = o.bsdf_mode == CT_BSDF_TYPED;
         const int mtype = mat.type;
         const T alpha = gmax(T(mat.roughness), T(1e-3f));
+        // `type` is the material's lobe, set before sampling: next-event
+        // estimation depends on the lobe, not on whether the sample is kept
         if ((!typed && mat.roughness < 0.1f) || (typed && mtype == CT_MAT_SPECULAR)) {
+            type = SPECULAR;
             wo = reflect(dir, iN);
             if (dot(N, wo) > T(0)) {
                 pdf = T(1);
-                type = SPECULAR;
                 f = color * (T(1) / m_abs(dot(iN, wo)));
                 valid = true;
             }
@@ -911,16 +933,12 @@ V3<T> path_sample(const or_scene& s, const Camera3<T>& cb, const ct_frame_opts& 
                 wo = reflect(dir, iN);
                 valid = dot(N, wo) > T(0);
             } else {
-                // glm::refract(I, N, eta), func_geometric.inl:112-122
-                const T eta = etaI / etaT;
-                const T dv = dot(iN, dir);
-                const T k = T(1) - eta * eta * (T(1) - dv * dv);
-                wo = eta * dir - (eta * dv + m_sqrt(k)) * iN;
-                valid = k >= T(0) && dot(N, wo) < T(0);
+                valid = refract_dir(dir, iN, etaI / etaT, &wo) && dot(N, wo) < T(0);
                 transmit = true;
             }
             if (valid) f = color * (T(1) / m_abs(dot(iN, wo)));
         } else if (typed && is_microfacet(mtype)) {
+            type = GLOSSY;
             V3<T> Tv, Bv;
             tangent_frame(iN, Tv, Bv);
             const V3<T> wol{dot(wiw, Tv), dot(wiw, Bv), dot(wiw, iN)};
@@ -931,22 +949,23 @@ V3<T> path_sample(const or_scene& s, const Camera3<T>& cb, const ct_frame_opts& 
                     wo = normalize(wil.x * Tv + wil.y * Bv + wil.z * iN);
                     if (dot(N, wo) > T(0)) {
                         mf_eval(mtype, alpha, color, wol, wil, &f, &pdf);
-                        type = GLOSSY;
                         valid = pdf > T(0);
                     }
                 }
             }
         } else {
+            type = DIFFUSE;
             wo = normalize(tangent_to_world(cos_weighted(r3, r4), iN));
             if (dot(N, wo) > T(0)) {
-                type = DIFFUSE;
                 pdf = gmax(T(0), dot(wo, iN)) * K<T>::inv_pi;
                 f = color * K<T>::inv_pi;
                 valid = true;
             }
         }
-        if (!valid) break;  // Q9
-        // --- next-event estimation (Kernel.cu:161-195)
+        // --- next-event estimation (Kernel.cu:161-195), for a diffuse or
+        // glossy lobe also when the BSDF sample is rejected below (Q9): with
+        // NEE skipped for rejected samples the MIS sum loses the light
+        // sample's share in proportion to the rejection rate (a bias)
         if (type != SPECULAR && o.shadow && lightCount > 0) {
             int light = ct_rand_int_max(&seed, (int)lightCount - 1);
             // RandomIntMax can reach lightCount once float(max)+0.99999f rounds up
@@ -995,6 +1014,7 @@ V3<T> path_sample(const or_scene& s, const Camera3<T>& cb, const ct_frame_opts& 
                 }
             }
         }
+        if (!valid) break;  // Q9: a rejected sample ends the path (after NEE)
         // --- continuation (Kernel.cu:197-205)
         if (pdf < T(1e-6f) || isnan_(pdf) || isnan_(thr.x) || isnan_(thr.y) || isnan_(thr.z) ||
             (thr.x <= T(0) && thr.y <= T(0) && thr.z <= T(0)))
@@ -1254,6 +1274,23 @@ void or_mf_sample_vndf32(int type, float alpha, const float* wo, float r1, float
     wm_out[0] = wm.x; wm_out[1] = wm.y; wm_out[2] = wm.z;
 }
 double or_fresnel_dielectric(double cos_i, double eta) { return fresnel_dielectric<double>(cos_i, eta); }
+
+int or_refract(int precision, const float* I, const float* N, float eta, double* out) {
+    if (precision == 64) {
+        V3<double> w;
+        const bool ok = refract_dir(ld<double>(I), ld<double>(N), (double)eta, &w);
+        out[0] = w.x; out[1] = w.y; out[2] = w.z;
+        return ok ? 1 : 0;
+    }
+    V3<float> w;
+    const bool ok = refract_dir(ld<float>(I), ld<float>(N), eta, &w);
+    out[0] = w.x; out[1] = w.y; out[2] = w.z;
+    return ok ? 1 : 0;
+}
+
+void or_scene_set_anyhit_order(or_scene* s, int slot_order) {
+    if (s) s->anyhit_slot_order = slot_order != 0;
+}
 
 int or_bsdf_local(int type, float roughness, const float* wo, const float* wi_in, float r1, float r2,
                   float* wi_out, float* f_out, float* pdf_out) {

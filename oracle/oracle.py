@@ -85,6 +85,8 @@ _sig = {
     "or_mf_lambda": (C.c_double, [C.c_int, C.c_double, C.POINTER(C.c_double)]),
     "or_mf_sample_vndf32": (None, [C.c_int, C.c_float, _fptr, C.c_float, C.c_float, _fptr]),
     "or_fresnel_dielectric": (C.c_double, [C.c_double, C.c_double]),
+    "or_refract": (C.c_int, [C.c_int, _fptr, _fptr, C.c_float, C.POINTER(C.c_double)]),
+    "or_scene_set_anyhit_order": (None, [_P, C.c_int]),
     "or_bsdf_local": (C.c_int, [C.c_int, C.c_float, _fptr, _fptr, C.c_float, C.c_float, _fptr, _fptr, _fptr]),
 }
 for _n, (_r, _a) in _sig.items():
@@ -148,6 +150,11 @@ class Scene:
             if lib.or_scene_build(self._h) != 0:
                 raise RuntimeError("oracle build failed")
 
+    def set_anyhit_order(self, slot_order: bool) -> None:
+        """Quad-tree any-hit visit order: False = the reference's sorted order,
+        True = the kernels' slot order (same occlusion, their work counts)."""
+        lib.or_scene_set_anyhit_order(self._h, 1 if slot_order else 0)
+
     def texel(self, texture: int, u: float, v: float) -> np.ndarray:
         out = np.zeros(4, np.float32)
         if lib.or_texel(self._h, texture, u, v, _fp(out)) != 0:
@@ -188,17 +195,18 @@ class Scene:
         return (t, p, cnt) if counters else (t, p)
 
     def trace_shadow(self, origins, dirs, tmax, accel: int, precision: int = 32, tri_eps: float = 1e-3,
-                     threads: int = 8):
+                     threads: int = 8, counters=False):
         o = np.ascontiguousarray(origins, np.float32)
         d = np.ascontiguousarray(dirs, np.float32)
         tm = np.ascontiguousarray(tmax, np.float32)
         n = o.shape[0]
         occ = np.empty(n, np.uint8)
+        cnt = np.zeros((n, 2), np.uint32) if counters else None
         r = lib.or_trace_shadow(self._h, accel, precision, n, _fp(o), _fp(d), _fp(tm), tri_eps,
-                                occ.ctypes.data_as(C.POINTER(C.c_uint8)), None, threads)
+                                occ.ctypes.data_as(C.POINTER(C.c_uint8)), _up(cnt) if counters else None, threads)
         if r != 0:
             raise RuntimeError("or_trace_shadow failed")
-        return occ
+        return (occ, cnt) if counters else occ
 
     def render(self, cam, opts, width: int, height: int, frame_base: int = 0, spp: int = 1, rows=None,
                precision: int = 32, threads: int = 8, accum=None):
@@ -222,3 +230,13 @@ class Scene:
         if getattr(self, "_h", None):
             lib.or_scene_destroy(self._h)
             self._h = None
+
+
+def refract(I, N, eta: float, precision: int = 64):
+    """glm::refract as the oracle's dielectric branch uses it: (direction,
+    valid); valid is False under total internal reflection."""
+    i = np.ascontiguousarray(I, np.float32)
+    n = np.ascontiguousarray(N, np.float32)
+    out = np.zeros(3, np.float64)
+    ok = lib.or_refract(precision, _fp(i), _fp(n), eta, out.ctypes.data_as(C.POINTER(C.c_double)))
+    return out, bool(ok)

@@ -59,6 +59,24 @@ def gpu_ctx(ct):
     ctx.close()
 
 
+@pytest.fixture(scope="session")
+def c3(ct, orc, gpu_ctx):
+    """The 7.24M-triangle scene with the trees of its device build (byte-
+    identical to the host build, which the builder tests pin against the
+    oracle's reference-algorithm build on smaller scenes); the oracle
+    traverses those trees instead of rebuilding them (a 30-plane rescan of
+    7.24M triangles takes minutes).  Tests upload it again with
+    gpu_ctx.upload_build(scene)."""
+    scene = ct.HostScene.generate(ct.SCENE_C3, 7, build=False)
+    gpu_ctx.upload_build(scene)
+    trees = gpu_ctx.read_trees()
+    arrays = scene.arrays()
+    arrays.update(trees)
+    osc = orc.Scene(arrays, build=False)
+    osc.set_trees(trees["bvh"], trees["mbvh"], trees["prim_indices"])
+    return scene, arrays, osc
+
+
 def camera_rays(cam, width: int, height: int, n: int, seed: int):
     """Jittered pinhole rays of the camera (same basis as the kernels)."""
     rng = np.random.default_rng(seed)
