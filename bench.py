@@ -5,20 +5,31 @@ quad BVH (ct_trace_frame).  value = (primary + extension + shadow rays traced)
 / wall time of the K timed steps, in Mrays/s.  Scene, tree and path state are
 resident on the device before timing starts.
 
-roofline: the closest-hit (extend) kernel.  Algorithmic bytes per closest-hit
-ray = 28 (origin, direction, tmax) + 8 (t, id) + 64 * quad-node tests + 36 *
-triangle tests; node and triangle tests are counted by an instrumented,
-untimed frame.  achieved = those bytes / the extend launches' device time
-(HIP events on the library stream around every extend launch of the timed
-frames).  Traversal launches overlap (the any-hit kernel of bounce b runs on
-a second stream beside the extend of bounce b + 1, and a frame's two chunks
-run on two lanes), so a launch's duration includes time shared with other
-kernels; `device_level` reports both traversal kernels' algorithmic bytes
-(shadow ray: 48 + 64 * node tests + 36 * triangle tests) over the frame wall
-time, a lower bound on the device's traversal bandwidth.
+roofline: the traversal kernel k_trace (one launch per bounce b traces the
+extension rays of bounce b and the shadow rays of bounce b - 1).  Algorithmic
+bytes per closest-hit ray = 28 (origin, direction, tmax) + 8 (t, id) + 64 *
+quad-node tests + 36 * triangle tests, per shadow ray 48 + 64 * node tests +
+36 * triangle tests; node and triangle tests are counted by an instrumented,
+untimed frame.  Three labelled figures:
+  - per launch (the `achieved` / `frac` of the line): bytes per launch / the
+    mean launch duration over the timed frames (HIP events on the lane's
+    stream around every launch).  A frame's two chunks run on two lanes, so
+    a launch shares the device with the other lane's kernels;
+  - isolated: the same bytes / the launch durations of a frame whose chunks
+    run one at a time (ct_trace_frame_serial, untimed by the wall clock):
+    each launch has the device to itself;
+  - device level: the frame's traversal bytes / the frame wall time (a lower
+    bound on the device's traversal bandwidth: the frame also shades).
 
 cpu_baseline: the CPU oracle (FP32 restatement of the reference loop, OpenMP
-over pixels) on whole frames of the same workload, about 10 s of CPU work.
+over pixels, OMP_NUM_THREADS threads or the process's CPU affinity) on whole
+frames of the same workload, about 10 s of CPU work; the host's core count,
+CPU model and OMP_NUM_THREADS are reported with it.
+
+c3 (with the default config 2): the 7.24M-triangle displaced sphere at
+3840x2160, depth 16, roulette from bounce 3 (the HBM-bound configuration):
+rays/s, the three roofline figures, frame 0 against the oracle on a
+3840x135 band through the middle, and a CPU baseline on that band.
 
 accuracy: frame 0 of the workload on the GPU against the oracle on a band of
 rows through the middle of the image (bit-identical fraction vs FP32,
@@ -77,6 +88,7 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-build", action="store_true", help="skip the device tree-build leg")
+    ap.add_argument("--no-c3", action="store_true", help="skip the 7.24M-triangle 4K leg of the default run")
     ap.add_argument("--build-only", action="store_true",
                     help="only the tree-build leg (device build vs host build of the workload's scene)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
@@ -119,71 +131,11 @@ def main() -> int:
     ctx.upload(scene)
     ctx.resize(W, H)
 
-    # instrumented frame: node / triangle tests of the extend and connect stages
-    cst, work = ctx.trace_frame_counted(cam, opts)
-    ctx.reset()
-    closest_rays = cst.primary_rays + cst.extension_rays
-    alg_bytes_frame = 36 * closest_rays + 64 * work[0] + 36 * work[1]
-    # shadow ray: 28 in + 12 colour + 4 pixel + 4 result, plus nodes and triangles
-    alg_bytes_shadow = 48 * cst.shadow_rays + 64 * work[2] + 36 * work[3]
-
-    for _ in range(args.warmup):
-        ctx.trace_frame(cam, opts)
-    ctx.reset()
-
-    rays = prim = ext = sh = 0
-    ext_ms = shade_ms = conn_ms = other_ms = gpu_ms = 0.0
-    launches = 0
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
-        st = ctx.trace_frame(cam, opts)
-        prim += st.primary_rays
-        ext += st.extension_rays
-        sh += st.shadow_rays
-        ext_ms += st.extend_ms
-        shade_ms += st.shade_ms
-        conn_ms += st.connect_ms
-        other_ms += st.other_ms
-        gpu_ms += st.gpu_ms
-        launches += st.extend_launches
-    t1 = time.perf_counter()
-    wall = t1 - t0
-    rays = prim + ext + sh
-    mrays = rays / wall / 1e6
-    ms_per_step = wall * 1e3 / args.steps
-
-    # roofline of the extend kernel: algorithmic bytes per launch / mean duration
-    closest_per_frame = (prim + ext) / args.steps
-    bytes_per_frame = alg_bytes_frame * (closest_per_frame / max(closest_rays, 1))
-    bytes_per_launch = bytes_per_frame * args.steps / max(launches, 1)
-    mean_launch_ms = ext_ms / max(launches, 1)
-    achieved = bytes_per_launch / (mean_launch_ms * 1e-3) / 1e9
-    roofline = {
-        "kernel": "k_extend (closest-hit traversal, quad BVH)",
-        "bound": "hbm",
-        "achieved": round(achieved, 1),
-        "peak": HBM_PEAK_GBS,
-        "unit": "GB/s",
-        "frac": round(achieved / HBM_PEAK_GBS, 4),
-        "traffic": measured_traffic(args.config, args.traversal),
-        "bytes_per_launch": int(bytes_per_launch),
-        "mean_launch_ms": round(mean_launch_ms, 4),
-        "node_tests_per_ray": round(work[0] / max(closest_rays, 1), 2),
-        "tri_tests_per_ray": round(work[1] / max(closest_rays, 1), 2),
-    }
-    # Traversal launches overlap: k_connect(b) runs beside k_extend(b + 1) on a
-    # second stream, and the frame's two chunks run on two lanes, so a
-    # launch's duration includes time shared with other kernels.  Device
-    # level: both traversal kernels' algorithmic bytes over the frame's wall
-    # time (a lower bound: the frame also contains shading).
-    shadow_per_frame = sh / args.steps
-    frame_bytes = bytes_per_frame + alg_bytes_shadow * (shadow_per_frame / max(cst.shadow_rays, 1))
-    dev = frame_bytes / (ms_per_step * 1e-3) / 1e9
-    roofline["device_level"] = {
-        "kernels": "k_extend + k_connect per frame over the frame wall time", "achieved": round(dev, 1),
-        "frac": round(dev / HBM_PEAK_GBS, 4), "bytes_per_frame": int(frame_bytes),
-        "shadow_node_tests_per_ray": round(work[2] / max(cst.shadow_rays, 1), 2),
-        "shadow_tri_tests_per_ray": round(work[3] / max(cst.shadow_rays, 1), 2)}
+    tm = timed_frames(ctx, cam, opts, args.steps, args.warmup)
+    wall, ms_per_step = tm["wall"], tm["ms_per_step"]
+    mrays = tm["rays"] / wall / 1e6
+    roofline = trace_roofline(ctx, cam, opts, tm, args.config, args.traversal)
+    prim, ext, sh = tm["primary"], tm["extension"], tm["shadow"]
 
     out = {
         "metric": METRIC,
@@ -207,9 +159,8 @@ def main() -> int:
         },
         "rays_per_step": {"primary": prim // args.steps, "extension": ext // args.steps, "shadow": sh // args.steps},
         "samples_per_s": round(W * H * args.steps / wall, 1),
-        "gpu_ms_per_step": round(gpu_ms / args.steps, 3),
-        "stage_ms_per_step": {"extend": round(ext_ms / args.steps, 3), "shade": round(shade_ms / args.steps, 3),
-                              "connect": round(conn_ms / args.steps, 3), "other": round(other_ms / args.steps, 3)},
+        "gpu_ms_per_step": round(tm["gpu_ms"] / args.steps, 3),
+        "stage_ms_per_step": {k: round(v / args.steps, 3) for k, v in tm["stage_ms"].items()},
         "roofline": roofline,
     }
 
@@ -222,6 +173,8 @@ def main() -> int:
     if not args.no_build:
         # replaces the scene with its device-built twin (identical trees and records)
         out["build"] = build_leg(ctx, scene, wide=args.traversal == "fast")
+    if args.config == 2 and not args.no_c3:
+        out["c3"] = c3_leg(ct, ctx, args)  # replaces the scene
     ctx.close()
     if not args.no_cpu and not args.no_build and args.config == 2:
         out["build"]["cpu_baseline"] = cpu_build_baseline(scene)
@@ -232,7 +185,108 @@ def main() -> int:
     return 0
 
 
-def accuracy_strip(ctx, scene, cam, opts, W, H, strip: int, frames: int = 16) -> dict:
+def timed_frames(ctx, cam, opts, steps: int, warmup: int) -> dict:
+    """warmup untimed frames, then `steps` frames timed by the wall clock
+    (each frame ends with the stream synchronisation of ct_trace_frame), with
+    the per-stage device times the library records on its streams."""
+    for _ in range(warmup):
+        ctx.trace_frame(cam, opts)
+    ctx.reset()
+    acc = {"primary": 0, "extension": 0, "shadow": 0, "launches": 0, "gpu_ms": 0.0,
+           "stage_ms": {"trace": 0.0, "shade": 0.0, "trace_last_shadow": 0.0, "other": 0.0}}
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        st = ctx.trace_frame(cam, opts)
+        acc["primary"] += st.primary_rays
+        acc["extension"] += st.extension_rays
+        acc["shadow"] += st.shadow_rays
+        acc["launches"] += st.extend_launches
+        acc["gpu_ms"] += st.gpu_ms
+        acc["stage_ms"]["trace"] += st.extend_ms
+        acc["stage_ms"]["shade"] += st.shade_ms
+        acc["stage_ms"]["trace_last_shadow"] += st.connect_ms
+        acc["stage_ms"]["other"] += st.other_ms
+    acc["wall"] = time.perf_counter() - t0
+    acc["steps"] = steps
+    acc["ms_per_step"] = acc["wall"] * 1e3 / steps
+    acc["rays"] = acc["primary"] + acc["extension"] + acc["shadow"]
+    return acc
+
+
+def trace_roofline(ctx, cam, opts, tm: dict, config: int, traversal: str) -> dict:
+    """The traversal kernel's algorithmic bytes (from an instrumented frame's
+    node and triangle test counts) against three times: the mean launch of
+    the timed frames, the launches of a serial frame (each alone on the
+    device) and the frame wall time."""
+    cst, work = ctx.trace_frame_counted(cam, opts)  # the same frame: identical counts (deterministic)
+    closest = cst.primary_rays + cst.extension_rays
+    bytes_frame = (36 * closest + 64 * work[0] + 36 * work[1]) + (48 * cst.shadow_rays + 64 * work[2] + 36 * work[3])
+    ser = ctx.trace_frame_serial(cam, opts)
+    ctx.reset()
+    steps = tm["steps"]
+    launches = tm["launches"] / steps
+    trace_ms = (tm["stage_ms"]["trace"] + tm["stage_ms"]["trace_last_shadow"]) / steps
+    bytes_per_launch = bytes_frame / max(launches, 1)
+    mean_launch_ms = trace_ms / max(launches, 1)
+    achieved = bytes_per_launch / (mean_launch_ms * 1e-3) / 1e9
+    iso_ms = ser.extend_ms + ser.connect_ms
+    iso = bytes_frame / (iso_ms * 1e-3) / 1e9
+    dev = bytes_frame / (tm["ms_per_step"] * 1e-3) / 1e9
+    return {
+        "kernel": "k_trace (closest hit of bounce b + any hit of bounce b-1, quad BVH)",
+        "bound": "hbm",
+        "achieved": round(achieved, 1),
+        "peak": HBM_PEAK_GBS,
+        "unit": "GB/s",
+        "frac": round(achieved / HBM_PEAK_GBS, 4),
+        "traffic": measured_traffic(config, traversal),
+        "label": "per launch: algorithmic bytes per k_trace launch / mean launch duration over the timed frames "
+                 "(two chunks in flight, so launches share the device)",
+        "bytes_per_launch": int(bytes_per_launch),
+        "mean_launch_ms": round(mean_launch_ms, 4),
+        "launches_per_frame": round(launches, 2),
+        "node_tests_per_ray": round(work[0] / max(closest, 1), 2),
+        "tri_tests_per_ray": round(work[1] / max(closest, 1), 2),
+        "shadow_node_tests_per_ray": round(work[2] / max(cst.shadow_rays, 1), 2),
+        "shadow_tri_tests_per_ray": round(work[3] / max(cst.shadow_rays, 1), 2),
+        "isolated": {"label": "the same bytes over the k_trace launches of a frame whose chunks run one at a time "
+                              "(each launch alone on the device)",
+                     "achieved": round(iso, 1), "frac": round(iso / HBM_PEAK_GBS, 4),
+                     "trace_ms_per_frame": round(iso_ms, 4), "launches": int(ser.extend_launches)},
+        "device_level": {"label": "the frame's traversal bytes over the frame wall time (the frame also shades)",
+                         "achieved": round(dev, 1), "frac": round(dev / HBM_PEAK_GBS, 4),
+                         "bytes_per_frame": int(bytes_frame)},
+    }
+
+
+def c3_leg(ct, ctx, args, steps: int = 10, warmup: int = 3) -> dict:
+    """The 7.24M-triangle displaced sphere at 4K, depth 16, roulette from
+    bounce 3, on the same context (trees built on the device: byte-identical
+    to the host build)."""
+    scene = ct.HostScene.generate(ct.SCENE_C3, args.seed, build=False)
+    ctx.upload_build(scene)
+    opts, W, H = scene.frame_opts()
+    opts.traversal = ct.TRAV_FAST if args.traversal == "fast" else ct.TRAV_EXACT
+    cam = scene.camera(W, H)
+    ctx.resize(W, H)
+    tm = timed_frames(ctx, cam, opts, steps, warmup)
+    out = {"workload": WORKLOADS[3] % (scene.stats()["triangles"], args.traversal),
+           "value": round(tm["rays"] / tm["wall"] / 1e6, 2), "unit": "Mrays/s", "steps": steps,
+           "ms_per_step": round(tm["ms_per_step"], 3),
+           "rays_per_step": {k: tm[k] // steps for k in ("primary", "extension", "shadow")},
+           "stage_ms_per_step": {k: round(v / steps, 3) for k, v in tm["stage_ms"].items()},
+           "roofline": trace_roofline(ctx, cam, opts, tm, 3, args.traversal)}
+    if not args.no_cpu:
+        trees = ctx.read_trees()
+        arrays = scene.arrays()
+        arrays.update(trees)
+        out["accuracy"] = accuracy_strip(ctx, None, cam, opts, W, H, strip=16, frames=1, arrays=arrays)
+        out["cpu_baseline"] = cpu_baseline(None, cam, opts, W, H, min(args.cpu_seconds, 5.0), strip=16,
+                                           arrays=arrays)
+    return out
+
+
+def accuracy_strip(ctx, scene, cam, opts, W, H, strip: int, frames: int = 16, arrays=None) -> dict:
     """Frame 0 of the workload on the GPU against the CPU oracle on a band of
     H / strip rows: bit-identical fraction against the FP32 restatement and
     rel-MSE (mean of (x - ref)^2 / (ref^2 + 1e-2)) against the FP64 one; then
@@ -241,13 +295,13 @@ def accuracy_strip(ctx, scene, cam, opts, W, H, strip: int, frames: int = 16) ->
     FP64 carries a single-sample error of order one, which averages out)."""
     import numpy as np
     from oracle import oracle as orc
-    threads = min(16, os.cpu_count() or 1)
+    threads = _cpu_threads()
     rows = H // strip
     r0 = (H - rows) // 2  # a band through the middle of the image
     ctx.resize(W, H)
     ctx.trace_frame(cam, opts, stats=False)
     acc, _ = ctx.read_accum()
-    osc = orc.Scene(scene.arrays())
+    osc = _oracle_scene(orc, scene, arrays)
     ref32 = np.zeros((H, W, 4), np.float32)
     ref64 = np.zeros((H, W, 4), np.float64)
     osc.render(cam, opts, W, H, 0, 1, rows=(r0, r0 + rows), threads=threads, accum=ref32)
@@ -256,6 +310,10 @@ def accuracy_strip(ctx, scene, cam, opts, W, H, strip: int, frames: int = 16) ->
     same = float(np.mean(g.view(np.uint32) == ref32[r0:r0 + rows].view(np.uint32)))
     x, r = g[..., :3].astype(np.float64), ref64[r0:r0 + rows, :, :3]
     rel = float(np.mean((x - r) ** 2 / (r ** 2 + 1e-2)))
+    res = {"rows": f"{r0}..{r0 + rows - 1} of {H}", "frame": 0, "bit_identical_vs_fp32_oracle": round(same, 6),
+           "rel_mse_vs_fp64_oracle": float(f"{rel:.3e}"), "nonzero_pixels": int(np.sum(np.any(x > 0, axis=2)))}
+    if frames <= 1:
+        return res
     # converged: the mean of `frames` accumulated frames against the FP64
     # oracle's mean of the same samples
     for _ in range(frames - 1):
@@ -267,9 +325,47 @@ def accuracy_strip(ctx, scene, cam, opts, W, H, strip: int, frames: int = 16) ->
     xc = acc[r0:r0 + rows, :, :3].astype(np.float64) / frames
     rc = ref64[r0:r0 + rows, :, :3] / frames
     relc = float(np.mean((xc - rc) ** 2 / (rc ** 2 + 1e-2)))
-    return {"rows": f"{r0}..{r0 + rows - 1} of {H}", "frame": 0, "bit_identical_vs_fp32_oracle": round(same, 6),
-            "rel_mse_vs_fp64_oracle": float(f"{rel:.3e}"), "nonzero_pixels": int(np.sum(np.any(x > 0, axis=2))),
-            "converged": {"frames": frames, "rel_mse_vs_fp64_oracle": float(f"{relc:.3e}")}}
+    res["converged"] = {"frames": frames, "rel_mse_vs_fp64_oracle": float(f"{relc:.3e}")}
+    return res
+
+
+def _oracle_scene(orc, scene, arrays=None):
+    """The oracle's twin of a scene; given arrays with trees (a device build
+    read back), it traverses those instead of rebuilding."""
+    if arrays is None:
+        return orc.Scene(scene.arrays())
+    osc = orc.Scene(arrays, build=False)
+    osc.set_trees(arrays["bvh"], arrays["mbvh"], arrays["prim_indices"])
+    return osc
+
+
+def _cpu_threads() -> int:
+    """The CPU baseline's threads: OMP_NUM_THREADS if set, else the CPUs this
+    process may run on."""
+    env = os.environ.get("OMP_NUM_THREADS", "")
+    if env.isdigit() and int(env) > 0:
+        return int(env)
+    try:
+        return len(os.sched_getaffinity(0))
+    except AttributeError:
+        return os.cpu_count() or 1
+
+
+def _host_info() -> dict:
+    model = ""
+    try:
+        for ln in open("/proc/cpuinfo"):
+            if ln.startswith("model name"):
+                model = ln.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    try:
+        aff = len(os.sched_getaffinity(0))
+    except AttributeError:
+        aff = None
+    return {"nproc": os.cpu_count(), "affinity_cpus": aff, "cpu_model": model,
+            "OMP_NUM_THREADS": os.environ.get("OMP_NUM_THREADS")}
 
 
 def measured_traffic(config: int, traversal: str):
@@ -540,15 +636,16 @@ def cpu_build_baseline(scene) -> dict:
             "sample": f"one build of the {scene.stats()['triangles']}-triangle scene"}
 
 
-def cpu_baseline(scene, cam, opts, W, H, min_seconds: float, max_frames: int = 256, strip: int = 1) -> dict:
+def cpu_baseline(scene, cam, opts, W, H, min_seconds: float, max_frames: int = 256, strip: int = 1,
+                 arrays=None) -> dict:
     """Oracle FP32 on the benchmark frames (frame indices 0, 1, ...) until at
     least min_seconds of CPU work, at most max_frames frames.  strip > 1
     renders a band of H / strip rows through the middle of each frame only
     (a bounded sample of the 4K workload)."""
     import numpy as np
     from oracle import oracle as orc
-    threads = min(16, os.cpu_count() or 1)
-    osc = orc.Scene(scene.arrays())
+    threads = _cpu_threads()
+    osc = _oracle_scene(orc, scene, arrays)
     rows = H // strip
     r0 = (H - rows) // 2  # a band through the middle of the image
     acc = np.zeros((H, W, 4), np.float32)
@@ -564,7 +661,8 @@ def cpu_baseline(scene, cam, opts, W, H, min_seconds: float, max_frames: int = 2
     what = f"full {W}x{H} frames" if strip == 1 else f"{W}x{rows} strips (rows {r0}..{r0 + rows - 1}) of {W}x{H} frames"
     return {"value": round(rays / dt / 1e6, 3), "unit": "Mrays/s", "cores": threads, "kind": "port",
             "sample": f"{frames} {what} at 1 spp (frame indices 0..{frames - 1}), {rays} rays, "
-                      f"{dt:.1f} s on {threads} OpenMP threads; oracle built -O3 -march=x86-64-v3 without fast-math"}
+                      f"{dt:.1f} s on {threads} OpenMP threads; oracle built -O3 -march=x86-64-v3 without fast-math",
+            "host": _host_info()}
 
 
 if __name__ == "__main__":

@@ -173,6 +173,7 @@ _sig = {
     "ct_trace_frame": (C.c_int, [_P, C.POINTER(Camera), C.POINTER(FrameOpts), C.POINTER(FrameStats)]),
     "ct_trace_frame_counted": (C.c_int, [_P, C.POINTER(Camera), C.POINTER(FrameOpts), C.POINTER(FrameStats),
                                          C.POINTER(C.c_uint64)]),
+    "ct_trace_frame_serial": (C.c_int, [_P, C.POINTER(Camera), C.POINTER(FrameOpts), C.POINTER(FrameStats)]),
     "ct_read_accum": (C.c_int, [_P, C.POINTER(C.c_float), C.c_size_t, C.POINTER(C.c_int32)]),
     "ct_write_accum": (C.c_int, [_P, C.POINTER(C.c_float), C.c_size_t, C.c_int32]),
     "ct_read_image": (C.c_int, [_P, C.POINTER(C.c_float), C.c_size_t]),
@@ -482,6 +483,13 @@ class Context:
         w = (C.c_uint64 * 4)()
         _check(lib.ct_trace_frame_counted(self._h, C.byref(cam), C.byref(opts), C.byref(st), w), self._h)
         return st, list(w)
+
+    def trace_frame_serial(self, cam: Camera, opts: FrameOpts):
+        """A frame with its chunks one at a time and every kernel timed
+        (each kernel's own device time, for measurement)."""
+        st = FrameStats()
+        _check(lib.ct_trace_frame_serial(self._h, C.byref(cam), C.byref(opts), C.byref(st)), self._h)
+        return st
 
     def read_accum(self):
         out = np.empty((self.height, self.width, 4), np.float32)

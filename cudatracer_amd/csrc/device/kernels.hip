@@ -3,9 +3,10 @@
 // Reference stages (src/CUDA/Kernel.cu):        here:
 //   setGlobals          :75-86                  k_setup (all bounce counters zeroed once per chunk)
 //   generatePrimaryRays :208-234                k_generate
-//   extend              :236-245                k_extend   (persistent, wave-fetched)
+//   extend              :236-245                k_trace    (persistent, wave-fetched; bounce b's
+//   connect             :247-256                            extension rays with bounce b - 1's
+//                                                           shadow rays in one launch)
 //   shade               :88-206                 k_shade    (ballot + block prefix-sum queue append)
-//   connect             :247-256                k_connect  (persistent, wave-fetched)
 //   accumulation        :100,134,255            k_resolve  (per-path radiance summed in sample order)
 //   draw_framebuffer    :258-270                k_tonemap
 //
@@ -89,21 +90,23 @@ __global__ void k_generate(const ViewDesc* __restrict__ views, const Segment* __
 }
 
 // --------------------------------------------------------------- extend --
-// Persistent closest-hit over the bounce's queue (see trace_persistent).
-template <int ACCEL, bool COUNT>
+// Persistent closest-hit over a queue of rays (see trace_persistent): the
+// ray-level entry points.
+template <int ACCEL>
 CT_DEV void extend_impl(const DevScene& sc, const TraceOpts& to, const Queues& q, int sel, int bounce,
-                        LaneStack& st, TravCount& tc) {
+                        LaneStack& st) {
     const uint32_t n = *cnt_active(q.counters, bounce);
     const float4* __restrict__ ro = q.ray_o[sel];
     const float4* __restrict__ rd = q.ray_d[sel];
     float2* hit = q.hit;
-    trace_persistent<ACCEL, false, COUNT>(
-        sc, to.teps, n, cnt_fetch_ext(q.counters, bounce), st, tc, q.err,
-        [&](uint32_t i) {
+    TravCount tc{0u, 0u, 0u, 0u};
+    trace_persistent<ACCEL, kClosest, false>(
+        sc, to.teps, n, 0u, cnt_fetch_ext(q.counters, bounce), st, tc, q.err,
+        [&](uint32_t i, bool) {
             const float4 o4 = ro[i], d4 = rd[i];
             return RayIn{xyz(o4), xyz(d4), 1e34f};
         },
-        [&](uint32_t i, float t, int prim) {
+        [&](uint32_t i, bool, float t, int prim) {
             if (prim == -2) { t = 1e34f; prim = -1; }
             hit[i] = make_float2(t, __int_as_float(prim));
         });
@@ -128,55 +131,67 @@ CT_DEV LaneStack make_stack(const Queues& q, int lds_depth) {
     return st;
 }
 
-// One kernel per tree and counting mode, so each gets its own register
+// One kernel per tree (and counting mode), so each gets its own register
 // budget.  Eight waves per SIMD (64 VGPRs) is the occupancy the traversal
 // needs to hide its memory latency; asked for explicitly, the scheduler
 // reaches it without spills (the counting copies spill a register).
-template <int TREE, bool COUNT>
-__global__ void __attribute__((amdgpu_waves_per_eu(8))) k_extend(DevScene sc, TraceOpts to, Queues q, int sel, int bounce, int lds_depth,
-                         unsigned long long* wstats) {
+template <int TREE>
+__global__ void __attribute__((amdgpu_waves_per_eu(8)))
+k_extend(DevScene sc, TraceOpts to, Queues q, int sel, int bounce, int lds_depth) {
     LaneStack st = make_stack(q, lds_depth);
-    TravCount tc{0u, 0u};
-    extend_impl<TREE, COUNT>(sc, to, q, sel, bounce, st, tc);
-    if (COUNT) {
-        atomicAdd(&wstats[0], (unsigned long long)tc.nodes);
-        atomicAdd(&wstats[1], (unsigned long long)tc.tris);
-    }
+    extend_impl<TREE>(sc, to, q, sel, bounce, st);
 }
 
-// -------------------------------------------------------------- connect --
-// Persistent any-hit over the bounce's shadow queue; an unoccluded ray's
-// colour is its path's radiance term of this bounce (Kernel.cu:247-256),
-// stored for shade(bounce + 2) or k_resolve to add in order.
+// ---------------------------------------------------------------- trace --
+// The frame loop's traversal launch of bounce b: the extension rays of
+// bounce b (extend, Kernel.cu:236-245: closest hit into q.hit) and the
+// shadow rays of bounce b - 1 (connect, Kernel.cu:247-256: any hit), one
+// persistent launch for both queues.  An unoccluded shadow ray's colour is
+// its path's radiance term of bounce b - 1, stored for shade(b + 1) or
+// k_resolve to add in order.  Bounce 0 has no shadow queue; the launch after
+// the last bounce (b = max_depth) has no extension rays.
 template <int ACCEL, bool COUNT>
-CT_DEV void connect_impl(const DevScene& sc, const TraceOpts& to, const Queues& q, const ChunkHdr* hdr, int bounce,
-                         LaneStack& st, TravCount& tc) {
-    const uint32_t n = *cnt_shadow(q.counters, bounce);
-    const uint32_t stamp = term_stamp(hdr->gen, bounce);
-    trace_persistent<ACCEL, true, COUNT>(
-        sc, to.teps, n, cnt_fetch_sh(q.counters, bounce), st, tc, q.err,
-        [&](uint32_t i) {
-            const float4 o4 = q.sh_o[bounce & 1][i], d4 = q.sh_d[bounce & 1][i];
-            return RayIn{xyz(o4), xyz(d4), o4.w};
+CT_DEV void trace_impl(const DevScene& sc, const TraceOpts& to, const Queues& q, const ChunkHdr* hdr, int sel,
+                       int bounce, LaneStack& st, TravCount& tc) {
+    const uint32_t n = *cnt_active(q.counters, bounce);
+    const int sb = bounce - 1;  // the shadow rays' bounce
+    const uint32_t n2 = bounce > 0 ? *cnt_shadow(q.counters, sb) : 0u;
+    const uint32_t stamp = term_stamp(hdr->gen, sb);
+    const float4* __restrict__ ro = q.ray_o[sel];
+    const float4* __restrict__ rd = q.ray_d[sel];
+    const float4* __restrict__ so = q.sh_o[sb & 1];
+    const float4* __restrict__ sd = q.sh_d[sb & 1];
+    trace_persistent<ACCEL, kDual, COUNT>(
+        sc, to.teps, n, n2, cnt_fetch_ext(q.counters, bounce), st, tc, q.err,
+        [&](uint32_t i, bool any) {
+            const float4 o4 = any ? so[i] : ro[i], d4 = any ? sd[i] : rd[i];
+            return RayIn{xyz(o4), xyz(d4), any ? o4.w : 1e34f};
         },
-        [&](uint32_t i, float, int prim) {
+        [&](uint32_t i, bool any, float t, int prim) {
+            if (!any) {
+                if (prim == -2) { t = 1e34f; prim = -1; }
+                q.hit[i] = make_float2(t, __int_as_float(prim));
+                return;
+            }
             if (prim != -1) return;  // occluded (or stack overflow, flagged)
-            const uint32_t slot = __float_as_uint(q.sh_d[bounce & 1][i].w);
-            const float4 c = q.sh_c[bounce & 1][i];
-            q.nbuf[bounce & 1][slot] = make_float4(c.x, c.y, c.z, __uint_as_float(stamp));
+            const uint32_t slot = __float_as_uint(sd[i].w);
+            const float4 c = q.sh_c[sb & 1][i];
+            q.nbuf[sb & 1][slot] = make_float4(c.x, c.y, c.z, __uint_as_float(stamp));
         });
 }
 
 template <int TREE, bool COUNT>
 __global__ void __attribute__((amdgpu_waves_per_eu(8)))
-k_connect(DevScene sc, TraceOpts to, Queues q, const ChunkHdr* hdr, int bounce, int lds_depth,
-          unsigned long long* wstats) {
+k_trace(DevScene sc, TraceOpts to, Queues q, const ChunkHdr* hdr, int sel, int bounce, int lds_depth,
+        unsigned long long* wstats) {
     LaneStack st = make_stack(q, lds_depth);
-    TravCount tc{0u, 0u};
-    connect_impl<TREE, COUNT>(sc, to, q, hdr, bounce, st, tc);
+    TravCount tc{0u, 0u, 0u, 0u};
+    trace_impl<TREE, COUNT>(sc, to, q, hdr, sel, bounce, st, tc);
     if (COUNT) {
-        atomicAdd(&wstats[2], (unsigned long long)tc.nodes);
-        atomicAdd(&wstats[3], (unsigned long long)tc.tris);
+        atomicAdd(&wstats[0], (unsigned long long)tc.nodes);
+        atomicAdd(&wstats[1], (unsigned long long)tc.tris);
+        atomicAdd(&wstats[2], (unsigned long long)tc.snodes);
+        atomicAdd(&wstats[3], (unsigned long long)tc.stris);
     }
 }
 
@@ -604,14 +619,14 @@ __global__ void k_tonemap(const float4* __restrict__ accum, uint32_t n, float sa
 template <int ACCEL>
 CT_DEV void flags_impl(const DevScene& sc, const TraceOpts& to, const Queues& q, uint32_t n, LaneStack& st,
                        uint8_t* out) {
-    TravCount tc{0u, 0u};
-    trace_persistent<ACCEL, true, false>(
-        sc, to.teps, n, cnt_fetch_sh(q.counters, 0), st, tc, q.err,
-        [&](uint32_t i) {
+    TravCount tc{0u, 0u, 0u, 0u};
+    trace_persistent<ACCEL, kAny, false>(
+        sc, to.teps, n, 0u, cnt_fetch_sh(q.counters, 0), st, tc, q.err,
+        [&](uint32_t i, bool) {
             const float4 o4 = q.sh_o[0][i], d4 = q.sh_d[0][i];
             return RayIn{xyz(o4), xyz(d4), o4.w};
         },
-        [&](uint32_t i, float, int prim) { out[i] = prim >= 0 ? 1 : 0; });
+        [&](uint32_t i, bool, float, int prim) { out[i] = prim >= 0 ? 1 : 0; });
 }
 
 template <int TREE>
@@ -642,41 +657,39 @@ int grid_of(const LaunchShape& ls, int tree) { return tree == 2 ? ls.grid2 : (tr
 
 template <int TREE>
 void extend_t(const LaunchShape& ls, const DevScene& sc, const TraceOpts& to, const Queues& q, int sel, int bounce,
-              unsigned long long* wstats, hipStream_t s) {
+              hipStream_t s) {
     const size_t lds = (size_t)ls.lds_depth * ls.block * 4;
     const dim3 grid(grid_of(ls, TREE)), block(ls.block);
-    if (wstats)
-        hipLaunchKernelGGL((k_extend<TREE, true>), grid, block, lds, s, sc, to, q, sel, bounce, ls.lds_depth, wstats);
-    else
-        hipLaunchKernelGGL((k_extend<TREE, false>), grid, block, lds, s, sc, to, q, sel, bounce, ls.lds_depth, wstats);
+    hipLaunchKernelGGL((k_extend<TREE>), grid, block, lds, s, sc, to, q, sel, bounce, ls.lds_depth);
 }
 template <int TREE>
-void connect_t(const LaunchShape& ls, const DevScene& sc, const TraceOpts& to, const Queues& q, const ChunkHdr* hdr,
-               int bounce, unsigned long long* wstats, hipStream_t s) {
+void trace_t(const LaunchShape& ls, const DevScene& sc, const TraceOpts& to, const Queues& q, const ChunkHdr* hdr,
+             int sel, int bounce, unsigned long long* wstats, hipStream_t s) {
     const size_t lds = (size_t)ls.lds_depth * ls.block * 4;
     const dim3 grid(grid_of(ls, TREE)), block(ls.block);
     if (wstats)
-        hipLaunchKernelGGL((k_connect<TREE, true>), grid, block, lds, s, sc, to, q, hdr, bounce, ls.lds_depth, wstats);
+        hipLaunchKernelGGL((k_trace<TREE, true>), grid, block, lds, s, sc, to, q, hdr, sel, bounce, ls.lds_depth,
+                           wstats);
     else
-        hipLaunchKernelGGL((k_connect<TREE, false>), grid, block, lds, s, sc, to, q, hdr, bounce, ls.lds_depth,
+        hipLaunchKernelGGL((k_trace<TREE, false>), grid, block, lds, s, sc, to, q, hdr, sel, bounce, ls.lds_depth,
                            wstats);
 }
 
 hipError_t launch_extend(const LaunchShape& ls, const DevScene& sc, const TraceOpts& to, const Queues& q, int sel,
-                         int bounce, unsigned long long* wstats, hipStream_t s) {
+                         int bounce, hipStream_t s) {
     switch (tree_of(to)) {
-        case 2: extend_t<2>(ls, sc, to, q, sel, bounce, wstats, s); break;
-        case 4: extend_t<4>(ls, sc, to, q, sel, bounce, wstats, s); break;
-        default: extend_t<8>(ls, sc, to, q, sel, bounce, wstats, s); break;
+        case 2: extend_t<2>(ls, sc, to, q, sel, bounce, s); break;
+        case 4: extend_t<4>(ls, sc, to, q, sel, bounce, s); break;
+        default: extend_t<8>(ls, sc, to, q, sel, bounce, s); break;
     }
     return hipGetLastError();
 }
-hipError_t launch_connect(const LaunchShape& ls, const DevScene& sc, const TraceOpts& to, const Queues& q,
-                          const ChunkHdr* hdr, int bounce, unsigned long long* wstats, hipStream_t s) {
+hipError_t launch_trace(const LaunchShape& ls, const DevScene& sc, const TraceOpts& to, const Queues& q,
+                        const ChunkHdr* hdr, int sel, int bounce, unsigned long long* wstats, hipStream_t s) {
     switch (tree_of(to)) {
-        case 2: connect_t<2>(ls, sc, to, q, hdr, bounce, wstats, s); break;
-        case 4: connect_t<4>(ls, sc, to, q, hdr, bounce, wstats, s); break;
-        default: connect_t<8>(ls, sc, to, q, hdr, bounce, wstats, s); break;
+        case 2: trace_t<2>(ls, sc, to, q, hdr, sel, bounce, wstats, s); break;
+        case 4: trace_t<4>(ls, sc, to, q, hdr, sel, bounce, wstats, s); break;
+        default: trace_t<8>(ls, sc, to, q, hdr, sel, bounce, wstats, s); break;
     }
     return hipGetLastError();
 }
@@ -783,14 +796,14 @@ hipError_t launch_shadow_flags(const LaunchShape& ls, const DevScene& sc, const 
 
 namespace ctk {
 // Resident blocks per CU of the persistent traversal kernels of one tree
-// (extend and connect, the smaller of the two) for a block size and LDS depth.
+// (extend and trace, the smaller of the two) for a block size and LDS depth.
 template <int TREE>
 static int per_cu(int block, size_t lds) {
     int nb = 0, nc = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, reinterpret_cast<const void*>(&ctd::k_extend<TREE, false>),
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, reinterpret_cast<const void*>(&ctd::k_extend<TREE>),
                                                      block, lds) != hipSuccess)
         return 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nc, reinterpret_cast<const void*>(&ctd::k_connect<TREE, false>),
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nc, reinterpret_cast<const void*>(&ctd::k_trace<TREE, false>),
                                                      block, lds) != hipSuccess)
         return 0;
     return nb < nc ? nb : nc;

@@ -26,11 +26,18 @@ bool set_trace_grids(LaunchShape& ls, int num_cus);
 hipError_t launch_setup(uint32_t* counters, const ctd::ChunkHdr* hdr, hipStream_t s);
 hipError_t launch_generate(const LaunchShape& ls, const ctd::ViewDesc* views, const ctd::Segment* segs,
                            const ctd::ChunkHdr* hdr, uint32_t max_n, const ctd::Queues& q, hipStream_t s);
+// Closest hit for the rays [0, active[bounce]) of q.ray_o/ray_d[sel] into
+// q.hit (the ray-level entry points).
 hipError_t launch_extend(const LaunchShape& ls, const ctd::DevScene& sc, const ctd::TraceOpts& to,
-                         const ctd::Queues& q, int sel, int bounce, unsigned long long* wstats, hipStream_t s);
-hipError_t launch_connect(const LaunchShape& ls, const ctd::DevScene& sc, const ctd::TraceOpts& to,
-                          const ctd::Queues& q, const ctd::ChunkHdr* hdr, int bounce, unsigned long long* wstats,
-                          hipStream_t s);
+                         const ctd::Queues& q, int sel, int bounce, hipStream_t s);
+// The frame loop's traversal of bounce b: closest hit for the extension rays
+// of bounce b and any hit for the shadow rays of bounce b - 1 in one launch
+// (bounce = max_depth: the last bounce's shadow rays).  wstats (may be null)
+// accumulates closest-hit node / triangle tests and any-hit node / triangle
+// tests.
+hipError_t launch_trace(const LaunchShape& ls, const ctd::DevScene& sc, const ctd::TraceOpts& to,
+                        const ctd::Queues& q, const ctd::ChunkHdr* hdr, int sel, int bounce,
+                        unsigned long long* wstats, hipStream_t s);
 hipError_t launch_shade(const LaunchShape& ls, const ctd::DevScene& sc, const ctd::ShadeOpts& so,
                         const ctd::ViewDesc* views, const ctd::Segment* segs, const ctd::ChunkHdr* hdr,
                         const ctd::Queues& q, int sel, int bounce, hipStream_t s);

@@ -20,7 +20,8 @@
 //     branch-free stores at the stack top (a store for a culled child is
 //     overwritten by the next one); entries beyond the LDS depth spill to
 //     the block's global region, laid out the same way.  The builder's depth limit (64) bounds the
-//     stack at 97 entries; kMaxStack guards user-supplied trees.
+//     stack at 97 entries; the stack bound (Queues::max_stack, 128 by
+//     default) guards user-supplied trees.
 #pragma once
 
 #include "dev_common.h"
@@ -32,9 +33,10 @@ namespace ctd {
 CT_DEV uint32_t fbits(float f) { return __float_as_uint(f); }
 CT_DEV uint32_t lane_id() { return __lane_id(); }
 
-// Optional per-ray work counters (node tests, triangle tests).
+// Optional per-lane work counters: node and triangle tests of closest-hit
+// rays, and of any-hit rays.
 struct TravCount {
-    uint32_t nodes, tris;
+    uint32_t nodes, tris, snodes, stris;
 };
 
 // Moller-Trumbore with precomputed edges (edge1 = p1 - p0 and edge2 = p2 - p0
@@ -60,7 +62,7 @@ CT_DEV bool tri_test(f3 o, f3 d, float& t, f3 p0, f3 edge1, f3 edge2, float eps)
 
 CT_DEV float tag(float v, uint32_t slot) { return __uint_as_float((fbits(v) & 0xFFFFFFFCu) | slot); }
 
-// Per-lane stack: LDS positions [0, depth), global spill [depth, kMaxStack).
+// Per-lane stack: LDS positions [0, depth), global spill [depth, max_stack).
 // Both are laid out [position][thread]: position p of thread i is at byte
 // 4 * (p * blockDim + i) of the LDS stack or, spilled, at byte
 // 4 * ((p - depth) * blockDim + i) of the block's spill region.  The stack is
@@ -73,7 +75,7 @@ struct LaneStack {
     uint32_t top;      // p * stride4 + 4 * threadIdx.x for the next free position p
     uint32_t stride4;  // 4 * blockDim.x
     uint32_t lds_end;  // depth * stride4: positions from here on are spilled
-    uint32_t max_end;  // kMaxStack * stride4
+    uint32_t max_end;  // max_stack * stride4
     __amdgpu_buffer_rsrc_t spill;  // this block's spill region
     bool overflow;
 
@@ -85,7 +87,7 @@ struct LaneStack {
         overflow = false;
     }
     CT_DEV bool empty() const { return top < stride4; }
-    // Store at byte a: LDS, spill region, or dropped beyond kMaxStack (push4
+    // Store at byte a: LDS, spill region, or dropped beyond max_stack (push4
     // flags the overflow).
     CT_DEV void put(uint32_t a, uint32_t v) {
         if (a < lds_end) lds(a) = v;
@@ -116,7 +118,7 @@ struct LaneStack {
             put(a1, v1);
             put(a2, v2);
             put(a3, v3);
-            if (top >= max_end + stride4) {  // a position at or beyond kMaxStack was needed
+            if (top >= max_end + stride4) {  // a position at or beyond max_stack was needed
                 overflow = true;
                 top = max_end + 4u * threadIdx.x;
             }
@@ -125,21 +127,17 @@ struct LaneStack {
 };
 
 // Push the hit children of a quad node: sorted by the tagged tmin (nearest
-// popped first) with the reference's 5-comparator network, or in slot order.
-template <bool SORT>
-CT_DEV void push_children(uint32_t hm, float m0, float m1, float m2, float m3, uint32_t e0, uint32_t e1,
+// popped first) with the reference's 5-comparator network, or (sort false,
+// any-hit queries) in slot order with slot 0 popped first.
+CT_DEV void push_children(bool sort, uint32_t hm, float m0, float m1, float m2, float m3, uint32_t e0, uint32_t e1,
                           uint32_t e2, uint32_t e3, LaneStack& st) {
     if (hm == 0u) return;
-    if (!SORT) {
-        st.push4(e0, hm & 1u, e1, (hm >> 1) & 1u, e2, (hm >> 2) & 1u, e3, (hm >> 3) & 1u);
-        return;
-    }
     float s0 = m0, s1 = m1, s2 = m2, s3 = m3, tmp;
-    if (s0 > s1) { tmp = s0; s0 = s1; s1 = tmp; }
-    if (s2 > s3) { tmp = s2; s2 = s3; s3 = tmp; }
-    if (s0 > s2) { tmp = s0; s0 = s2; s2 = tmp; }
-    if (s1 > s3) { tmp = s1; s1 = s3; s3 = tmp; }
-    if (s2 > s3) { tmp = s2; s2 = s3; s3 = tmp; }
+    if (sort && s0 > s1) { tmp = s0; s0 = s1; s1 = tmp; }
+    if (sort && s2 > s3) { tmp = s2; s2 = s3; s3 = tmp; }
+    if (sort && s0 > s2) { tmp = s0; s0 = s2; s2 = tmp; }
+    if (sort && s1 > s3) { tmp = s1; s1 = s3; s3 = tmp; }
+    if (sort && s2 > s3) { tmp = s2; s2 = s3; s3 = tmp; }
     auto entry = [&](uint32_t slot) { return slot == 0 ? e0 : (slot == 1 ? e1 : (slot == 2 ? e2 : e3)); };
     const uint32_t q3 = fbits(s3) & 3u, q2 = fbits(s2) & 3u, q1 = fbits(s1) & 3u, q0 = fbits(s0) & 3u;
     st.push4(entry(q3), (hm >> q3) & 1u, entry(q2), (hm >> q2) & 1u, entry(q1), (hm >> q1) & 1u, entry(q0),
@@ -148,11 +146,11 @@ CT_DEV void push_children(uint32_t hm, float m0, float m1, float m2, float m3, u
 
 // Quad node test (MBVHNode::intersect): four slabs in the reference's float
 // order, tmin tagged with the slot, the 5-comparator network, then pushes in
-// the order of traverseMBVH (sorted position 3 first).  SORT = false skips
+// the order of traverseMBVH (sorted position 3 first).  sort = false skips
 // the network: an any-hit query's result does not depend on the visit order
 // (its t never shrinks, so every culling decision is the same).
-template <bool SORT>
-CT_DEV void node4_step(const float4* __restrict__ nodes4, uint32_t e, f3 o, f3 inv, float t, LaneStack& st) {
+CT_DEV void node4_step(bool sort, const float4* __restrict__ nodes4, uint32_t e, f3 o, f3 inv, float t,
+                       LaneStack& st) {
     const float4* n = nodes4 + 8 * (size_t)e;
     const float4 bminx = n[0], bmaxx = n[1], bminy = n[2], bmaxy = n[3], bminz = n[4], bmaxz = n[5];
     const float4 ch = n[6], cn = n[7];
@@ -187,7 +185,7 @@ CT_DEV void node4_step(const float4* __restrict__ nodes4, uint32_t e, f3 o, f3 i
     const uint32_t e1 = c1 == -1 ? fbits(ch.y) : (kLeafBit | fbits(ch.y));
     const uint32_t e2 = c2 == -1 ? fbits(ch.z) : (kLeafBit | fbits(ch.z));
     const uint32_t e3 = c3 == -1 ? fbits(ch.w) : (kLeafBit | fbits(ch.w));
-    push_children<SORT>(hm, m0, m1, m2, m3, e0, e1, e2, e3, st);
+    push_children(sort, hm, m0, m1, m2, m3, e0, e1, e2, e3, st);
 }
 
 // Fast quad node test: the same tagged entry distances, hit flags and push
@@ -229,9 +227,10 @@ CT_DEV bool fast_ray(f3 o, f3 inv, uint32_t plane_flags) {
 }
 
 // nodes4f: the tree in the fast layout (k_prepare_quad): six plane rows and
-// the four stack entries, seven loads per node.
-template <bool SORT>
-CT_DEV void node4_fast(const float4* __restrict__ nodes4f, uint32_t e, f3 o, f3 inv, float t, LaneStack& st) {
+// the four stack entries, seven loads per node.  sort = false (any hit):
+// the comparators are off, slot 0 is popped first.
+CT_DEV void node4_fast(bool sort, const float4* __restrict__ nodes4f, uint32_t e, f3 o, f3 inv, float t,
+                       LaneStack& st) {
     const float4* n = nodes4f + 8 * (size_t)e;
     const float4 bminx = n[0], bmaxx = n[1], bminy = n[2], bmaxy = n[3], bminz = n[4], bmaxz = n[5];
     const float4 ent = n[6];
@@ -277,22 +276,18 @@ CT_DEV void node4_fast(const float4* __restrict__ nodes4f, uint32_t e, f3 o, f3 
     // kEmptyEntry), so the count test is the entry test: the sorted path
     // carries kNoEntry through the network and never pushes it.
     const uint32_t e0 = fbits(ent.x), e1 = fbits(ent.y), e2 = fbits(ent.z), e3 = fbits(ent.w);
-    const bool h0 = x0 > 0.0f && m0 <= x0 && m0 < t && (SORT || e0 != kNoEntry);
-    const bool h1 = x1 > 0.0f && m1 <= x1 && m1 < t && (SORT || e1 != kNoEntry);
-    const bool h2 = x2 > 0.0f && m2 <= x2 && m2 < t && (SORT || e2 != kNoEntry);
-    const bool h3 = x3 > 0.0f && m3 <= x3 && m3 < t && (SORT || e3 != kNoEntry);
+    const bool h0 = x0 > 0.0f && m0 <= x0 && m0 < t && (sort || e0 != kNoEntry);
+    const bool h1 = x1 > 0.0f && m1 <= x1 && m1 < t && (sort || e1 != kNoEntry);
+    const bool h2 = x2 > 0.0f && m2 <= x2 && m2 < t && (sort || e2 != kNoEntry);
+    const bool h3 = x3 > 0.0f && m3 <= x3 && m3 < t && (sort || e3 != kNoEntry);
     if (!(h0 || h1 || h2 || h3)) return;
-    if (!SORT) {
-        st.push4(e0, h0, e1, h1, e2, h2, e3, h3);
-        return;
-    }
     // comparators (0,1) and (2,3) on keys and entries, then (0,2) and (1,3)
-    // on entries (their keys are not compared again)
+    // on entries (their keys are not compared again); none without sort
     uint32_t q0 = h0 ? e0 : kNoEntry, q1 = h1 ? e1 : kNoEntry, q2 = h2 ? e2 : kNoEntry, q3 = h3 ? e3 : kNoEntry;
-    const bool s01 = m0 > m1, s23 = m2 > m3;
+    const bool s01 = sort && m0 > m1, s23 = sort && m2 > m3;
     const float a = s01 ? m1 : m0, b = s01 ? m0 : m1, c = s23 ? m3 : m2, d = s23 ? m2 : m3;
     const uint32_t ea = s01 ? q1 : q0, eb = s01 ? q0 : q1, ec = s23 ? q3 : q2, ed = s23 ? q2 : q3;
-    const bool s02 = a > c, s13 = b > d;
+    const bool s02 = sort && a > c, s13 = sort && b > d;
     q0 = s02 ? ec : ea;
     q2 = s02 ? ea : ec;
     q1 = s13 ? ed : eb;
@@ -306,9 +301,9 @@ CT_DEV void node4_fast(const float4* __restrict__ nodes4f, uint32_t e, f3 o, f3 
 // decoded boxes contain the exact ones by a grid step, which covers the
 // rounding of this form).  The nearest hit child is pushed last (popped
 // first), the others in slot order: a full sort of 8 measured slower.
-// SORT = false (any-hit) pushes in slot order.
-template <bool SORT>
-CT_DEV void fnode8_step(const uint4* __restrict__ fnodes, uint32_t e, f3 o, f3 inv, float t, LaneStack& st) {
+// sort = false (any-hit) pushes in slot order.
+CT_DEV void fnode8_step(bool sort, const uint4* __restrict__ fnodes, uint32_t e, f3 o, f3 inv, float t,
+                        LaneStack& st) {
     const uint4* n = fnodes + 8 * (size_t)e;
     const uint4 h = n[0], qx = n[1], qy = n[2], qz = n[3], ca = n[4], cb = n[5];
     const float sx = __uint_as_float((h.w & 0xFFu) << 23), sy = __uint_as_float(((h.w >> 8) & 0xFFu) << 23),
@@ -343,7 +338,7 @@ CT_DEV void fnode8_step(const uint4* __restrict__ fnodes, uint32_t e, f3 o, f3 i
     }
     if (hm == 0u) return;
     const uint32_t ent[8] = {ca.x, ca.y, ca.z, ca.w, cb.x, cb.y, cb.z, cb.w};
-    if (SORT) {
+    if (sort) {
         // nearest hit child on top; slot 7 takes the nearest child's place
         const uint32_t a0 = (bslot & 1u) ? ent[1] : ent[0], a1 = (bslot & 1u) ? ent[3] : ent[2];
         const uint32_t a2 = (bslot & 1u) ? ent[5] : ent[4], a3 = (bslot & 1u) ? ent[7] : ent[6];
@@ -401,15 +396,25 @@ struct RayIn {
 
 constexpr uint32_t kNoLeaf = 0xFFFFFFFFu;
 
-// The queue [0, n) is cut into kFetchRanges contiguous ranges, four per XCD:
-// workgroups are dispatched to the XCDs round-robin by block id, so the
-// blocks b = x (mod 8) share XCD x's L2 and trace neighbouring rays of the
-// ranges x, x + 8, x + 16 and x + 24 (range = block id mod 32).  Each range's
-// claim counter is on its own cache line: with all counters in one line the
-// atomics of all XCDs serialised and throughput halved.  A range is drained
-// only by its own blocks (moving waves to other ranges at the end measured
-// slower); every range has blocks as long as the grid has at least
-// kFetchRanges of them, smaller grids move on to the next ranges.
+// What a launch traces: every ray closest-hit (extend, ray-level closest
+// hit), every ray any-hit (ray-level shadow rays), or two queues at once, a
+// closest-hit queue and an any-hit queue, each lane in the mode of the ray
+// it holds (the frame loop's bounce b: the extension rays of bounce b and
+// the shadow rays of bounce b - 1 in one launch, so neither launch's tail
+// idles the machine).
+enum TraceMode { kClosest = 0, kAny = 1, kDual = 2 };
+
+// The queues [0, n) and [0, n2) are cut into kFetchRanges contiguous ranges
+// each, four per XCD: workgroups are dispatched to the XCDs round-robin by
+// block id, so the blocks b = x (mod 8) share XCD x's L2 and trace
+// neighbouring rays of the ranges x, x + 8, x + 16 and x + 24 (range = block
+// id mod 32).  Range r's claim counter counts through its rays of the first
+// queue, then its rays of the second; each counter is on its own cache line:
+// with all counters in one line the atomics of all XCDs serialised and
+// throughput halved.  A range is drained only by its own blocks (moving
+// waves to other ranges at the end measured slower); every range has blocks
+// as long as the grid has at least kFetchRanges of them, smaller grids move
+// on to the next ranges.
 //
 // A wave claims rays for its idle lanes once kRefillIdle of its 64 lanes are
 // idle (or all, near the end): 4% faster than waiting for the whole wave.
@@ -417,21 +422,28 @@ constexpr uint32_t kRefillIdle = 48;
 
 struct Fetcher {
     uint32_t* base;  // counter of range 0 (range r at base + r * kFetchSlot)
-    uint32_t n;
+    uint32_t n, n2;  // rays of the first and the second queue
     int range;
     bool steal;
 
-    // Claim up to `want` rays: [first, first + count).  Wave-uniform.
-    CT_DEV bool get(uint32_t want, uint32_t& first, uint32_t& count) {
+    // Claim up to `want` indices of a range's claim space: [first, first +
+    // count).  Index v < nfirst is ray lo + v of the first queue, otherwise
+    // ray lo2 + v - nfirst of the second.  Wave-uniform.
+    CT_DEV bool get(uint32_t want, uint32_t& first, uint32_t& count, uint32_t& lo, uint32_t& nfirst,
+                    uint32_t& lo2) {
         for (int tries = 0; tries < kFetchRanges; ++tries) {
-            const uint32_t lo = (uint32_t)(((uint64_t)n * (uint32_t)range) / kFetchRanges);
+            lo = (uint32_t)(((uint64_t)n * (uint32_t)range) / kFetchRanges);
             const uint32_t hi = (uint32_t)(((uint64_t)n * (uint32_t)(range + 1)) / kFetchRanges);
+            lo2 = (uint32_t)(((uint64_t)n2 * (uint32_t)range) / kFetchRanges);
+            const uint32_t hi2 = (uint32_t)(((uint64_t)n2 * (uint32_t)(range + 1)) / kFetchRanges);
+            nfirst = hi - lo;
+            const uint32_t total = nfirst + (hi2 - lo2);
             uint32_t got = 0;
             if (lane_id() == 0) got = atomicAdd(base + range * kFetchSlot, want);
             got = __shfl(got, 0);
-            if (got < hi - lo) {
-                first = lo + got;
-                count = min(want, hi - lo - got);
+            if (got < total) {
+                first = got;
+                count = min(want, total - got);
                 return true;
             }
             if (!steal) return false;
@@ -441,28 +453,35 @@ struct Fetcher {
     }
 };
 
-template <int ACCEL, bool AnyHit, bool COUNT, class Source, class Sink>
-CT_DEV void trace_persistent(const DevScene& sc, float eps, uint32_t n, uint32_t* fetch, LaneStack& st,
+// n rays of the first queue and, with MODE kDual, n2 any-hit rays of the
+// second.  src(i, any) loads ray i of its queue, sink(i, any, t, prim) takes
+// the result: prim = the hit primitive, -1 none, -2 stack overflow (the
+// error flag is raised and the ray stops).
+template <int ACCEL, int MODE, bool COUNT, class Source, class Sink>
+CT_DEV void trace_persistent(const DevScene& sc, float eps, uint32_t n, uint32_t n2, uint32_t* fetch, LaneStack& st,
                              TravCount& cnt, uint32_t* err, Source src, Sink sink) {
     bool alive = false, exhausted = false, fast = false;
+    bool any = MODE == kAny;  // the lane's ray is an any-hit query
     uint32_t ray = 0, k = kNoLeaf;
     f3 o = mk3(0, 0, 0), d = mk3(0, 0, 0), inv = mk3(0, 0, 0);
     float t = 0.0f;
     int prim = -1;
-    Fetcher fe{fetch, n, (int)(blockIdx.x % kFetchRanges), gridDim.x < (uint32_t)kFetchRanges};
+    Fetcher fe{fetch, n, MODE == kDual ? n2 : 0u, (int)(blockIdx.x % kFetchRanges), gridDim.x < (uint32_t)kFetchRanges};
     while (true) {
         // ---- refill the idle lanes once kRefillIdle of them are idle (or
         // all of them), one claim per wave (wave-uniform)
         const uint64_t idle = __ballot(!alive);
         const uint32_t nidle = (uint32_t)__popcll(idle);
         if (!exhausted && (nidle >= kRefillIdle || idle == __ballot(true))) {
-            uint32_t first = 0, count = 0;
-            if (!fe.get(nidle, first, count)) exhausted = true;
+            uint32_t first = 0, count = 0, lo = 0, nfirst = 0, lo2 = 0;
+            if (!fe.get(nidle, first, count, lo, nfirst, lo2)) exhausted = true;
             if (!alive && !exhausted) {
                 const uint32_t rank = (uint32_t)__popcll(idle & ((1ull << lane_id()) - 1ull));
-                const uint32_t idx = first + rank;
+                const uint32_t v = first + rank;
                 if (rank < count) {
-                    const RayIn r = src(idx);
+                    if (MODE == kDual) any = v >= nfirst;
+                    const uint32_t idx = (MODE == kDual && any) ? lo2 + (v - nfirst) : lo + v;
+                    const RayIn r = src(idx, any);
                     ray = idx;
                     o = r.o;
                     d = r.d;
@@ -489,12 +508,18 @@ CT_DEV void trace_persistent(const DevScene& sc, float eps, uint32_t n, uint32_t
                 if (e & kLeafBit) {
                     k = e & ~kLeafBit;
                 } else {
-                    if (COUNT) ++cnt.nodes;
+                    if (COUNT) {
+                        if (any) ++cnt.snodes;
+                        else ++cnt.nodes;
+                    }
                     if (ACCEL == 4) {
-                        if (fast) node4_fast<!AnyHit>(sc.nodes4f, e, o, inv, t, st);
-                        else node4_step<!AnyHit>(sc.nodes4, e, o, inv, t, st);
+                        if (fast) node4_fast(!any, sc.nodes4f, e, o, inv, t, st);
+                        else node4_step(!any, sc.nodes4, e, o, inv, t, st);
                     } else {
-                        fnode8_step<!AnyHit>(sc.fnodes, e, o, inv, t, st);
+                        // any-hit lanes of a two-queue launch sort too (one
+                        // code path per wave)
+                        fnode8_step(MODE == kAny ? false : (MODE == kDual ? true : !any), sc.fnodes, e, o, inv, t,
+                                    st);
                     }
                 }
             } else {
@@ -506,14 +531,20 @@ CT_DEV void trace_persistent(const DevScene& sc, float eps, uint32_t n, uint32_t
                     // leaf of a user-supplied tree has none)
                     if (count > 0) k = fbits(lo.w);
                 } else {
-                    if (COUNT) cnt.nodes += 2;
+                    if (COUNT) {
+                        if (any) cnt.snodes += 2;
+                        else cnt.nodes += 2;
+                    }
                     node2_step(sc.nodes2, fbits(lo.w), o, inv, st);
                 }
             }
         }
         bool done = false;
         if (k != kNoLeaf) {
-            if (COUNT) ++cnt.tris;
+            if (COUNT) {
+                if (any) ++cnt.stris;
+                else ++cnt.tris;
+            }
             const float4 a = sc.tris[3 * (size_t)k];
             const float4 b = sc.tris[3 * (size_t)k + 1];
             const float4 c = sc.tris[3 * (size_t)k + 2];
@@ -525,7 +556,7 @@ CT_DEV void trace_persistent(const DevScene& sc, float eps, uint32_t n, uint32_t
             const uint32_t w = fbits(a.w);
             if (tri_test(o, d, t, xyz(a), xyz(b), xyz(c), eps)) {
                 prim = (int)(w & ~kLastInLeaf);
-                if (AnyHit) done = true;
+                if (any) done = true;
             }
             k = (w & kLastInLeaf) ? kNoLeaf : k + 1;
         }
@@ -535,7 +566,7 @@ CT_DEV void trace_persistent(const DevScene& sc, float eps, uint32_t n, uint32_t
             prim = -2;
         }
         if (done || (k == kNoLeaf && st.empty())) {
-            sink(ray, t, prim);
+            sink(ray, any, t, prim);
             alive = false;
         }
     }

@@ -12,6 +12,16 @@
 #include "scene.h"
 #include "scenegen.h"
 
+// The ABI structs mirror the reference's records byte for byte, so a caller
+// can pass its arrays by reinterpret_cast: Material (Material.cuh:29-41,
+// 13 four-byte fields), BVHNode (BVHNode.cuh:29-40), MBVHNode
+// (MBVHNode.cuh:28-63) and the camera state (Camera.h:81-92).
+static_assert(sizeof(ct_material) == 52, "ct_material must be the 52-byte core::Material");
+static_assert(sizeof(ct_bvh_node) == 32, "ct_bvh_node must be the 32-byte BVHNode");
+static_assert(sizeof(ct_mbvh_node) == 128, "ct_mbvh_node must be the 128-byte MBVHNode");
+static_assert(sizeof(ct_camera) == 40, "ct_camera: position, forward, up, plane distance");
+static_assert(sizeof(ct_frame_opts) == 52, "ct_frame_opts: 13 four-byte fields");
+
 struct ct_host_scene {
     ct::TriangleList list;
     std::unique_ptr<ct::BVHTree> bvh;

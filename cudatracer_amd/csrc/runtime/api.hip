@@ -14,12 +14,6 @@ using ctd::Segment;
 using ctd::ViewDesc;
 using namespace ctr;
 
-#define HIPCHK(expr, what)                                     \
-    do {                                                       \
-        hipError_t e_ = (expr);                                \
-        if (e_ != hipSuccess) return ctr::hip_fail(c, e_, what); \
-    } while (0)
-
 namespace {
 
 bool finite3(const float* p) { return std::isfinite(p[0]) && std::isfinite(p[1]) && std::isfinite(p[2]); }
@@ -580,7 +574,7 @@ int ct_trace_frame(ct_ctx* c, const ct_camera* cam, const ct_frame_opts* opts, c
     std::vector<Run> runs{{0u, 0u, (uint32_t)total}};
     const uint32_t cap = (uint32_t)std::min<uint64_t>(total, c->chunk_default);
     ChunkStats st;
-    r = run_paths(c, views, runs, cap, c->accum, o, stats != nullptr, false, false, &st);
+    r = run_paths(c, views, runs, cap, c->accum, o, stats ? kRunTimed : 0, &st);
     if (r != CT_OK) return r;
     c->samples += o.spp;
     fill_stats(st, stats);
@@ -607,11 +601,37 @@ int ct_trace_frame_counted(ct_ctx* c, const ct_camera* cam, const ct_frame_opts*
     std::vector<Run> runs{{0u, 0u, (uint32_t)total}};
     const uint32_t cap = (uint32_t)std::min<uint64_t>(total, c->chunk_default);
     ChunkStats st;
-    r = run_paths(c, views, runs, cap, c->accum, o, false, true, false, &st);
+    r = run_paths(c, views, runs, cap, c->accum, o, kRunCount, &st);
     if (r != CT_OK) return r;
     c->samples += o.spp;
     fill_stats(st, stats);
     for (int k = 0; k < 4; ++k) work[k] = st.work[k];
+    return CT_OK;
+}
+
+// Frame with its chunks run one at a time and every kernel timed: each
+// traversal launch has the device to itself, so its time is its own (for
+// the benchmark's isolated roofline figure; slower than ct_trace_frame).
+int ct_trace_frame_serial(ct_ctx* c, const ct_camera* cam, const ct_frame_opts* opts, ct_frame_stats* stats) {
+    if (!c || !stats) return CT_ERR_INVALID_ARG;
+    ct_frame_opts o;
+    if (opts) o = *opts;
+    else ct_frame_opts_default(&o);
+    int r = validate_opts(c, o);
+    if (r != CT_OK) return r;
+    if (!camera_ok(cam)) return fail(c, CT_ERR_INVALID_ARG, "camera is not finite or degenerate");
+    if (!c->accum) return fail(c, CT_ERR_INVALID_ARG, "no frame buffer (call ct_resize)");
+    (void)hipSetDevice(c->device);
+    const uint64_t total = (uint64_t)c->width * c->height * (uint64_t)o.spp;
+    if (total >= (1ull << 32)) return fail(c, CT_ERR_INVALID_ARG, "width*height*spp must be < 2^32");
+    std::vector<ViewDesc> views{make_view(*cam, c->width, c->height, o.spp, (uint32_t)c->samples, 0)};
+    std::vector<Run> runs{{0u, 0u, (uint32_t)total}};
+    const uint32_t cap = (uint32_t)std::min<uint64_t>(total, c->chunk_default);
+    ChunkStats st;
+    r = run_paths(c, views, runs, cap, c->accum, o, kRunTimed | kRunSerial, &st);
+    if (r != CT_OK) return r;
+    c->samples += o.spp;
+    fill_stats(st, stats);
     return CT_OK;
 }
 
@@ -717,7 +737,7 @@ int ct_trace_rays(ct_ctx* c, const float* origins, const float* dirs, uint32_t n
         HIPCHK(hipMemcpyAsync(c->ln[0].q.ray_d[0], rd.data(), m * sizeof(float4), hipMemcpyHostToDevice, c->ln[0].stream), "rays");
         HIPCHK(hipMemcpyAsync(c->ln[0].d_hdr, &h, sizeof(h), hipMemcpyHostToDevice, c->ln[0].stream), "header");
         HIPCHK(ctk::launch_setup(c->ln[0].q.counters, c->ln[0].d_hdr, c->ln[0].stream), "setup");
-        HIPCHK(ctk::launch_extend(c->shape, c->sc, to, c->ln[0].q, 0, 0, nullptr, c->ln[0].stream), "extend");
+        HIPCHK(ctk::launch_extend(c->shape, c->sc, to, c->ln[0].q, 0, 0, c->ln[0].stream), "extend");
         uint32_t errflag = 0;
         HIPCHK(hipMemcpyAsync(hits.data(), c->ln[0].q.hit, m * sizeof(float2), hipMemcpyDeviceToHost, c->ln[0].stream), "hits");
         HIPCHK(hipMemcpyAsync(&errflag, c->ln[0].q.err, 4, hipMemcpyDeviceToHost, c->ln[0].stream), "flags");
@@ -798,7 +818,7 @@ int ct_trace_rays_device(ct_ctx* c, const void* d_rays_o, const void* d_rays_d, 
     HIPCHK(hipMemcpyAsync(c->ln[0].d_hdr, &h, sizeof(h), hipMemcpyHostToDevice, c->ln[0].stream), "header");
     HIPCHK(ctk::launch_setup(c->ln[0].q.counters, c->ln[0].d_hdr, c->ln[0].stream), "setup");
     HIPCHK(hipEventRecord(c->ln[0].ev[0], c->ln[0].stream), "event");
-    HIPCHK(ctk::launch_extend(c->shape, c->sc, to, q, 0, 0, nullptr, c->ln[0].stream), "extend");
+    HIPCHK(ctk::launch_extend(c->shape, c->sc, to, q, 0, 0, c->ln[0].stream), "extend");
     HIPCHK(hipEventRecord(c->ln[0].ev[1], c->ln[0].stream), "event");
     uint32_t errflag = 0;
     HIPCHK(hipMemcpyAsync(&errflag, c->ln[0].q.err, 4, hipMemcpyDeviceToHost, c->ln[0].stream), "flags");
@@ -848,7 +868,7 @@ int ct_render_batch(ct_ctx* c, const ct_view* views, int32_t n_views, const ct_f
     ChunkStats st;
     int chunks_before = 0;
     (void)chunks_before;
-    r = run_paths(c, vd, runs, cap, arena, o, false, false, true, &st);
+    r = run_paths(c, vd, runs, cap, arena, o, kRunGraph, &st);
     if (r == CT_OK) {
         for (int32_t i = 0; i < n_views && e == hipSuccess; ++i)
             e = hipMemcpyAsync(views[i].accum_out, arena + vd[i].accum_offset,
@@ -905,7 +925,7 @@ int ct_autotune(ct_ctx* c, const ct_camera* cam, const ct_frame_opts* opts, int3
         *t = 0;
         for (int rep = 0; rep < 3; ++rep) {
             ChunkStats st;
-            const int rr = run_paths(c, views, runs, (uint32_t)(npix * spp_shape), acc, o, false, false, false, &st);
+            const int rr = run_paths(c, views, runs, (uint32_t)(npix * spp_shape), acc, o, 0, &st);
             if (rr != CT_OK) return rr;
             if (rep > 0) *t += st.total_ms;
         }
@@ -940,7 +960,7 @@ int ct_autotune(ct_ctx* c, const ct_camera* cam, const ct_frame_opts* opts, int3
     std::vector<Run> r2{{0u, 0u, (uint32_t)(npix * spp_chunk)}};
     auto time_chunk = [&](uint32_t ch, double* t) -> int {
         ChunkStats st;
-        const int rr = run_paths(c, v2, r2, ch, acc, o, false, false, false, &st);
+        const int rr = run_paths(c, v2, r2, ch, acc, o, 0, &st);
         if (rr == CT_OK) { *t = st.total_ms; ++trials; }
         return rr;
     };

@@ -34,12 +34,7 @@ static_assert(kLanes >= 2, "at least two lanes");
 struct Lane {
     ctd::Queues q{};
     hipStream_t stream = nullptr;
-    // connect(b) runs here, beside extend(b + 1) on `stream`; fork / join
-    // events order it after shade(b) and before shade(b + 1)
-    hipStream_t side = nullptr;
-    hipEvent_t fork[ctd::kMaxBounce] = {}, join[ctd::kMaxBounce] = {};
     hipEvent_t resolved = nullptr;   // after this lane's latest resolve
-    uint32_t* spill_side = nullptr;  // stack spill area of the side stream's traversal
     unsigned long long* wstats = nullptr;
     ctd::ViewDesc* d_views = nullptr;
     uint32_t d_views_cap = 0;
@@ -60,6 +55,7 @@ struct Lane {
     uint32_t n_paths = 0;
     int max_depth = 0;
     bool timed = false, count = false;
+    bool nee = false;  // the chunk ends with a launch for the last bounce's shadow rays
 };
 // h_read layout (uint32 words)
 constexpr int kReadCounters = 2 * (ctd::kMaxBounce + 1);
@@ -102,6 +98,13 @@ struct ct_ctx {
     uint32_t spill_threads = 0;
 };
 
+// Return the failure of a HIP call as CT_ERR_DEVICE on context `c`.
+#define HIPCHK(expr, what)                                       \
+    do {                                                         \
+        hipError_t e_ = (expr);                                  \
+        if (e_ != hipSuccess) return ctr::hip_fail(c, e_, what); \
+    } while (0)
+
 namespace ctr {
 
 int fail(ct_ctx* c, int code, const std::string& msg);
@@ -125,7 +128,7 @@ constexpr size_t kPathSlotBytes = 6 * 16 + 2 * 4 + 8 + 6 * 16 + 16 + 4 * 16;
 int ensure_paths(ct_ctx* c, uint32_t cap);
 int ensure_tables(ct_ctx* c, Lane& L, uint32_t nviews, uint32_t nsegs);
 void drop_graphs(ct_ctx* c);
-// Wait for both lanes' streams (side streams included).
+// Wait for every lane's stream.
 void sync_lanes(ct_ctx* c);
 ctd::ViewDesc make_view(const ct_camera& cam, int width, int height, int spp, uint32_t frame_base,
                         uint64_t accum_offset);
@@ -138,8 +141,14 @@ ctd::ViewDesc make_view(const ct_camera& cam, int width, int height, int spp, ui
 struct Run {
     uint32_t view, first_path, n;
 };
+// How run_paths runs the chunks: kRunTimed records an event pair around every
+// kernel (per-stage times in ChunkStats), kRunCount counts node / triangle
+// tests, kRunGraph replays each lane's captured bounce loop, kRunSerial
+// runs one chunk at a time (nothing overlaps: per-kernel times are the
+// kernels' own, for measurement).
+enum RunFlags { kRunTimed = 1, kRunCount = 2, kRunGraph = 4, kRunSerial = 8 };
 int run_paths(ct_ctx* c, const std::vector<ctd::ViewDesc>& views, const std::vector<Run>& runs, uint32_t cap,
-              float4* accum, const ct_frame_opts& o, bool timed, bool count, bool use_graph, ChunkStats* st);
+              float4* accum, const ct_frame_opts& o, int flags, ChunkStats* st);
 
 }  // namespace ctr
 

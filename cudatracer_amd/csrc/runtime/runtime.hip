@@ -31,12 +31,6 @@ int hip_fail(ct_ctx* c, hipError_t e, const char* what) {
     return fail(c, CT_ERR_DEVICE, os.str());
 }
 
-#define HIPCHK(expr, what)                                     \
-    do {                                                       \
-        hipError_t e_ = (expr);                                \
-        if (e_ != hipSuccess) return ctr::hip_fail(c, e_, what); \
-    } while (0)
-
 int dalloc(ct_ctx* c, void** p, size_t bytes, const char* what) {
     *p = nullptr;
     if (bytes == 0) bytes = 16;
@@ -83,7 +77,6 @@ void drop_graphs(ct_ctx* c) {
 void sync_lanes(ct_ctx* c) {
     for (Lane& L : c->ln) {
         if (L.stream) (void)hipStreamSynchronize(L.stream);
-        if (L.side) (void)hipStreamSynchronize(L.side);
         L.pending = false;
     }
 }
@@ -135,7 +128,6 @@ static void free_paths(ct_ctx* c) {
         L.h_read = nullptr;
         L.q = ctd::Queues{};
         L.wstats = nullptr;
-        L.spill_side = nullptr;
         L.d_hdr = nullptr;
     }
     c->lane_cap = 0;
@@ -204,7 +196,6 @@ int ensure_paths(ct_ctx* c, uint32_t cap) {
         r = A((void**)&L.q.counters, ctd::kCounterWords * 4, "counters");
         if (r == CT_OK) r = A((void**)&L.q.err, 64, "error flags");
         if (r == CT_OK) r = A((void**)&L.q.spill, (size_t)threads * ctd::kMaxStack * 4, "stack spill");
-        if (r == CT_OK) r = A((void**)&L.spill_side, (size_t)threads * ctd::kMaxStack * 4, "stack spill (side)");
         if (r == CT_OK) r = A((void**)&L.wstats, 64, "work counters");
         if (r == CT_OK) r = A((void**)&L.d_hdr, sizeof(ChunkHdr), "chunk header");
         if (r == CT_OK && hipHostMalloc((void**)&L.h_read, kReadWords * 4, hipHostMallocDefault) != hipSuccess) {
@@ -317,11 +308,12 @@ static ctd::ShadeOpts shade_opts(const ct_frame_opts& o) {
 }
 
 // The kernel sequence of one chunk on a lane, up to the resolve
-// (setGlobals -> generate -> depth x (extend, shade, connect)), in steps:
-// step 0 is the prologue, step 1 + b is bounce b, then finish().  Chunks of
-// several lanes are enqueued step by step in turn, so every lane starts at
-// once instead of after the host has enqueued the whole of the previous
-// lanes' chunks.  timed records an event pair around every kernel.
+// (setGlobals -> generate -> depth x (trace, shade) -> trace of the last
+// shadow rays), in steps: step 0 is the prologue, step 1 + b is bounce b,
+// then finish().  Chunks of several lanes are enqueued step by step in turn,
+// so every lane starts at once instead of after the host has enqueued the
+// whole of the previous lanes' chunks.  timed records an event pair around
+// every kernel.
 struct BounceSeq {
     ct_ctx* c = nullptr;
     Lane* L = nullptr;
@@ -330,15 +322,12 @@ struct BounceSeq {
     ctd::ShadeOpts so{};
     unsigned long long* ws = nullptr;
     bool timed = false, nee = false;
-    ctd::Queues qside{};
     int e = 0, sel = 0;
 
     BounceSeq() = default;
     BounceSeq(ct_ctx* c_, Lane* L_, const ct_frame_opts& o_, bool count, bool timed_)
         : c(c_), L(L_), o(o_), to(trace_opts(o_)), so(shade_opts(o_)), ws(count ? L_->wstats : nullptr),
-          timed(timed_), nee(o_.shadow && c_->sc.n_lights > 0), qside(L_->q) {
-        qside.spill = L_->spill_side;
-    }
+          timed(timed_), nee(o_.shadow && c_->sc.n_lights > 0) {}
     int steps() const { return 1 + o.max_depth; }
     void rec() {
         if (timed) (void)hipEventRecord(L->ev[e++], L->stream);
@@ -355,40 +344,30 @@ struct BounceSeq {
             rec();
             return hipSuccess;
         }
-        // connect(b) on the side stream overlaps extend(b + 1) and shade(b + 1):
-        // both read queues shade(b) wrote.  Connect's radiance term of bounce
-        // b is added by shade(b + 2) or resolve (the additions keep their
-        // order), so shade(b + 2) and resolve wait for connect(b).  Its
-        // traversal stacks spill to their own area.
+        // trace(b): the extension rays of bounce b and the shadow rays of
+        // bounce b - 1 (their radiance terms are added by shade(b + 1) or
+        // resolve, in order); then shade(b) appends bounce b + 1's extension
+        // rays and bounce b's shadow rays
         const int b = k - 1;
         rec();
-        if ((r = ctk::launch_extend(c->shape, c->sc, to, L->q, sel, b, ws, s)) != hipSuccess) return r;
+        if ((r = ctk::launch_trace(c->shape, c->sc, to, L->q, L->d_hdr, sel, b, ws, s)) != hipSuccess) return r;
         rec();
-        if (nee && b > 1 && (r = hipStreamWaitEvent(s, L->join[b - 2], 0)) != hipSuccess) return r;
         rec();
         if ((r = ctk::launch_shade(c->shape, c->sc, so, L->d_views, L->d_segs, L->d_hdr, L->q, sel, b, s)) !=
             hipSuccess)
             return r;
         rec();
-        if (nee) {
-            if ((r = hipEventRecord(L->fork[b], s)) != hipSuccess) return r;
-            if ((r = hipStreamWaitEvent(L->side, L->fork[b], 0)) != hipSuccess) return r;
-            if (timed) (void)hipEventRecord(L->ev[e], L->side);
-            ++e;
-            if ((r = ctk::launch_connect(c->shape, c->sc, to, qside, L->d_hdr, b, ws, L->side)) != hipSuccess) return r;
-            if (timed) (void)hipEventRecord(L->ev[e], L->side);
-            ++e;
-            if ((r = hipEventRecord(L->join[b], L->side)) != hipSuccess) return r;
-        } else {
-            rec();
-            rec();
-        }
         sel ^= 1;
         return hipSuccess;
     }
+    // the last bounce's shadow rays (the launch has no extension rays)
     hipError_t finish() {
         hipError_t r;
-        if (nee && (r = hipStreamWaitEvent(L->stream, L->join[o.max_depth - 1], 0)) != hipSuccess) return r;
+        rec();
+        if (nee && (r = ctk::launch_trace(c->shape, c->sc, to, L->q, L->d_hdr, sel, o.max_depth, ws, L->stream)) !=
+                       hipSuccess)
+            return r;
+        rec();
         L->n_events = e;
         return hipSuccess;
     }
@@ -503,6 +482,7 @@ static int close_chunk(ct_ctx* c, Lane& L, uint32_t n, float4* accum, const ct_f
     L.pending = true;
     L.n_paths = n;
     L.max_depth = o.max_depth;
+    L.nee = o.shadow && c->sc.n_lights > 0;
     L.timed = timed;
     L.count = count;
     return CT_OK;
@@ -527,7 +507,8 @@ static int finish_chunk(ct_ctx* c, Lane& L, ChunkStats* st) {
     float ms = 0;
     if (hipEventElapsedTime(&ms, L.ev[kEvTotal0], L.ev[kEvTotal0 + 1]) == hipSuccess) st->total_ms += ms;
     if (L.timed) {
-        // event pairs: [generate] then per bounce [extend][shade][connect]; [resolve]
+        // event pairs: [generate] then per bounce [trace][shade]; [trace of
+        // the last shadow rays]; [resolve]
         int e = 0;
         auto pair = [&](int i) {
             float m = 0;
@@ -540,8 +521,11 @@ static int finish_chunk(ct_ctx* c, Lane& L, ChunkStats* st) {
             st->extend_ms += pair(e);
             st->extend_launches += 1;
             st->shade_ms += pair(e + 2);
-            st->connect_ms += pair(e + 4);
-            e += 6;
+            e += 4;
+        }
+        if (L.nee) {
+            st->connect_ms += pair(e);
+            st->extend_launches += 1;
         }
         st->other_ms += pair(kEvResolve);
     }
@@ -568,7 +552,9 @@ static void abort_lanes(ct_ctx* c) {
 static constexpr uint64_t kSplitMin = 1u << 18;
 
 int run_paths(ct_ctx* c, const std::vector<ViewDesc>& views, const std::vector<Run>& runs, uint32_t cap,
-              float4* accum, const ct_frame_opts& o, bool timed, bool count, bool use_graph, ChunkStats* st) {
+              float4* accum, const ct_frame_opts& o, int flags, ChunkStats* st) {
+    const bool timed = flags & kRunTimed, count = flags & kRunCount, use_graph = flags & kRunGraph,
+               serial = flags & kRunSerial;
     uint64_t total = 0;
     for (const Run& run : runs) total += run.n;
     if (total >= kSplitMin && cap > 0) {
@@ -618,6 +604,7 @@ int run_paths(ct_ctx* c, const std::vector<ViewDesc>& views, const std::vector<R
             Lane& L = c->ln[(lane + i) % kLanes];
             if ((rr = close_chunk(c, L, group[i].fill, accum, o, timed, count, prev)) != CT_OK) return rr;
             prev = L.resolved;
+            if (serial && (rr = finish_chunk(c, L, st)) != CT_OK) return rr;  // done before the next starts
         }
         lane = (lane + g) % kLanes;
         launched += g;
@@ -626,7 +613,7 @@ int run_paths(ct_ctx* c, const std::vector<ViewDesc>& views, const std::vector<R
     };
     auto flush = [&](std::vector<Segment>& chunk_segs, uint32_t chunk_fill, uint32_t chunk_tasks) -> int {
         group.push_back(Staged{chunk_segs, chunk_fill, chunk_tasks});
-        return launched + (int)group.size() < kLanes ? CT_OK : launch_group();
+        return (!serial && launched + (int)group.size() < kLanes) ? CT_OK : launch_group();
     };
     std::vector<Segment> segs;
     uint32_t fill = 0, tasks = 0;
@@ -745,11 +732,8 @@ int ct_ctx_create(const ct_device_opts* opts, ct_ctx** out) {
     if (hipMemGetInfo(&freeb, &totalb) != hipSuccess) { delete c; return CT_ERR_DEVICE; }
     c->budget = d.mem_budget ? d.mem_budget : (uint64_t)(0.9 * (double)freeb);
     for (Lane& L : c->ln) {
-        bool ok = hipStreamCreate(&L.stream) == hipSuccess && hipStreamCreate(&L.side) == hipSuccess &&
+        bool ok = hipStreamCreate(&L.stream) == hipSuccess &&
                   hipEventCreateWithFlags(&L.resolved, hipEventDisableTiming) == hipSuccess;
-        for (int b = 0; b < ctd::kMaxBounce && ok; ++b)
-            ok = hipEventCreateWithFlags(&L.fork[b], hipEventDisableTiming) == hipSuccess &&
-                 hipEventCreateWithFlags(&L.join[b], hipEventDisableTiming) == hipSuccess;
         for (auto& ev : L.ev)
             if (ok) ok = hipEventCreate(&ev) == hipSuccess;
         if (!ok) {
@@ -775,12 +759,7 @@ void ct_ctx_destroy(ct_ctx* c) {
         if (L.h_stage) (void)hipHostFree(L.h_stage);
         for (auto& ev : L.ev)
             if (ev) (void)hipEventDestroy(ev);
-        for (int b = 0; b < ctd::kMaxBounce; ++b) {
-            if (L.fork[b]) (void)hipEventDestroy(L.fork[b]);
-            if (L.join[b]) (void)hipEventDestroy(L.join[b]);
-        }
         if (L.resolved) (void)hipEventDestroy(L.resolved);
-        if (L.side) (void)hipStreamDestroy(L.side);
         if (L.stream) (void)hipStreamDestroy(L.stream);
     }
     delete c;

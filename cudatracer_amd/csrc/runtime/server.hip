@@ -263,7 +263,7 @@ void ct_server::loop() {
                 rc = CT_ERR_DEVICE;
         ChunkStats st;
         if (rc == CT_OK)
-            rc = run_paths(ctx, views, runs, chunk, arena_ptr, chunk_opts, false, false, true, &st);
+            rc = run_paths(ctx, views, runs, chunk, arena_ptr, chunk_opts, kRunGraph, &st);
         std::unique_lock<std::mutex> lk(mu);
         if (rc != CT_OK) {
             fatal = rc;

@@ -186,9 +186,15 @@ typedef struct ct_frame_stats {
     double gpu_ms;            /* device time of the frame's chunks (HIP events; chunks on
                                  the two lanes overlap, so this can exceed wall time) */
     double extend_ms, shade_ms, connect_ms, other_ms; /* per-stage device time, if profiled;
-                                 stages of different chunks overlap */
+                                 stages of different chunks overlap.  extend_ms:
+                                 the traversal launches of the bounces (each
+                                 traces bounce b's extension rays and bounce
+                                 b - 1's shadow rays); connect_ms: the launches
+                                 of the last bounce's shadow rays             */
     uint32_t chunks;          /* wavefront chunks the frame was split into    */
-    uint32_t extend_launches; /* closest-hit kernel launches (chunks x depth) */
+    uint32_t extend_launches; /* traversal launches of the bounces (chunks x depth;
+                                 plus one per chunk for the last shadow rays
+                                 when NEE is on)                              */
 } ct_frame_stats;
 
 /* Fill with the reference defaults (depth 5, 1 spp, no RR, BVH4, reference BSDF,
@@ -446,6 +452,13 @@ int ct_host_scene_stats(const ct_host_scene* s, uint64_t out[8]);
  * ct_trace_frame (atomics), never used in timed runs. */
 int ct_trace_frame_counted(ct_ctx* ctx, const ct_camera* cam, const ct_frame_opts* opts,
                            ct_frame_stats* stats, uint64_t work[4]);
+/* ct_trace_frame with the frame's wavefront chunks run one at a time and
+ * every kernel timed (stats required): each traversal launch has the device
+ * to itself, so stats->extend_ms / connect_ms are the kernels' own times
+ * (in ct_trace_frame two chunks and their launches overlap).  Same image;
+ * slower, for measurement only. */
+int ct_trace_frame_serial(ct_ctx* ctx, const ct_camera* cam, const ct_frame_opts* opts,
+                          ct_frame_stats* stats);
 
 /* The wide compressed tree of CT_TRAV_FAST, which ct_scene_upload builds
  * from the view's binary tree (or quad tree): up to 8 children per node,

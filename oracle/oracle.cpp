@@ -170,7 +170,7 @@ struct or_scene {
     bool built = false;
     // any-hit visit order of the quad tree: false = the reference's sorted
     // order (traverseMBVHShadow calls intersect, MBVHNode.cuh:165-205), true =
-    // slot order, slot 3 popped first (the order the kernels visit any-hit
+    // slot order, slot 0 popped first (the order the kernels visit any-hit
     // children in: the result is order-free, the work counts are not)
     bool anyhit_slot_order = false;
     uint32_t ntris() const { return (uint32_t)indices.size() / 3; }
@@ -478,7 +478,7 @@ bool traverse_bvh4(const or_scene& s, V3<T> o, V3<T> d, T* t, int* hit, T eps, C
         c.nodes++;
         mbvh_intersect(n, o, inv, *t, tmin, res);
         if (AnyHit && s.anyhit_slot_order) {
-            for (int i = 0; i < 4; ++i)
+            for (int i = 3; i >= 0; --i)
                 if (res[i]) stack[++sp] = {n.childs[i], n.counts[i]};
             continue;
         }

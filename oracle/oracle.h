@@ -98,7 +98,7 @@ double or_fresnel_dielectric(double cos_i, double eta);
  * 64): writes the direction, returns 1 unless total internal reflection. */
 int or_refract(int precision, const float* I, const float* N, float eta, double* out);
 /* Any-hit visit order of the quad tree: 0 = the reference's sorted order
- * (default), 1 = slot order with slot 3 first, the order of the kernels'
+ * (default), 1 = slot order with slot 0 first, the order of the kernels'
  * any-hit traversal.  Occlusion results do not depend on it; node and
  * triangle test counts do. */
 void or_scene_set_anyhit_order(or_scene* s, int slot_order);

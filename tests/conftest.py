@@ -77,6 +77,48 @@ def c3(ct, orc, gpu_ctx):
     return scene, arrays, osc
 
 
+class DeviceArray:
+    """Device memory for the device-pointer entry points, through the HIP
+    runtime the library itself links (torch ships its own ROCm runtime,
+    which sees no GPU once the library's runtime owns it in the process)."""
+
+    _hip = None
+
+    def __init__(self, host: np.ndarray):
+        if DeviceArray._hip is None:
+            import cudatracer_amd  # noqa: F401  (loads the library and its HIP runtime)
+            path = next((ln.split()[-1] for ln in open("/proc/self/maps") if "libamdhip64" in ln), None)
+            if path is None:
+                raise RuntimeError("the library's HIP runtime is not loaded")
+            lib = C.CDLL(path)
+            lib.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
+            lib.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+            lib.hipFree.argtypes = [C.c_void_p]
+            DeviceArray._hip = lib
+        self.host = np.ascontiguousarray(host)
+        self.ptr = C.c_void_p()
+        if DeviceArray._hip.hipMalloc(C.byref(self.ptr), self.host.nbytes) != 0:
+            raise MemoryError("hipMalloc")
+        self.upload()
+
+    def upload(self):
+        assert DeviceArray._hip.hipMemcpy(self.ptr, self.host.ctypes.data, self.host.nbytes, 1) == 0  # H2D
+
+    def download(self) -> np.ndarray:
+        out = np.empty_like(self.host)
+        assert DeviceArray._hip.hipMemcpy(out.ctypes.data, self.ptr, out.nbytes, 2) == 0  # D2H
+        return out
+
+    @property
+    def addr(self) -> int:
+        return self.ptr.value
+
+    def __del__(self):
+        if getattr(self, "ptr", None) and self.ptr.value and DeviceArray._hip is not None:
+            DeviceArray._hip.hipFree(self.ptr)
+            self.ptr = C.c_void_p()
+
+
 def camera_rays(cam, width: int, height: int, n: int, seed: int):
     """Jittered pinhole rays of the camera (same basis as the kernels)."""
     rng = np.random.default_rng(seed)

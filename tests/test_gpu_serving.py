@@ -92,8 +92,9 @@ def test_server_honours_every_option(ct, gpu_ctx, c2):
     cam = ct.look_at((-80.0, 12.0, -75.0), (5.0, 8.0, 10.0), 70.0)
     w, h = 160, 96
     variants = []
+    assert opts.light_pdf == ct.LIGHTPDF_REFERENCE and opts.indirect == 1 and opts.rr_start == -1
     for field, value in (("spp", 2), ("light_pdf", ct.LIGHTPDF_UNIFORM), ("indirect", 0),
-                         ("normal_epsilon", 1e-3), ("max_depth", 3), ("rr_start", 1)):
+                         ("normal_epsilon", 4 * opts.normal_epsilon), ("max_depth", 3), ("rr_start", 1)):
         o = opts.copy()
         o.spp = 2
         setattr(o, field, value)

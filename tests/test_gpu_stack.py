@@ -98,7 +98,7 @@ def test_stack_overflow_reported_and_cleared(ct, gpu_ctx, c1):
     """50 levels need 151 entries: ct_trace_rays, ct_trace_rays_device and
     ct_trace_frame return CT_ERR_STACK_OVERFLOW (the kernels end normally,
     the overflowing rays stop), and the context works afterwards."""
-    import torch
+    from conftest import DeviceArray
     arrays = chain_scene(ct, 50, seed=3)
     av = ArrayView(ct, arrays)
     gpu_ctx.upload_view(av.view)
@@ -109,14 +109,13 @@ def test_stack_overflow_reported_and_cleared(ct, gpu_ctx, c1):
     assert e.value.code == ct.CT_ERR_STACK_OVERFLOW
     # the device-pointer entry point reports it too, and clears the flag
     n = o.shape[0]
-    d_o = torch.zeros((n, 4), dtype=torch.float32, device="cuda")
-    d_d = torch.zeros((n, 4), dtype=torch.float32, device="cuda")
-    d_o[:, :3] = torch.from_numpy(o).cuda()
-    d_d[:, :3] = torch.from_numpy(d).cuda()
-    hits = torch.zeros((n, 2), dtype=torch.float32, device="cuda")
-    torch.cuda.synchronize()
+    h_o = np.zeros((n, 4), np.float32)
+    h_d = np.zeros((n, 4), np.float32)
+    h_o[:, :3], h_d[:, :3] = o, d
+    d_o, d_d = DeviceArray(h_o), DeviceArray(h_d)
+    hits = DeviceArray(np.zeros((n, 2), np.float32))
     with pytest.raises(ct.CTError) as e:
-        gpu_ctx.trace_rays_device(d_o.data_ptr(), d_d.data_ptr(), n, hits.data_ptr(), opts)
+        gpu_ctx.trace_rays_device(d_o.addr, d_d.addr, n, hits.addr, opts)
     assert e.value.code == ct.CT_ERR_STACK_OVERFLOW
     cam = ct.look_at((-30.0, 0.5, 0.3), (0.0, 0.0, 0.0), 20.0)
     gpu_ctx.resize(32, 32)
@@ -131,5 +130,5 @@ def test_stack_overflow_reported_and_cleared(ct, gpu_ctx, c1):
     gpu_ctx.trace_frame(scene.camera(64, 64), o1)
     t, p = gpu_ctx.trace_rays(o[:8], d[:8], o1)
     assert t.shape == (8,)
-    ms = gpu_ctx.trace_rays_device(d_o.data_ptr(), d_d.data_ptr(), 64, hits.data_ptr(), o1)
+    ms = gpu_ctx.trace_rays_device(d_o.addr, d_d.addr, 64, hits.addr, o1)
     assert ms > 0

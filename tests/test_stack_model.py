@@ -17,11 +17,12 @@ U32 = 0xFFFFFFFF
 
 
 class LaneStackModel:
-    def __init__(self, block: int, depth: int, tid: int):
+    def __init__(self, block: int, depth: int, tid: int, max_stack: int = K_MAX_STACK):
+        depth = min(depth, max_stack)  # make_stack: no spill when the LDS part holds the whole bound
         self.stride4 = 4 * block
         self.lds_end = depth * self.stride4
-        self.max_end = K_MAX_STACK * self.stride4
-        self.region = (K_MAX_STACK - depth) * self.stride4
+        self.max_end = max_stack * self.stride4
+        self.region = (max_stack - depth) * self.stride4
         self.tid = tid
         self.lds = {}
         self.spill = {}
@@ -77,11 +78,11 @@ class LaneStackModel:
 
 @pytest.mark.parametrize("block,depth", [(64, 8), (128, 16), (256, 16), (256, 24), (512, 32), (1024, 8),
                                          (256, 127)])
-def test_lane_stack_bounds_and_lifo(block, depth):
-    rng = np.random.default_rng(block * 131 + depth)
+def test_lane_stack_bounds_and_lifo(block, depth, max_stack=K_MAX_STACK):
+    rng = np.random.default_rng(block * 131 + depth + max_stack)
     for trial in range(60):
         tid = int(rng.integers(0, block))
-        st = LaneStackModel(block, depth, tid)
+        st = LaneStackModel(block, depth, tid, max_stack)
         ref = []
         st.push4([0, 0, 0, 0], [True, False, False, False])  # the root, as trace_persistent pushes it
         ref.append(0)
@@ -92,7 +93,7 @@ def test_lane_stack_bounds_and_lifo(block, depth):
                 vals = [int(x) for x in rng.integers(1, 1 << 30, 4)]
                 need = len(ref) + sum(keep)
                 st.push4(vals, keep)
-                if need > K_MAX_STACK:
+                if need > max_stack:
                     assert st.overflow, (need, trial)
                     break  # the lane stops
                 assert not st.overflow
@@ -121,3 +122,4 @@ def test_overflow_exactly_past_max_stack():
         assert not st.overflow  # 128 entries: positions 0..127
         st.push4([5, 6, 7, 8], [False, False, False, True])
         assert st.overflow
+
