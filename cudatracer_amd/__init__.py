@@ -183,6 +183,8 @@ _sig = {
     "ct_trace_shadow_rays": (C.c_int, [_P, C.POINTER(C.c_float), C.POINTER(C.c_float), C.POINTER(C.c_float),
                                        C.c_uint32, C.POINTER(FrameOpts), C.POINTER(C.c_uint8)]),
     "ct_trace_rays_device": (C.c_int, [_P, _P, _P, C.c_uint32, C.POINTER(FrameOpts), _P, C.POINTER(C.c_float)]),
+    "ct_trace_rays_device_noalloc": (C.c_int, [_P, _P, _P, C.c_uint32, C.POINTER(FrameOpts), _P,
+                                               C.POINTER(C.c_float)]),
     "ct_render_batch": (C.c_int, [_P, C.POINTER(View), C.c_int32, C.POINTER(FrameOpts), C.POINTER(BatchStats)]),
     "ct_autotune": (C.c_int, [_P, C.POINTER(Camera), C.POINTER(FrameOpts), C.c_int32, C.c_int32,
                               C.POINTER(TuneResult)]),
@@ -534,6 +536,14 @@ class Context:
         ms = C.c_float()
         _check(lib.ct_trace_rays_device(self._h, C.c_void_p(d_o), C.c_void_p(d_d), n, C.byref(opts),
                                         C.c_void_p(d_hits), C.byref(ms)), self._h)
+        return ms.value
+
+    def trace_rays_device_noalloc(self, d_o: int, d_d: int, n: int, d_hits: int, opts: FrameOpts) -> float:
+        """ct_trace_rays_device without device allocations (the library's
+        per-GPU scratch array; 48-entry traversal stack)."""
+        ms = C.c_float()
+        _check(lib.ct_trace_rays_device_noalloc(self._h, C.c_void_p(d_o), C.c_void_p(d_d), n, C.byref(opts),
+                                                C.c_void_p(d_hits), C.byref(ms)), self._h)
         return ms.value
 
     def render_batch(self, views: list, opts: FrameOpts):

@@ -155,8 +155,9 @@ struct Queues {
     float4* ebuf[2];    // shade's radiance term of bounce b at [b % 2] (xyz), .w = stamp (bits)
     float4* nbuf[2];    // connect's radiance term of bounce b at [b % 2] (xyz), .w = stamp (bits)
     uint32_t* counters; // see CounterIndex
-    uint32_t* spill;    // traversal stack overflow area
+    uint32_t* spill;    // traversal stack overflow area: per block (max_stack - LDS depth) x blockDim words
     uint32_t* err;      // error flags
+    int max_stack = kMaxStack;  // traversal stack entries (LDS + spill); deeper trees report an overflow
 };
 
 // counters[] layout: for bounce b in [0, kMaxBounce]

@@ -55,6 +55,15 @@ __global__ void k_setup(uint32_t* counters, const ChunkHdr* hdr) {
     if (threadIdx.x == 0) *cnt_active(counters, 0) = hdr->n;
 }
 
+// The same for a ray-level launch of n rays, with the size as an argument
+// (no header in device memory) and the error flags cleared too.
+__global__ void k_setup_rays(uint32_t* counters, uint32_t* err, uint32_t n) {
+    for (int i = threadIdx.x; i < kCounterWords; i += blockDim.x) counters[i] = 0u;
+    if (threadIdx.x < 16) err[threadIdx.x] = 0u;
+    __syncthreads();
+    if (threadIdx.x == 0) *cnt_active(counters, 0) = n;
+}
+
 // ------------------------------------------------------------- generate --
 // generatePrimaryRays + Ray::generate (Kernel.cu:208-234, Ray.cuh:53-61,85-95)
 __global__ void k_generate(const ViewDesc* __restrict__ views, const Segment* __restrict__ segs,
@@ -121,10 +130,12 @@ CT_DEV LaneStack make_stack(const Queues& q, int lds_depth) {
     // made scalar here, so the stride and the spill resource stay in SGPRs
     const uint32_t bdim = __builtin_amdgcn_readfirstlane(blockDim.x);
     st.stride4 = 4u * bdim;
-    st.lds_end = (uint32_t)lds_depth * st.stride4;
-    st.max_end = (uint32_t)kMaxStack * st.stride4;
-    // the block's spill region: positions [lds_depth, kMaxStack) of its threads
-    const uint32_t region = (uint32_t)(kMaxStack - lds_depth) * st.stride4;
+    // positions [0, lds_depth) in LDS, [lds_depth, max_stack) in the spill
+    // region (none when the LDS part holds them all)
+    const int depth = lds_depth < q.max_stack ? lds_depth : q.max_stack;
+    st.lds_end = (uint32_t)depth * st.stride4;
+    st.max_end = (uint32_t)q.max_stack * st.stride4;
+    const uint32_t region = (uint32_t)(q.max_stack - depth) * st.stride4;
     char* base = reinterpret_cast<char*>(q.spill) + (size_t)blockIdx.x * region;
     st.spill = __builtin_amdgcn_make_buffer_rsrc(base, 0, (int)region, kBufferWord3);
     st.reset();
@@ -644,6 +655,10 @@ using namespace ctd;
 
 hipError_t launch_setup(uint32_t* counters, const ChunkHdr* hdr, hipStream_t s) {
     hipLaunchKernelGGL(k_setup, dim3(1), dim3(256), 0, s, counters, hdr);
+    return hipGetLastError();
+}
+hipError_t launch_setup_rays(uint32_t* counters, uint32_t* err, uint32_t n, hipStream_t s) {
+    hipLaunchKernelGGL(k_setup_rays, dim3(1), dim3(256), 0, s, counters, err, n);
     return hipGetLastError();
 }
 hipError_t launch_generate(const LaunchShape& ls, const ViewDesc* views, const Segment* segs,

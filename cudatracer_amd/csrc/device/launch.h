@@ -24,6 +24,8 @@ struct LaunchShape {
 bool set_trace_grids(LaunchShape& ls, int num_cus);
 
 hipError_t launch_setup(uint32_t* counters, const ctd::ChunkHdr* hdr, hipStream_t s);
+// Counters for a ray-level launch of n rays (error flags, 16 words, cleared).
+hipError_t launch_setup_rays(uint32_t* counters, uint32_t* err, uint32_t n, hipStream_t s);
 hipError_t launch_generate(const LaunchShape& ls, const ctd::ViewDesc* views, const ctd::Segment* segs,
                            const ctd::ChunkHdr* hdr, uint32_t max_n, const ctd::Queues& q, hipStream_t s);
 // Closest hit for the rays [0, active[bounce]) of q.ray_o/ray_d[sel] into

@@ -96,6 +96,10 @@ struct ct_ctx {
     uint32_t lane_cap = 0;
     std::vector<void*> path_bufs;
     uint32_t spill_threads = 0;
+
+    // this device's copy of the allocation-free entry point's scratch array
+    // (its device address; runtime/noalloc.hip)
+    void* scratch = nullptr;
 };
 
 // Return the failure of a HIP call as CT_ERR_DEVICE on context `c`.
@@ -164,6 +168,12 @@ struct DeviceTrees {
 // Build the binary and quad trees of v's triangles on lane 0's stream; the
 // result equals the host build byte for byte.
 int device_build(ct_ctx* c, const ct_scene_view* v, DeviceTrees* out, ct_build_stats* st);
+}  // namespace ctr
+
+namespace ctr {
+// Look up the device address of this device's scratch array (c->device
+// current); once per device, kept for the process.
+int scratch_init(ct_ctx* c);
 }  // namespace ctr
 
 // Set the launch shape (block, LDS stack depth) and size the persistent grid.

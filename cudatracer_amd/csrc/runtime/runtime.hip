@@ -741,6 +741,10 @@ int ct_ctx_create(const ct_device_opts* opts, ct_ctx** out) {
             return CT_ERR_DEVICE;
         }
     }
+    if (scratch_init(c) != CT_OK) {
+        ct_ctx_destroy(c);
+        return CT_ERR_DEVICE;
+    }
     ctr_apply_shape(c, block, lds);
     if (d.chunk_paths > 0) c->chunk_default = (uint32_t)d.chunk_paths;
     *out = c;

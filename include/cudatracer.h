@@ -288,6 +288,21 @@ int ct_trace_rays_device(ct_ctx* ctx, const void* d_rays_o, const void* d_rays_d
                          uint32_t n, const ct_frame_opts* opts, void* d_hits,
                          float* kernel_ms);
 
+/* ct_trace_rays_device without any device allocation (no counterpart in the
+ * reference: for callers that cannot allocate, such as code under HIP graph
+ * capture, tight serving loops and harnesses that forbid hipMalloc).  The
+ * caller passes the rays and hits as for ct_trace_rays_device and must have
+ * uploaded the scene; the kernel's counters, error flags and traversal-stack
+ * spill live in one 32 MiB + 64 KiB device array per GPU, held until the
+ * process ends.  Limits: a traversal stack of 48 entries (deeper trees return
+ * CT_ERR_STACK_OVERFLOW, those rays reported as misses), and launch shapes
+ * whose spill does not fit the array are refused (CT_ERR_INVALID_ARG).  All
+ * calls on one GPU share the array: calls that overlap on one GPU race and
+ * must be serialised by the caller.  Same results as ct_trace_rays_device. */
+int ct_trace_rays_device_noalloc(ct_ctx* ctx, const void* d_rays_o, const void* d_rays_d,
+                                 uint32_t n, const ct_frame_opts* opts, void* d_hits,
+                                 float* kernel_ms);
+
 /* --------------------------------------------------- batched multi-view */
 /* Many camera views of uneven resolution and spp rendered by one set of
  * wavefront launches (one persistent path queue, hipGraph-captured bounce

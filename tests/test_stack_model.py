@@ -76,9 +76,10 @@ class LaneStackModel:
                 self.top = self.max_end + 4 * self.tid
 
 
+@pytest.mark.parametrize("max_stack", [K_MAX_STACK, 48])
 @pytest.mark.parametrize("block,depth", [(64, 8), (128, 16), (256, 16), (256, 24), (512, 32), (1024, 8),
                                          (256, 127)])
-def test_lane_stack_bounds_and_lifo(block, depth, max_stack=K_MAX_STACK):
+def test_lane_stack_bounds_and_lifo(block, depth, max_stack):
     rng = np.random.default_rng(block * 131 + depth + max_stack)
     for trial in range(60):
         tid = int(rng.integers(0, block))
@@ -123,3 +124,25 @@ def test_overflow_exactly_past_max_stack():
         st.push4([5, 6, 7, 8], [False, False, False, True])
         assert st.overflow
 
+
+def test_noalloc_grid_fits_scratch():
+    """The allocation-free launch (runtime/noalloc.hip) cuts the persistent
+    grid to the blocks whose spill regions fit the 32 MiB array: block b's
+    region starts at b * (48 - depth) * blockDim * 4, so every spilled
+    access of the model above lies inside the array for any launch shape the
+    context accepts; shapes that fit fewer than 32 blocks (one per fetch
+    range) are refused.  The control part holds the counters and flags."""
+    spill_bytes, ctl_bytes, stack = 32 << 20, 64 << 10, 48
+    counter_words = 192 + 2 * 32 * 96 + 64  # kCounterWords
+    assert counter_words * 4 <= ctl_bytes - 256
+    for block in (64, 128, 256, 512, 1024):
+        for depth in (1, 8, 16, 24, 32, 47, 48, 64, 128):
+            d = min(depth, stack)
+            per_block = (stack - d) * block * 4
+            fit = spill_bytes // per_block if per_block else 1 << 30
+            if fit < 32:
+                continue  # refused with CT_ERR_INVALID_ARG
+            grid = min(2048, fit)
+            assert grid * per_block <= spill_bytes
+            top = LaneStackModel(block, depth, block - 1, stack)
+            assert top.region == per_block
