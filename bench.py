@@ -96,6 +96,7 @@ def main() -> int:
     ap.add_argument("--summarize", nargs="+", metavar="DB", help="reduce rocprofv3 result databases and exit")
     ap.add_argument("--block", type=int, default=0, help="threads per block of the traversal kernels (0 = default)")
     ap.add_argument("--stack-lds", type=int, default=0, help="per-lane LDS stack entries (0 = default)")
+    ap.add_argument("--chunk", type=int, default=0, help="paths per wavefront chunk (0 = the library default)")
     ap.add_argument("--serve-chunk", type=int, default=0, help="paths per server chunk (0 = server default)")
     ap.add_argument("--sort", type=int, default=-1, choices=[-1, 0, 1],
                     help="block-local material sort before shading (-1 = the library default)")
@@ -119,7 +120,7 @@ def main() -> int:
         opts.sort_by_material = args.sort
     cam = scene.camera(W, H)
     st_scene = scene.stats()
-    ctx = ct.Context(0, block_size=args.block, stack_lds=args.stack_lds)
+    ctx = ct.Context(0, block_size=args.block, stack_lds=args.stack_lds, chunk_paths=args.chunk)
     if args.build_only:
         out = {"workload": WORKLOADS[args.config] % (st_scene["triangles"], args.traversal), "device": ctx.name(),
                "build": build_leg(ctx, scene, wide=args.traversal == "fast", reps=max(args.steps, 1))}

@@ -126,18 +126,17 @@ struct LaneStack {
     }
 };
 
-// Push the hit children of a quad node: sorted by the tagged tmin (nearest
-// popped first) with the reference's 5-comparator network, or (sort false,
-// any-hit queries) in slot order with slot 0 popped first.
-CT_DEV void push_children(bool sort, uint32_t hm, float m0, float m1, float m2, float m3, uint32_t e0, uint32_t e1,
+// Push the hit children of a quad node, sorted by the tagged tmin (nearest
+// popped first) with the reference's 5-comparator network.
+CT_DEV void push_children(uint32_t hm, float m0, float m1, float m2, float m3, uint32_t e0, uint32_t e1,
                           uint32_t e2, uint32_t e3, LaneStack& st) {
     if (hm == 0u) return;
     float s0 = m0, s1 = m1, s2 = m2, s3 = m3, tmp;
-    if (sort && s0 > s1) { tmp = s0; s0 = s1; s1 = tmp; }
-    if (sort && s2 > s3) { tmp = s2; s2 = s3; s3 = tmp; }
-    if (sort && s0 > s2) { tmp = s0; s0 = s2; s2 = tmp; }
-    if (sort && s1 > s3) { tmp = s1; s1 = s3; s3 = tmp; }
-    if (sort && s2 > s3) { tmp = s2; s2 = s3; s3 = tmp; }
+    if (s0 > s1) { tmp = s0; s0 = s1; s1 = tmp; }
+    if (s2 > s3) { tmp = s2; s2 = s3; s3 = tmp; }
+    if (s0 > s2) { tmp = s0; s0 = s2; s2 = tmp; }
+    if (s1 > s3) { tmp = s1; s1 = s3; s3 = tmp; }
+    if (s2 > s3) { tmp = s2; s2 = s3; s3 = tmp; }
     auto entry = [&](uint32_t slot) { return slot == 0 ? e0 : (slot == 1 ? e1 : (slot == 2 ? e2 : e3)); };
     const uint32_t q3 = fbits(s3) & 3u, q2 = fbits(s2) & 3u, q1 = fbits(s1) & 3u, q0 = fbits(s0) & 3u;
     st.push4(entry(q3), (hm >> q3) & 1u, entry(q2), (hm >> q2) & 1u, entry(q1), (hm >> q1) & 1u, entry(q0),
@@ -146,11 +145,8 @@ CT_DEV void push_children(bool sort, uint32_t hm, float m0, float m1, float m2, 
 
 // Quad node test (MBVHNode::intersect): four slabs in the reference's float
 // order, tmin tagged with the slot, the 5-comparator network, then pushes in
-// the order of traverseMBVH (sorted position 3 first).  sort = false skips
-// the network: an any-hit query's result does not depend on the visit order
-// (its t never shrinks, so every culling decision is the same).
-CT_DEV void node4_step(bool sort, const float4* __restrict__ nodes4, uint32_t e, f3 o, f3 inv, float t,
-                       LaneStack& st) {
+// the order of traverseMBVH and traverseMBVHShadow (sorted position 3 first).
+CT_DEV void node4_step(const float4* __restrict__ nodes4, uint32_t e, f3 o, f3 inv, float t, LaneStack& st) {
     const float4* n = nodes4 + 8 * (size_t)e;
     const float4 bminx = n[0], bmaxx = n[1], bminy = n[2], bmaxy = n[3], bminz = n[4], bmaxz = n[5];
     const float4 ch = n[6], cn = n[7];
@@ -185,7 +181,7 @@ CT_DEV void node4_step(bool sort, const float4* __restrict__ nodes4, uint32_t e,
     const uint32_t e1 = c1 == -1 ? fbits(ch.y) : (kLeafBit | fbits(ch.y));
     const uint32_t e2 = c2 == -1 ? fbits(ch.z) : (kLeafBit | fbits(ch.z));
     const uint32_t e3 = c3 == -1 ? fbits(ch.w) : (kLeafBit | fbits(ch.w));
-    push_children(sort, hm, m0, m1, m2, m3, e0, e1, e2, e3, st);
+    push_children(hm, m0, m1, m2, m3, e0, e1, e2, e3, st);
 }
 
 // Fast quad node test: the same tagged entry distances, hit flags and push
@@ -227,10 +223,8 @@ CT_DEV bool fast_ray(f3 o, f3 inv, uint32_t plane_flags) {
 }
 
 // nodes4f: the tree in the fast layout (k_prepare_quad): six plane rows and
-// the four stack entries, seven loads per node.  sort = false (any hit):
-// the comparators are off, slot 0 is popped first.
-CT_DEV void node4_fast(bool sort, const float4* __restrict__ nodes4f, uint32_t e, f3 o, f3 inv, float t,
-                       LaneStack& st) {
+// the four stack entries, seven loads per node.
+CT_DEV void node4_fast(const float4* __restrict__ nodes4f, uint32_t e, f3 o, f3 inv, float t, LaneStack& st) {
     const float4* n = nodes4f + 8 * (size_t)e;
     const float4 bminx = n[0], bmaxx = n[1], bminy = n[2], bmaxy = n[3], bminz = n[4], bmaxz = n[5];
     const float4 ent = n[6];
@@ -276,18 +270,18 @@ CT_DEV void node4_fast(bool sort, const float4* __restrict__ nodes4f, uint32_t e
     // kEmptyEntry), so the count test is the entry test: the sorted path
     // carries kNoEntry through the network and never pushes it.
     const uint32_t e0 = fbits(ent.x), e1 = fbits(ent.y), e2 = fbits(ent.z), e3 = fbits(ent.w);
-    const bool h0 = x0 > 0.0f && m0 <= x0 && m0 < t && (sort || e0 != kNoEntry);
-    const bool h1 = x1 > 0.0f && m1 <= x1 && m1 < t && (sort || e1 != kNoEntry);
-    const bool h2 = x2 > 0.0f && m2 <= x2 && m2 < t && (sort || e2 != kNoEntry);
-    const bool h3 = x3 > 0.0f && m3 <= x3 && m3 < t && (sort || e3 != kNoEntry);
+    const bool h0 = x0 > 0.0f && m0 <= x0 && m0 < t;
+    const bool h1 = x1 > 0.0f && m1 <= x1 && m1 < t;
+    const bool h2 = x2 > 0.0f && m2 <= x2 && m2 < t;
+    const bool h3 = x3 > 0.0f && m3 <= x3 && m3 < t;
     if (!(h0 || h1 || h2 || h3)) return;
     // comparators (0,1) and (2,3) on keys and entries, then (0,2) and (1,3)
-    // on entries (their keys are not compared again); none without sort
+    // on entries (their keys are not compared again)
     uint32_t q0 = h0 ? e0 : kNoEntry, q1 = h1 ? e1 : kNoEntry, q2 = h2 ? e2 : kNoEntry, q3 = h3 ? e3 : kNoEntry;
-    const bool s01 = sort && m0 > m1, s23 = sort && m2 > m3;
+    const bool s01 = m0 > m1, s23 = m2 > m3;
     const float a = s01 ? m1 : m0, b = s01 ? m0 : m1, c = s23 ? m3 : m2, d = s23 ? m2 : m3;
     const uint32_t ea = s01 ? q1 : q0, eb = s01 ? q0 : q1, ec = s23 ? q3 : q2, ed = s23 ? q2 : q3;
-    const bool s02 = sort && a > c, s13 = sort && b > d;
+    const bool s02 = a > c, s13 = b > d;
     q0 = s02 ? ec : ea;
     q2 = s02 ? ea : ec;
     q1 = s13 ? ed : eb;
@@ -513,8 +507,12 @@ CT_DEV void trace_persistent(const DevScene& sc, float eps, uint32_t n, uint32_t
                         else ++cnt.nodes;
                     }
                     if (ACCEL == 4) {
-                        if (fast) node4_fast(!any, sc.nodes4f, e, o, inv, t, st);
-                        else node4_step(!any, sc.nodes4, e, o, inv, t, st);
+                        // any-hit rays visit children nearest first too, like
+                        // traverseMBVHShadow (MBVHNode.cuh:165-205): in a
+                        // two-queue launch every lane then runs one code path
+                        // (slot order for any-hit lanes measured 2% slower)
+                        if (fast) node4_fast(sc.nodes4f, e, o, inv, t, st);
+                        else node4_step(sc.nodes4, e, o, inv, t, st);
                     } else {
                         // any-hit lanes of a two-queue launch sort too (one
                         // code path per wave)

@@ -168,11 +168,6 @@ struct or_scene {
     std::vector<uint32_t> prims;
     std::vector<Box> aabbs;
     bool built = false;
-    // any-hit visit order of the quad tree: false = the reference's sorted
-    // order (traverseMBVHShadow calls intersect, MBVHNode.cuh:165-205), true =
-    // slot order, slot 0 popped first (the order the kernels visit any-hit
-    // children in: the result is order-free, the work counts are not)
-    bool anyhit_slot_order = false;
     uint32_t ntris() const { return (uint32_t)indices.size() / 3; }
     const float* vtx(uint32_t tri, int k) const { return &vertices[3 * (size_t)indices[3 * (size_t)tri + k]]; }
     const float* nrm(uint32_t tri, int k) const { return &normals[3 * (size_t)indices[3 * (size_t)tri + k]]; }
@@ -477,11 +472,6 @@ bool traverse_bvh4(const or_scene& s, V3<T> o, V3<T> d, T* t, int* hit, T eps, C
         bool res[4];
         c.nodes++;
         mbvh_intersect(n, o, inv, *t, tmin, res);
-        if (AnyHit && s.anyhit_slot_order) {
-            for (int i = 3; i >= 0; --i)
-                if (res[i]) stack[++sp] = {n.childs[i], n.counts[i]};
-            continue;
-        }
         for (int i = 3; i >= 0; --i) {
             const int idx = tag_of(tmin[i]);
             if (res[idx]) stack[++sp] = {n.childs[idx], n.counts[idx]};
@@ -1286,10 +1276,6 @@ int or_refract(int precision, const float* I, const float* N, float eta, double*
     const bool ok = refract_dir(ld<float>(I), ld<float>(N), eta, &w);
     out[0] = w.x; out[1] = w.y; out[2] = w.z;
     return ok ? 1 : 0;
-}
-
-void or_scene_set_anyhit_order(or_scene* s, int slot_order) {
-    if (s) s->anyhit_slot_order = slot_order != 0;
 }
 
 int or_bsdf_local(int type, float roughness, const float* wo, const float* wi_in, float r1, float r2,

@@ -97,11 +97,6 @@ double or_fresnel_dielectric(double cos_i, double eta);
 /* glm::refract(I, N, eta) as the dielectric branch uses it (precision 32 or
  * 64): writes the direction, returns 1 unless total internal reflection. */
 int or_refract(int precision, const float* I, const float* N, float eta, double* out);
-/* Any-hit visit order of the quad tree: 0 = the reference's sorted order
- * (default), 1 = slot order with slot 0 first, the order of the kernels'
- * any-hit traversal.  Occlusion results do not depend on it; node and
- * triangle test counts do. */
-void or_scene_set_anyhit_order(or_scene* s, int slot_order);
 /* BSDF evaluate/sample in the local frame for tests: returns f (rgb=1 albedo)
  * and pdf.  mode 0 evaluate(wo, wi), mode 1 sample(wo, r1, r2) -> wi. */
 int or_bsdf_local(int type, float roughness, const float* wo, const float* wi_in, float r1, float r2,

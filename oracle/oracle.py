@@ -86,7 +86,6 @@ _sig = {
     "or_mf_sample_vndf32": (None, [C.c_int, C.c_float, _fptr, C.c_float, C.c_float, _fptr]),
     "or_fresnel_dielectric": (C.c_double, [C.c_double, C.c_double]),
     "or_refract": (C.c_int, [C.c_int, _fptr, _fptr, C.c_float, C.POINTER(C.c_double)]),
-    "or_scene_set_anyhit_order": (None, [_P, C.c_int]),
     "or_bsdf_local": (C.c_int, [C.c_int, C.c_float, _fptr, _fptr, C.c_float, C.c_float, _fptr, _fptr, _fptr]),
 }
 for _n, (_r, _a) in _sig.items():
@@ -149,11 +148,6 @@ class Scene:
         if build:
             if lib.or_scene_build(self._h) != 0:
                 raise RuntimeError("oracle build failed")
-
-    def set_anyhit_order(self, slot_order: bool) -> None:
-        """Quad-tree any-hit visit order: False = the reference's sorted order,
-        True = the kernels' slot order (same occlusion, their work counts)."""
-        lib.or_scene_set_anyhit_order(self._h, 1 if slot_order else 0)
 
     def texel(self, texture: int, u: float, v: float) -> np.ndarray:
         out = np.zeros(4, np.float32)

@@ -354,8 +354,8 @@ def test_ray_and_work_counts_equal_oracle(ct, orc, gpu_ctx, config, size):
     shadow rays of a frame (ct_frame_stats, from the device queue counters)
     and the closest-hit / any-hit node and triangle tests
     (ct_trace_frame_counted) equal the oracle's counts of the same frame
-    exactly.  The oracle visits any-hit children in the kernels' slot order
-    for this (set_anyhit_order); the occlusion results do not depend on it."""
+    exactly: the kernels visit the children of both kinds of rays in the
+    reference's order (MBVHNode.cuh:126-205), as the oracle does."""
     from conftest import get_scene
     scene, arrays, osc = get_scene(ct, orc, config)
     gpu_ctx.upload(scene)
@@ -364,11 +364,7 @@ def test_ray_and_work_counts_equal_oracle(ct, orc, gpu_ctx, config, size):
     cam = scene.camera(w, h)
     gpu_ctx.resize(w, h)
     st, work = gpu_ctx.trace_frame_counted(cam, opts)
-    osc.set_anyhit_order(True)
-    try:
-        _, ost = osc.render(cam, opts, w, h, 0, 1, threads=16)
-    finally:
-        osc.set_anyhit_order(False)
+    _, ost = osc.render(cam, opts, w, h, 0, 1, threads=16)
     got = {"primary_rays": st.primary_rays, "extension_rays": st.extension_rays, "shadow_rays": st.shadow_rays,
            "node_tests_closest": work[0], "tri_tests_closest": work[1], "node_tests_shadow": work[2],
            "tri_tests_shadow": work[3]}
