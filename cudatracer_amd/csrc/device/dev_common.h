@@ -82,6 +82,8 @@ struct ViewDesc {
     int32_t width, height, spp;
     uint32_t frame_base;
     uint64_t accum_offset;  // float4 index of the view's pixel 0 in the accumulator arena
+    uint32_t strip_mul;     // stride of the full pixel strips' order (pixel_of), coprime to their count
+    uint32_t pad;
 };
 
 // A run of consecutive path slots of one view inside a chunk.
@@ -98,16 +100,23 @@ struct Segment {
 // strips of kTileRows rows column by column, so the 64 primary rays of a
 // wave cover a 16 x 4 tile and share more of the tree than a run of one
 // row (4 rows measured ahead of 1, 8 and 16 on the 7.24M-triangle 4K scene).
-// The last strip has h % kTileRows rows when h is not a multiple.
-// A bijection of [0, w * h); seeds and the accumulator use the pixel itself,
+// The n full strips are taken in the order strip = k * mul mod n (mul
+// coprime to n, about 0.618 n), so any run of ordinals, such as one
+// wavefront chunk, covers strips spread over the whole image: a frame's
+// chunks get similar work (with contiguous halves the lower half of the 4K
+// scene took its lane twice as long as the upper half's).  The last strip
+// has h % kTileRows rows when h is not a multiple, and comes last.  A
+// bijection of [0, w * h); seeds and the accumulator use the pixel itself,
 // so the order changes no result.
 #ifndef CT_TILE_ROWS
 #define CT_TILE_ROWS 4
 #endif
 constexpr uint32_t kTileRows = CT_TILE_ROWS;
-CT_DEV uint32_t pixel_of(uint32_t ord, uint32_t w, uint32_t h) {
-    const uint32_t strip = ord / (kTileRows * w);
-    const uint32_t r = ord - strip * (kTileRows * w);
+CT_DEV uint32_t pixel_of(uint32_t ord, uint32_t w, uint32_t h, uint32_t mul) {
+    const uint32_t k = ord / (kTileRows * w);
+    const uint32_t r = ord - k * (kTileRows * w);
+    const uint32_t nfull = h / kTileRows;
+    const uint32_t strip = k < nfull ? (uint32_t)(((uint64_t)k * mul) % nfull) : k;
     const uint32_t y0 = strip * kTileRows;
     const uint32_t rows = min(kTileRows, h - y0);
     const uint32_t x = r / rows;

@@ -42,7 +42,7 @@ CT_DEV SlotInfo decode_slot(const ViewDesc* __restrict__ views, const Segment* _
     const uint32_t spp = (uint32_t)v.spp;
     const uint32_t ord = local / spp;
     SlotInfo si;
-    si.pixel = pixel_of(ord, (uint32_t)v.width, (uint32_t)v.height);
+    si.pixel = pixel_of(ord, (uint32_t)v.width, (uint32_t)v.height, v.strip_mul);
     si.frame = v.frame_base + (local - ord * spp);
     si.view = sg.view;
     return si;
@@ -77,7 +77,7 @@ __global__ void k_generate(const ViewDesc* __restrict__ views, const Segment* __
         const uint32_t spp = (uint32_t)v.spp;
         const uint32_t ord = local / spp;
         const uint32_t frame = v.frame_base + (local - ord * spp);
-        const uint32_t pixel = pixel_of(ord, (uint32_t)v.width, (uint32_t)v.height);
+        const uint32_t pixel = pixel_of(ord, (uint32_t)v.width, (uint32_t)v.height, v.strip_mul);
         const uint32_t x = pixel % (uint32_t)v.width, y = pixel / (uint32_t)v.width;
         uint32_t seed = ct_primary_seed(pixel, frame);
         const float jx = ct_rand_float(&seed);
@@ -601,7 +601,7 @@ __global__ void k_resolve(const ViewDesc* __restrict__ views, const Segment* __r
         const uint32_t p = sg.first_path / spp + (j - sg.pix_offset);
         const uint32_t g0 = max(p * spp, sg.first_path);
         const uint32_t g1 = min((p + 1) * spp, sg.first_path + sg.n);
-        float4* a = accum + v.accum_offset + pixel_of(p, (uint32_t)v.width, (uint32_t)v.height);
+        float4* a = accum + v.accum_offset + pixel_of(p, (uint32_t)v.width, (uint32_t)v.height, v.strip_mul);
         float4 acc = *a;
         for (uint32_t g = g0; g < g1; ++g) {
             const uint32_t s = sg.chunk_offset + (g - sg.first_path);

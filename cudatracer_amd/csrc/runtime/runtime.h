@@ -136,6 +136,8 @@ void drop_graphs(ct_ctx* c);
 void sync_lanes(ct_ctx* c);
 ctd::ViewDesc make_view(const ct_camera& cam, int width, int height, int spp, uint32_t frame_base,
                         uint64_t accum_offset);
+// Stride of the pixel-strip order for n full strips (see ctd::pixel_of).
+uint32_t strip_stride(uint32_t n);
 
 // Split a list of (view, first_path, n) runs into chunks of at most cap paths
 // and run them in order, alternating between the two lanes (a workload that

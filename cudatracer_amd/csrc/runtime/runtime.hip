@@ -240,6 +240,27 @@ int ensure_tables(ct_ctx* c, Lane& L, uint32_t nviews, uint32_t nsegs) {
     return CT_OK;
 }
 
+// The stride of pixel_of's strip order for n full strips: the integer
+// nearest 0.618 n (the golden-ratio stride, which spreads every run of
+// consecutive strips evenly over the image) that is coprime to n.
+uint32_t strip_stride(uint32_t n) {
+    if (n <= 2) return 1;
+    auto gcd = [](uint32_t a, uint32_t b) {
+        while (b) {
+            const uint32_t t = a % b;
+            a = b;
+            b = t;
+        }
+        return a;
+    };
+    const uint32_t target = (uint32_t)std::lround(0.6180339887498949 * n);
+    for (uint32_t d = 0; d < n; ++d) {
+        if (target + d < n && gcd(target + d, n) == 1) return target + d;
+        if (target >= d + 1 && gcd(target - d - 1, n) == 1) return target - d - 1;
+    }
+    return 1;
+}
+
 // Camera basis of sample(), Kernel.cu:274-290 (float, glm evaluation order).
 ViewDesc make_view(const ct_camera& cam, int width, int height, int spp, uint32_t frame_base,
                    uint64_t accum_offset) {
@@ -281,6 +302,7 @@ ViewDesc make_view(const ct_camera& cam, int width, int height, int spp, uint32_
     v.spp = spp;
     v.frame_base = frame_base;
     v.accum_offset = accum_offset;
+    v.strip_mul = strip_stride((uint32_t)height / ctd::kTileRows);
     return v;
 }
 

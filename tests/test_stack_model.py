@@ -146,3 +146,46 @@ def test_noalloc_grid_fits_scratch():
             assert grid * per_block <= spill_bytes
             top = LaneStackModel(block, depth, block - 1, stack)
             assert top.region == per_block
+
+
+def _strip_stride(n: int) -> int:
+    """Host rule of runtime.hip's strip_stride: the integer nearest 0.618 n
+    that is coprime to n (1 for n <= 2)."""
+    import math
+    if n <= 2:
+        return 1
+    t = int(math.floor(0.6180339887498949 * n + 0.5))
+    for d in range(n):
+        if t + d < n and math.gcd(t + d, n) == 1:
+            return t + d
+        if t >= d + 1 and math.gcd(t - d - 1, n) == 1:
+            return t - d - 1
+    return 1
+
+
+def _pixel_of(o, w: int, h: int, mul: int, rows_per_strip: int = 4):
+    """pixel_of of device/dev_common.h over an array of ordinals."""
+    o = np.asarray(o, dtype=np.int64)
+    k, r = np.divmod(o, rows_per_strip * w)
+    nfull = h // rows_per_strip
+    strip = np.where(k < nfull, (k * mul) % max(nfull, 1), k)
+    y0 = strip * rows_per_strip
+    rows = np.minimum(rows_per_strip, h - y0)
+    x, dy = np.divmod(r, rows)
+    return (y0 + dy) * w + x
+
+
+@pytest.mark.parametrize("w,h", [(1, 1), (1, 7), (5, 3), (33, 17), (64, 6), (37, 23), (256, 256), (480, 270),
+                                 (1920, 1080), (3840, 135), (3840, 2160), (7, 4 * 97 + 3)])
+def test_pixel_order_is_a_bijection_and_spreads_chunks(w, h):
+    """pixel_of (device/dev_common.h) maps the ordinals [0, w h) onto the
+    pixels one to one for every size; on frames of 256 rows and more each
+    half of the ordinals (a frame's two wavefront chunks) covers rows spread
+    over the whole image, so the chunks carry similar work."""
+    mul = _strip_stride(h // 4)
+    pix = _pixel_of(np.arange(w * h), w, h, mul)
+    assert np.array_equal(np.sort(pix), np.arange(w * h))
+    if h >= 256:
+        for half in (pix[: (w * h) // 2], pix[(w * h) // 2:]):
+            hist = np.histogram(half // w, bins=8, range=(0, h))[0]
+            assert hist.min() > 0.6 * hist.mean(), hist
