@@ -80,7 +80,7 @@ def main() -> int:
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--seed", type=int, default=7)
-    ap.add_argument("--traversal", default="exact", choices=["exact", "fast"])
+    ap.add_argument("--traversal", default="exact", choices=["exact", "fast", "compact"])
     ap.add_argument("--config", type=int, default=2, choices=[2, 3, 4],
                     help="2: atrium 1080p depth 8 (the headline workload); 3: displaced sphere 4K depth 16, RR; "
                          "4: the atrium with textures, normal maps and a sky texture")
@@ -115,7 +115,7 @@ def main() -> int:
 
     scene = ct.HostScene.generate({2: ct.SCENE_C2, 3: ct.SCENE_C3, 4: ct.SCENE_C2_TEXTURED}[args.config], args.seed)
     opts, W, H = scene.frame_opts()
-    opts.traversal = ct.TRAV_FAST if args.traversal == "fast" else ct.TRAV_EXACT
+    opts.traversal = getattr(ct, "TRAV_" + args.traversal.upper())
     if args.sort >= 0:
         opts.sort_by_material = args.sort
     cam = scene.camera(W, H)
@@ -267,7 +267,7 @@ def c3_leg(ct, ctx, args, steps: int = 10, warmup: int = 3) -> dict:
     scene = ct.HostScene.generate(ct.SCENE_C3, args.seed, build=False)
     ctx.upload_build(scene)
     opts, W, H = scene.frame_opts()
-    opts.traversal = ct.TRAV_FAST if args.traversal == "fast" else ct.TRAV_EXACT
+    opts.traversal = getattr(ct, "TRAV_" + args.traversal.upper())
     cam = scene.camera(W, H)
     ctx.resize(W, H)
     tm = timed_frames(ctx, cam, opts, steps, warmup)

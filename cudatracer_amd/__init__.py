@@ -31,7 +31,7 @@ CT_ERR_INVALID_SCENE = 9
 MAT_LIGHT, MAT_SPECULAR, MAT_FRESNEL, MAT_LAMBERTIAN, MAT_BECKMANN, MAT_GGX = 0, 1, 2, 3, 4, 5
 ACCEL_BVH2, ACCEL_BVH4 = 0, 1
 BSDF_REFERENCE, BSDF_TYPED = 0, 1
-TRAV_EXACT, TRAV_FAST = 0, 1
+TRAV_EXACT, TRAV_FAST, TRAV_COMPACT = 0, 1, 2
 LIGHTPDF_REFERENCE, LIGHTPDF_UNIFORM = 0, 1
 SCENE_C1, SCENE_C2, SCENE_C3, SCENE_C2_TEXTURED = 1, 2, 3, 4
 CT_BUILD_WIDE = 1

@@ -55,6 +55,7 @@ struct DevScene {
     const float4* nodes4f;  // the same nodes for the fast node test: six plane rows, four stack entries, zeros
     const float4* nodes2;   // 2 per binary node: {min, leftFirst}, {max, count}
     const uint4* fnodes;    // 8 per wide compressed node (128 B), layout in host/cnode.h
+    const uint4* cnodes;    // 4 per quad node in the compact layout (64 B, cnode4_step in device/traverse.h)
     const float4* sh_pos;   // 3 per primitive: v0 (w = material index bits), v1, v2
     const float4* sh_nrm;   // 4 per primitive: n0, n1, n2, centre normal
     const float4* mats;     // 2 per material: {albedo, roughness}, {type bits, ior, alpha, 0}
@@ -67,13 +68,14 @@ struct DevScene {
     int sky_enabled;
     int sky_tex;            // -1: the constant sky colour
     float sky_r, sky_g, sky_b;
-    uint32_t plane_flags;   // kPlaneNegZero | kPlaneNaN | kEmptyEntry over the quad tree
+    uint32_t plane_flags;   // kPlaneNegZero | kPlaneNaN | kEmptyEntry | kCompactBad over the quad tree
 };
 
 // Properties of the quad tree's box planes, found at upload (k_plane_flags).
 constexpr uint32_t kPlaneNegZero = 1u;  // some plane is -0.0
 constexpr uint32_t kPlaneNaN = 2u;      // some plane is NaN
 constexpr uint32_t kEmptyEntry = 4u;    // some empty slot (count 0) has a child other than 0xFFFFFFFF
+constexpr uint32_t kCompactBad = 8u;    // some node's boxes have no compact encoding (not finite, or too large)
 
 // One camera view (plain frame, batch member or served request).
 struct ViewDesc {

@@ -667,8 +667,14 @@ hipError_t launch_generate(const LaunchShape& ls, const ViewDesc* views, const S
     hipLaunchKernelGGL(k_generate, dim3(blocks), dim3(256), 0, s, views, segs, hdr, q);
     return hipGetLastError();
 }
-int tree_of(const TraceOpts& to) { return to.traversal == CT_TRAV_FAST ? 8 : (to.accel == CT_ACCEL_BVH2 ? 2 : 4); }
-int grid_of(const LaunchShape& ls, int tree) { return tree == 2 ? ls.grid2 : (tree == 4 ? ls.grid4 : ls.grid8); }
+int tree_of(const TraceOpts& to) {
+    if (to.traversal == CT_TRAV_FAST) return 8;
+    if (to.traversal == CT_TRAV_COMPACT) return kTreeCompact;
+    return to.accel == CT_ACCEL_BVH2 ? 2 : 4;
+}
+int grid_of(const LaunchShape& ls, int tree) {
+    return tree == 2 ? ls.grid2 : (tree == 4 ? ls.grid4 : (tree == kTreeCompact ? ls.gridc : ls.grid8));
+}
 
 template <int TREE>
 void extend_t(const LaunchShape& ls, const DevScene& sc, const TraceOpts& to, const Queues& q, int sel, int bounce,
@@ -695,6 +701,7 @@ hipError_t launch_extend(const LaunchShape& ls, const DevScene& sc, const TraceO
     switch (tree_of(to)) {
         case 2: extend_t<2>(ls, sc, to, q, sel, bounce, s); break;
         case 4: extend_t<4>(ls, sc, to, q, sel, bounce, s); break;
+        case kTreeCompact: extend_t<kTreeCompact>(ls, sc, to, q, sel, bounce, s); break;
         default: extend_t<8>(ls, sc, to, q, sel, bounce, s); break;
     }
     return hipGetLastError();
@@ -704,6 +711,7 @@ hipError_t launch_trace(const LaunchShape& ls, const DevScene& sc, const TraceOp
     switch (tree_of(to)) {
         case 2: trace_t<2>(ls, sc, to, q, hdr, sel, bounce, wstats, s); break;
         case 4: trace_t<4>(ls, sc, to, q, hdr, sel, bounce, wstats, s); break;
+        case kTreeCompact: trace_t<kTreeCompact>(ls, sc, to, q, hdr, sel, bounce, wstats, s); break;
         default: trace_t<8>(ls, sc, to, q, hdr, sel, bounce, wstats, s); break;
     }
     return hipGetLastError();
@@ -747,21 +755,76 @@ hipError_t launch_tonemap8(const float4* accum, uint32_t n, float samples, uint3
     hipLaunchKernelGGL(k_tonemap8, dim3(blocks), dim3(256), 0, s, accum, n, samples, out);
     return hipGetLastError();
 }
+// One axis of a compact quad node (cnode4_step, device/traverse.h): the
+// children's [lo, hi] (slots in `valid`) as 8-bit offsets on the grid
+// p + q 2^k.  The smallest step whose 256 points span the range (plus a step
+// of margin each side) first, coarser ones until every decoded plane
+// fmaf(q, 2^k, p) contains its box in float; quantisation rounds outwards
+// and pads one more step, as the wide nodes' encoder (host/cnode.cpp).
+// False if no step up to 2^kCompactMaxExp works or a plane is not finite.
+CT_DEV bool compact_axis(const float lo[4], const float hi[4], uint32_t valid, float& p_out, uint32_t& k_biased,
+                         uint32_t& wlo, uint32_t& whi) {
+    float mn = INFINITY, mx = -INFINITY;
+    for (int i = 0; i < 4; ++i) {
+        if (!((valid >> i) & 1u)) continue;
+        if (!isfinite(lo[i]) || !isfinite(hi[i])) return false;
+        mn = fminf(mn, lo[i]);
+        mx = fmaxf(mx, hi[i]);
+    }
+    p_out = 0.0f;
+    k_biased = 127u;
+    wlo = whi = 0u;
+    if (!valid) return true;
+    const double ext = (double)mx - (double)mn;
+    int k = -126;
+    if (ext > 0.0) k = max(-126, (int)ceil(log2(ext / 250.0)));
+    for (; k <= kCompactMaxExp; ++k) {
+        const double s = ldexp(1.0, k);
+        const float sf = (float)s;  // exact: -126 <= k <= 60
+        float p = (float)((double)mn - s);
+        if ((double)p > (double)mn - s) p = nextafterf(p, -INFINITY);
+        if (!isfinite(p)) return false;
+        bool ok = true;
+        uint32_t a = 0u, b = 0u;
+        for (int i = 0; i < 4 && ok; ++i) {
+            if (!((valid >> i) & 1u)) continue;
+            const double ql = floor(((double)lo[i] - (double)p) / s) - 1.0;
+            const double qh = ceil(((double)hi[i] - (double)p) / s) + 1.0;
+            if (ql < 0.0 || qh > 255.0 || ql > qh) { ok = false; break; }
+            if (!(fmaf((float)ql, sf, p) <= lo[i]) || !(fmaf((float)qh, sf, p) >= hi[i])) { ok = false; break; }
+            a |= (uint32_t)ql << (8 * i);
+            b |= (uint32_t)qh << (8 * i);
+        }
+        if (ok) {
+            p_out = p;
+            k_biased = (uint32_t)(k + 127);
+            wlo = a;
+            whi = b;
+            return true;
+        }
+    }
+    return false;
+}
+
 // Per quad node: the fast node test's layout (the six plane rows, the four
 // stack entries kLeafBit | first triangle record or the child node, a zero
-// row) and the tree's kPlaneNegZero / kPlaneNaN / kEmptyEntry flags (flags
-// cleared by the caller; one atomic per wave that found something).
+// row), the compact layout of cnode4_step (same entries, kNoEntry for an
+// empty slot) and the tree's kPlaneNegZero / kPlaneNaN / kEmptyEntry /
+// kCompactBad flags (flags cleared by the caller; one atomic per wave that
+// found something).
 __global__ void k_prepare_quad(const float4* __restrict__ nodes4, uint32_t n4, float4* __restrict__ fast,
-                               uint32_t* flags) {
+                               uint4* __restrict__ compact, uint32_t* flags) {
     uint32_t f = 0;
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n4; i += gridDim.x * blockDim.x) {
         const float4* n = nodes4 + 8 * (size_t)i;
         float4* o = fast + 8 * (size_t)i;
+        float pl[6][4];
         for (int r = 0; r < 6; ++r) {
             const float4 v = n[r];
             o[r] = v;
             const float c[4] = {v.x, v.y, v.z, v.w};
             for (int k = 0; k < 4; ++k) {
+                pl[r][k] = c[k];
                 if (__float_as_uint(c[k]) == 0x80000000u) f |= ctd::kPlaneNegZero;
                 if (isnan(c[k])) f |= ctd::kPlaneNaN;
             }
@@ -771,23 +834,36 @@ __global__ void k_prepare_quad(const float4* __restrict__ nodes4, uint32_t n4, f
                                    __float_as_uint(ch.w)};
         const uint32_t count[4] = {__float_as_uint(cn.x), __float_as_uint(cn.y), __float_as_uint(cn.z),
                                    __float_as_uint(cn.w)};
-        uint32_t e[4];
+        uint32_t e[4], ce[4], valid = 0u;
         for (int k = 0; k < 4; ++k) {
             e[k] = count[k] == 0xFFFFFFFFu ? child[k] : (ctd::kLeafBit | child[k]);
             if (count[k] == 0u && child[k] != 0xFFFFFFFFu) f |= ctd::kEmptyEntry;
+            ce[k] = count[k] == 0u ? ctd::kNoEntry : e[k];
+            if (count[k] != 0u) valid |= 1u << k;
         }
         o[6] = make_float4(__uint_as_float(e[0]), __uint_as_float(e[1]), __uint_as_float(e[2]), __uint_as_float(e[3]));
         o[7] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        float p[3];
+        uint32_t kb[3], wl[3], wh[3];
+        for (int a = 0; a < 3; ++a)
+            if (!compact_axis(pl[2 * a], pl[2 * a + 1], valid, p[a], kb[a], wl[a], wh[a])) f |= ctd::kCompactBad;
+        uint4* w = compact + 4 * (size_t)i;
+        w[0] = make_uint4(__float_as_uint(p[0]), __float_as_uint(p[1]), __float_as_uint(p[2]),
+                          kb[0] | (kb[1] << 8) | (kb[2] << 16));
+        w[1] = make_uint4(wl[0], wh[0], wl[1], wh[1]);
+        w[2] = make_uint4(wl[2], wh[2], 0u, 0u);
+        w[3] = make_uint4(ce[0], ce[1], ce[2], ce[3]);
     }
     uint32_t w = 0;
-    for (uint32_t bit : {ctd::kPlaneNegZero, ctd::kPlaneNaN, ctd::kEmptyEntry})
+    for (uint32_t bit : {ctd::kPlaneNegZero, ctd::kPlaneNaN, ctd::kEmptyEntry, ctd::kCompactBad})
         if (__ballot(f & bit)) w |= bit;
     if (__lane_id() == 0 && w) atomicOr(flags, w);
 }
-hipError_t launch_prepare_quad(const float4* nodes4, uint32_t n4, float4* fast, uint32_t* flags, hipStream_t s) {
+hipError_t launch_prepare_quad(const float4* nodes4, uint32_t n4, float4* fast, uint4* compact, uint32_t* flags,
+                               hipStream_t s) {
     const uint32_t blocks = std::min<uint32_t>((n4 + 255) / 256, 4096u);
     if (blocks == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_prepare_quad, dim3(blocks), dim3(256), 0, s, nodes4, n4, fast, flags);
+    hipLaunchKernelGGL(k_prepare_quad, dim3(blocks), dim3(256), 0, s, nodes4, n4, fast, compact, flags);
     return hipGetLastError();
 }
 hipError_t launch_tonemap(const float4* accum, uint32_t n, float samples, float4* out, hipStream_t s) {
@@ -803,6 +879,8 @@ hipError_t launch_shadow_flags(const LaunchShape& ls, const DevScene& sc, const 
     const dim3 grid(grid_of(ls, tree)), block(ls.block);
     if (tree == 2) hipLaunchKernelGGL(k_shadow_flags<2>, grid, block, lds, s, sc, to, q, n, ls.lds_depth, out);
     else if (tree == 4) hipLaunchKernelGGL(k_shadow_flags<4>, grid, block, lds, s, sc, to, q, n, ls.lds_depth, out);
+    else if (tree == kTreeCompact)
+        hipLaunchKernelGGL(k_shadow_flags<kTreeCompact>, grid, block, lds, s, sc, to, q, n, ls.lds_depth, out);
     else hipLaunchKernelGGL(k_shadow_flags<8>, grid, block, lds, s, sc, to, q, n, ls.lds_depth, out);
     return hipGetLastError();
 }
@@ -827,14 +905,16 @@ static int per_cu(int block, size_t lds) {
 bool set_trace_grids(LaunchShape& ls, int num_cus) {
     const size_t lds = (size_t)ls.lds_depth * ls.block * 4;
     const int p2 = per_cu<2>(ls.block, lds), p4 = per_cu<4>(ls.block, lds), p8 = per_cu<8>(ls.block, lds);
-    if (p2 <= 0 || p4 <= 0 || p8 <= 0) return false;
+    const int pc = per_cu<ctd::kTreeCompact>(ls.block, lds);
+    if (p2 <= 0 || p4 <= 0 || p8 <= 0 || pc <= 0) return false;
     // at least one block per fetch range (the persistent kernels drain ranges
     // by block id)
     const int lo = ctd::kFetchRanges;
     ls.grid2 = std::max(lo, num_cus * p2);
     ls.grid4 = std::max(lo, num_cus * p4);
     ls.grid8 = std::max(lo, num_cus * p8);
-    ls.trace_blocks = std::max(ls.grid2, std::max(ls.grid4, ls.grid8));
+    ls.gridc = std::max(lo, num_cus * pc);
+    ls.trace_blocks = std::max(std::max(ls.grid2, ls.grid4), std::max(ls.grid8, ls.gridc));
     return true;
 }
 }  // namespace ctk

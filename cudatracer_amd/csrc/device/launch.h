@@ -13,7 +13,7 @@ struct LaunchShape {
     int block = 256;          // threads per block for extend / connect
     int lds_depth = 16;       // per-lane LDS stack entries
     int trace_blocks = 1024;  // largest persistent grid (sizes the stack spill area)
-    int grid2 = 1024, grid4 = 1024, grid8 = 1024;  // persistent grids per tree
+    int grid2 = 1024, grid4 = 1024, grid8 = 1024, gridc = 1024;  // persistent grids per tree (c: compact quad)
     int shade_blocks = 1024;  // grid-stride grid of shade
     int max_blocks = 65536;   // cap for the simple grid-stride kernels
 };
@@ -48,9 +48,11 @@ hipError_t launch_resolve(const LaunchShape& ls, const ctd::ViewDesc* views, con
                           hipStream_t s);
 hipError_t launch_tonemap(const float4* accum, uint32_t n, float samples, float4* out, hipStream_t s);
 hipError_t launch_tonemap8(const float4* accum, uint32_t n, float samples, uint32_t* out, hipStream_t s);
-// The fast node test's copy of a quad tree (8 float4 per node) and the
-// tree's plane / slot flags (*flags cleared first).
-hipError_t launch_prepare_quad(const float4* nodes4, uint32_t n4, float4* fast, uint32_t* flags, hipStream_t s);
+// The fast node test's copy of a quad tree (8 float4 per node), its compact
+// layout (4 uint4 per node) and the tree's plane / slot flags (*flags
+// cleared first).
+hipError_t launch_prepare_quad(const float4* nodes4, uint32_t n4, float4* fast, uint4* compact, uint32_t* flags,
+                               hipStream_t s);
 hipError_t launch_shadow_flags(const LaunchShape& ls, const ctd::DevScene& sc, const ctd::TraceOpts& to,
                                const ctd::Queues& q, uint32_t n, uint8_t* out, hipStream_t s);
 

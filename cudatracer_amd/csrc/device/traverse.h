@@ -289,6 +289,79 @@ CT_DEV void node4_fast(const float4* __restrict__ nodes4f, uint32_t e, f3 o, f3 
     st.push4(q3, q3 != kNoEntry, q2, q2 != kNoEntry, q1, q1 != kNoEntry, q0, q0 != kNoEntry);
 }
 
+// The compact quad tree (ACCEL kTreeCompact): node i of the quad tree in 16
+// words (64 bytes, half a cache line, against the 112 bytes node4_fast
+// reads), written by k_prepare_quad:
+//   0..2    p.x, p.y, p.z: the node's grid corner (float bits)
+//   3       biased exponents of the x, y, z grid steps 2^k (bytes 0..2)
+//   4, 5    low and high x offsets of slots 0..3 (byte i = slot i)
+//   6, 7    low and high y offsets;  8, 9  low and high z offsets
+//   10, 11  zero
+//   12..15  the four stack entries (kNoEntry for an empty slot)
+// A child's box is [p + lo 2^k, p + hi 2^k] per axis, quantised outwards with
+// one more step of margin (as the wide nodes, host/cnode.h), so it contains
+// the quad node's box.  The ray moves into the node's grid once per node,
+// t = q (2^k / d) + (p - o) / d: one conversion and one fma per plane, with
+// the hardware min/max (no slab value is NaN for a compact_ray).  Entry
+// distances carry the slot tag and go through node4_fast's network, so the
+// children are visited nearest first, in the exact traversal's order except
+// where the quantised distances order two children differently.
+constexpr int kTreeCompact = 5;
+
+// Rays the compact node test takes: finite origin, |1/d| <= 2^64 (so
+// 2^k / d stays finite for the steps k <= kCompactMaxExp the encoder allows).
+// The others use node4_step on the same tree.
+constexpr int kCompactMaxExp = 60;
+CT_DEV bool compact_ray(f3 o, f3 inv) {
+    constexpr float kMax = 3.40282347e38f, kInvMax = 18446744073709551616.0f;  // 2^64
+    return fabsf(inv.x) <= kInvMax && fabsf(inv.y) <= kInvMax && fabsf(inv.z) <= kInvMax && fabsf(o.x) <= kMax &&
+           fabsf(o.y) <= kMax && fabsf(o.z) <= kMax;
+}
+
+CT_DEV void cnode4_step(const uint4* __restrict__ cnodes, uint32_t e, f3 o, f3 inv, float t, LaneStack& st) {
+    const uint4* n = cnodes + 4 * (size_t)e;
+    const uint4 h = n[0], qa = n[1], qb = n[2], ent = n[3];
+    const float ax = __uint_as_float((h.w & 0xFFu) << 23) * inv.x;
+    const float ay = __uint_as_float(((h.w >> 8) & 0xFFu) << 23) * inv.y;
+    const float az = __uint_as_float(((h.w >> 16) & 0xFFu) << 23) * inv.z;
+    const float bx = (__uint_as_float(h.x) - o.x) * inv.x, by = (__uint_as_float(h.y) - o.y) * inv.y,
+                bz = (__uint_as_float(h.z) - o.z) * inv.z;
+    float m0, m1, m2, m3, x0, x1, x2, x3;
+#define CT_CSLAB(MN, MX, SH)                                                                              \
+    {                                                                                                     \
+        const float lx = fmaf((float)((qa.x >> SH) & 0xFFu), ax, bx), hx = fmaf((float)((qa.y >> SH) & 0xFFu), ax, bx); \
+        const float ly = fmaf((float)((qa.z >> SH) & 0xFFu), ay, by), hy = fmaf((float)((qa.w >> SH) & 0xFFu), ay, by); \
+        const float lz = fmaf((float)((qb.x >> SH) & 0xFFu), az, bz), hz = fmaf((float)((qb.y >> SH) & 0xFFu), az, bz); \
+        MN = fmaxf(fmaxf(fminf(lx, hx), fminf(ly, hy)), fminf(lz, hz));                                  \
+        MX = fminf(fminf(fmaxf(lx, hx), fmaxf(ly, hy)), fmaxf(lz, hz));                                  \
+    }
+    CT_CSLAB(m0, x0, 0u)
+    CT_CSLAB(m1, x1, 8u)
+    CT_CSLAB(m2, x2, 16u)
+    CT_CSLAB(m3, x3, 24u)
+#undef CT_CSLAB
+    m0 = tag(m0, 0u);
+    m1 = tag(m1, 1u);
+    m2 = tag(m2, 2u);
+    m3 = tag(m3, 3u);
+    const bool h0 = x0 > 0.0f && m0 <= x0 && m0 < t;
+    const bool h1 = x1 > 0.0f && m1 <= x1 && m1 < t;
+    const bool h2 = x2 > 0.0f && m2 <= x2 && m2 < t;
+    const bool h3 = x3 > 0.0f && m3 <= x3 && m3 < t;
+    if (!(h0 || h1 || h2 || h3)) return;
+    uint32_t q0 = h0 ? ent.x : kNoEntry, q1 = h1 ? ent.y : kNoEntry, q2 = h2 ? ent.z : kNoEntry,
+             q3 = h3 ? ent.w : kNoEntry;
+    const bool s01 = m0 > m1, s23 = m2 > m3;
+    const float a = s01 ? m1 : m0, b = s01 ? m0 : m1, c = s23 ? m3 : m2, d = s23 ? m2 : m3;
+    const uint32_t ea = s01 ? q1 : q0, eb = s01 ? q0 : q1, ec = s23 ? q3 : q2, ed = s23 ? q2 : q3;
+    const bool s02 = a > c, s13 = b > d;
+    q0 = s02 ? ec : ea;
+    q2 = s02 ? ea : ec;
+    q1 = s13 ? ed : eb;
+    q3 = s13 ? eb : ed;
+    st.push4(q3, q3 != kNoEntry, q2, q2 != kNoEntry, q1, q1 != kNoEntry, q0, q0 != kNoEntry);
+}
+
 // Wide compressed node (128 B, up to 8 children, layout in host/cnode.h).
 // The ray is moved into the node's grid once per node, t = q * (s / d) +
 // (p - o) / d, so each child plane costs one conversion and one fma (the
@@ -381,8 +454,8 @@ CT_DEV void node2_step(const float4* __restrict__ nodes2, uint32_t lf, f3 o, f3 
 
 // ----------------------------------------------------------------------
 // Persistent traversal driver.  Source: how ray i is loaded; Sink: what is
-// done with the result.  ACCEL 4 = quad tree, 8 = wide compressed tree,
-// 2 = binary tree.
+// done with the result.  ACCEL 4 = quad tree, kTreeCompact = the quad tree
+// in the compact layout, 8 = wide compressed tree, 2 = binary tree.
 struct RayIn {
     f3 o, d;
     float tmax;
@@ -482,7 +555,8 @@ CT_DEV void trace_persistent(const DevScene& sc, float eps, uint32_t n, uint32_t
                     t = r.tmax;
                     prim = -1;
                     inv = mk3(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
-                    fast = ACCEL == 4 && fast_ray(o, inv, sc.plane_flags);
+                    fast = (ACCEL == 4 && fast_ray(o, inv, sc.plane_flags)) ||
+                           (ACCEL == kTreeCompact && compact_ray(o, inv));
                     st.reset();
                     st.push4(0u, true, 0u, false, 0u, false, 0u, false);
                     k = kNoLeaf;
@@ -498,7 +572,7 @@ CT_DEV void trace_persistent(const DevScene& sc, float eps, uint32_t n, uint32_t
         // ---- one step: pop (+ node test) and/or one triangle
         if (k == kNoLeaf) {
             const uint32_t e = st.top_pop();
-            if (ACCEL == 4 || ACCEL == 8) {
+            if (ACCEL == 4 || ACCEL == kTreeCompact || ACCEL == 8) {
                 if (e & kLeafBit) {
                     k = e & ~kLeafBit;
                 } else {
@@ -512,6 +586,9 @@ CT_DEV void trace_persistent(const DevScene& sc, float eps, uint32_t n, uint32_t
                         // two-queue launch every lane then runs one code path
                         // (slot order for any-hit lanes measured 2% slower)
                         if (fast) node4_fast(sc.nodes4f, e, o, inv, t, st);
+                        else node4_step(sc.nodes4, e, o, inv, t, st);
+                    } else if (ACCEL == kTreeCompact) {
+                        if (fast) cnode4_step(sc.cnodes, e, o, inv, t, st);
                         else node4_step(sc.nodes4, e, o, inv, t, st);
                     } else {
                         // any-hit lanes of a two-queue launch sort too (one

@@ -282,29 +282,37 @@ int upload_shading(ct_ctx* c, const Shading& sh, ctd::DevScene& sc) {
     return r;
 }
 
-// The fast node test's copy of the uploaded quad tree (sc.nodes4f) and the
-// tree's flags (sc.plane_flags): the fast test needs planes that are not NaN,
-// empty slots whose child is 0xFFFFFFFF, and origins without +0 components
-// only where some plane is -0.
+// The fast node test's copy of the uploaded quad tree (sc.nodes4f), its
+// compact layout (sc.cnodes, 64 bytes per node) and the tree's flags
+// (sc.plane_flags): the fast test needs planes that are not NaN, empty slots
+// whose child is 0xFFFFFFFF, and origins without +0 components only where
+// some plane is -0; the compact traversal needs every node encoded.
 int prepare_quad(ct_ctx* c, ctd::DevScene& sc) {
     sc.plane_flags = 0;
     sc.nodes4f = nullptr;
+    sc.cnodes = nullptr;
     if (!sc.nodes4 || sc.n_nodes4 == 0) return CT_OK;
     void* fast = nullptr;
     int r = dalloc(c, &fast, (size_t)sc.n_nodes4 * sizeof(ct_mbvh_node), "fast quad nodes");
     if (r != CT_OK) return r;
     c->scene_bufs.push_back(fast);
+    void* compact = nullptr;
+    r = dalloc(c, &compact, (size_t)sc.n_nodes4 * 64, "compact quad nodes");
+    if (r != CT_OK) return r;
+    c->scene_bufs.push_back(compact);
     uint32_t* d = nullptr;
     HIPCHK(hipMalloc(&d, sizeof(uint32_t)), "quad tree flags");
     const hipStream_t s = c->ln[0].stream;
     uint32_t h = 0;
     hipError_t e = hipMemsetAsync(d, 0, sizeof(uint32_t), s);
-    if (e == hipSuccess) e = ctk::launch_prepare_quad(sc.nodes4, sc.n_nodes4, reinterpret_cast<float4*>(fast), d, s);
+    if (e == hipSuccess) e = ctk::launch_prepare_quad(sc.nodes4, sc.n_nodes4, reinterpret_cast<float4*>(fast),
+                                                           static_cast<uint4*>(compact), d, s);
     if (e == hipSuccess) e = hipMemcpyAsync(&h, d, sizeof(uint32_t), hipMemcpyDeviceToHost, s);
     if (e == hipSuccess) e = hipStreamSynchronize(s);
     (void)hipFree(d);
     if (e != hipSuccess) return ctr::hip_fail(c, e, "quad tree flags");
     sc.nodes4f = reinterpret_cast<const float4*>(fast);
+    sc.cnodes = static_cast<const uint4*>(compact);
     sc.plane_flags = h;
     return CT_OK;
 }

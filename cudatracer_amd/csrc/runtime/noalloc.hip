@@ -84,6 +84,7 @@ extern "C" int ct_trace_rays_device_noalloc(ct_ctx* c, const void* d_rays_o, con
     ls.grid2 = std::min(ls.grid2, cap);
     ls.grid4 = std::min(ls.grid4, cap);
     ls.grid8 = std::min(ls.grid8, cap);
+    ls.gridc = std::min(ls.gridc, cap);
     uint8_t* base = static_cast<uint8_t*>(c->scratch);  // the device address, never the symbol itself
     ctd::Queues q{};
     q.ray_o[0] = (float4*)d_rays_o;

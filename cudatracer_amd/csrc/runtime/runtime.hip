@@ -90,7 +90,8 @@ int check_opts(const ct_ctx* c, const ct_frame_opts& o, const char** why) {
     if (o.spp < 1 || o.spp > (1 << 20)) return bad(CT_ERR_INVALID_ARG, "spp must be in [1, 2^20]");
     if (o.accel != CT_ACCEL_BVH2 && o.accel != CT_ACCEL_BVH4) return bad(CT_ERR_INVALID_ARG, "bad accel");
     if (o.bsdf_mode != CT_BSDF_REFERENCE && o.bsdf_mode != CT_BSDF_TYPED) return bad(CT_ERR_INVALID_ARG, "bad bsdf_mode");
-    if (o.traversal != CT_TRAV_EXACT && o.traversal != CT_TRAV_FAST) return bad(CT_ERR_INVALID_ARG, "bad traversal");
+    if (o.traversal != CT_TRAV_EXACT && o.traversal != CT_TRAV_FAST && o.traversal != CT_TRAV_COMPACT)
+        return bad(CT_ERR_INVALID_ARG, "bad traversal");
     if (o.light_pdf != CT_LIGHTPDF_REFERENCE && o.light_pdf != CT_LIGHTPDF_UNIFORM)
         return bad(CT_ERR_INVALID_ARG, "bad light_pdf");
     if (!(o.triangle_epsilon >= 0.0f) || !(o.normal_epsilon >= 0.0f) || !(o.dist_epsilon >= 0.0f))
@@ -100,6 +101,10 @@ int check_opts(const ct_ctx* c, const ct_frame_opts& o, const char** why) {
     if (o.accel == CT_ACCEL_BVH4 && !c->has_bvh4) return bad(CT_ERR_INVALID_ARG, "scene has no quad tree");
     if (o.traversal == CT_TRAV_FAST && (c->sc.n_fnodes == 0 || c->wide_stack_bound > (uint32_t)ctd::kMaxStack))
         return bad(CT_ERR_INVALID_ARG, "fast traversal not available for this scene (wide tree too deep)");
+    if (o.traversal == CT_TRAV_COMPACT && o.accel != CT_ACCEL_BVH4)
+        return bad(CT_ERR_INVALID_ARG, "compact traversal runs on the quad tree (accel CT_ACCEL_BVH4)");
+    if (o.traversal == CT_TRAV_COMPACT && (!c->sc.cnodes || (c->sc.plane_flags & ctd::kCompactBad)))
+        return bad(CT_ERR_INVALID_ARG, "compact traversal not available for this scene (a box has no 8-bit encoding)");
     return CT_OK;
 }
 
@@ -407,7 +412,7 @@ static std::string graph_key(const ct_ctx* c, const Lane& L, const ct_frame_opts
     os << c->lane_cap << ':' << o.max_depth << ':' << o.rr_start << ':' << o.accel << ':' << o.bsdf_mode << ':'
        << o.traversal << ':' << o.shadow << ':' << o.indirect << ':' << o.normal_epsilon << ':' << o.dist_epsilon
        << ':' << o.triangle_epsilon << ':' << o.light_pdf << ':' << o.sort_by_material << ':' << c->shape.block << ':'
-       << c->shape.lds_depth << ':' << c->shape.grid2 << ':' << c->shape.grid4 << ':' << c->shape.grid8 << ':'
+       << c->shape.lds_depth << ':' << c->shape.grid2 << ':' << c->shape.grid4 << ':' << c->shape.grid8 << ':' << c->shape.gridc << ':'
        << c->shape.shade_blocks << ':' << (const void*)L.d_views << ':' << (const void*)L.d_segs << ':'
        << (const void*)L.q.counters << ':' << (const void*)c->sc.tris << ':' << (const void*)c->sc.mats;
     return os.str();
@@ -711,7 +716,7 @@ void ctr_apply_shape(ct_ctx* c, int block, int lds_depth) {
     c->shape.block = block;
     c->shape.lds_depth = lds_depth;
     if (!ctk::set_trace_grids(c->shape, c->num_cus)) {
-        c->shape.grid2 = c->shape.grid4 = c->shape.grid8 = c->shape.trace_blocks = std::max(8, c->num_cus * 4);
+        c->shape.grid2 = c->shape.grid4 = c->shape.grid8 = c->shape.gridc = c->shape.trace_blocks = std::max(8, c->num_cus * 4);
     }
     c->shape.shade_blocks = std::max(1, c->num_cus * 8);
 }

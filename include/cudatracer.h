@@ -153,8 +153,13 @@ enum { CT_ACCEL_BVH2 = 0, CT_ACCEL_BVH4 = 1 };
 enum { CT_BSDF_REFERENCE = 0, CT_BSDF_TYPED = 1 };
 /* CT_TRAV_EXACT: reference node order and culling, bit-exact hit ids.
  * CT_TRAV_FAST: wide compressed tree (8 children, 128-byte nodes) for either
- * accel setting; hit distances equal, ids equal except exact t-ties. */
-enum { CT_TRAV_EXACT = 0, CT_TRAV_FAST = 1 };
+ * accel setting; hit distances equal, ids equal except exact t-ties.
+ * CT_TRAV_COMPACT: the quad tree itself (MBVHNode.cuh:28-63: same nodes,
+ * children and leaves) re-laid as 64-byte nodes with 8-bit child boxes on a
+ * per-node grid (conservative); CT_ACCEL_BVH4 only.  Children are visited
+ * nearest first as in exact mode; hit distances equal, ids equal except
+ * exact t-ties. */
+enum { CT_TRAV_EXACT = 0, CT_TRAV_FAST = 1, CT_TRAV_COMPACT = 2 };
 /* Light-selection pdf in the MIS weights.  CT_LIGHTPDF_REFERENCE multiplies
  * the per-triangle solid-angle pdf by the light count as the reference does
  * (Kernel.cu:128,184), which biases direct light by 1/count^2 when a scene

@@ -67,6 +67,42 @@ def test_c3_rays_bit_exact(ct, orc, gpu_ctx, c3):
     assert np.array_equal(occ_gpu, occ_ref)
 
 
+def test_c3_compact_rays(ct, orc, gpu_ctx, c3):
+    """The compact quad tree (64-byte nodes, 8-bit child boxes) on C3, whose
+    small boxes lie far from the origin relative to their size: closest-hit
+    t equal to the oracle's exact traversal except on a few rays (where t
+    then differs by ulps) and ids except exact t ties; any-hit flags equal."""
+    scene, arrays, osc = c3
+    gpu_ctx.upload_build(scene)
+    opts, _, _ = scene.frame_opts()
+    opts.accel = ct.ACCEL_BVH4
+    opts.traversal = ct.TRAV_COMPACT
+    n = 1 << 20
+    o, d = ray_set(scene, arrays, n, seed=104)
+    print(f"C3 compact closest hit: {n} rays", flush=True)
+    t_gpu, p_gpu = gpu_ctx.trace_rays(o, d, opts)
+    t_ref, p_ref = osc.trace(o, d, 1, 32, opts.triangle_epsilon, threads=16)
+    t_ref32 = t_ref.astype(np.float32)
+    same_t = t_gpu.view(np.uint32) == t_ref32.view(np.uint32)
+    diff_t = np.flatnonzero(~same_t)
+    ties = np.flatnonzero(same_t & (p_gpu != p_ref))
+    print(f"C3 compact: hits {int(np.sum(p_ref >= 0))}, t differs {diff_t.size}, id ties {ties.size}")
+    assert diff_t.size <= n // 100000
+    if diff_t.size:
+        rel = np.abs(t_gpu[diff_t] - t_ref32[diff_t]) / t_ref32[diff_t]
+        assert np.all(rel <= 1e-6), rel.max()
+    assert ties.size <= n // 10000
+    m = 1 << 18
+    rng = np.random.default_rng(35)
+    ro, rd = ray_set(scene, arrays, m, seed=36)
+    tm = rng.uniform(0.0, 300.0, m).astype(np.float32)
+    occ_gpu = gpu_ctx.trace_shadow_rays(ro, rd, tm, opts)
+    occ_ref = osc.trace_shadow(ro, rd, tm, 1, 32, opts.triangle_epsilon, threads=16)
+    bad = int(np.sum(occ_gpu != occ_ref))
+    print(f"C3 compact any hit: occluded {int(occ_ref.sum())} of {m}, differ {bad}")
+    assert bad <= m // 100000
+
+
 def test_c3_radiance_band_depth16_roulette(ct, orc, gpu_ctx, c3):
     scene, arrays, osc = c3
     gpu_ctx.upload_build(scene)

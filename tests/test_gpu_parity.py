@@ -56,27 +56,30 @@ def test_closest_hit_matches_brute_force_c1(ct, orc, gpu_ctx, c1):
 
 
 @pytest.mark.parametrize("config", [1, 2])
-def test_fast_traversal_closest_hit(ct, orc, gpu_ctx, config):
-    """The wide compressed tree (CT_TRAV_FAST) against the oracle's exact
-    traversal: the same triangle records are tested with the same arithmetic,
-    so t is bit-identical wherever both find the same nearest surface; ids
-    differ only on exact t ties.  The few rays whose t differs are ones where
-    the exact traversal's tagged tmin (the reference's low-bit slot tag) culls
-    a box within ulps of the current hit; there the difference is a few ulps."""
+@pytest.mark.parametrize("mode", ["fast", "compact"])
+def test_fast_traversal_closest_hit(ct, orc, gpu_ctx, config, mode):
+    """The wide compressed tree (CT_TRAV_FAST) and the compact quad tree
+    (CT_TRAV_COMPACT, 64-byte nodes with 8-bit child boxes) against the
+    oracle's exact traversal: the same triangle records are tested with the
+    same arithmetic, so t is bit-identical wherever both find the same nearest
+    surface; ids differ only on exact t ties.  The few rays whose t differs
+    are ones where the exact traversal's tagged tmin (the reference's low-bit
+    slot tag) culls a box within ulps of the current hit; there the
+    difference is a few ulps."""
     from conftest import get_scene
     scene, arrays, osc = get_scene(ct, orc, config)
     _upload(gpu_ctx, scene)
     o, d = ray_set(scene, arrays, N_RAYS, seed=200 + config)
     opts, _, _ = scene.frame_opts()
     opts.accel = ct.ACCEL_BVH4
-    opts.traversal = ct.TRAV_FAST
+    opts.traversal = ct.TRAV_FAST if mode == "fast" else ct.TRAV_COMPACT
     t_gpu, p_gpu = gpu_ctx.trace_rays(o, d, opts)
     t_ref, p_ref = osc.trace(o, d, 1, 32, opts.triangle_epsilon, threads=16)
     t_ref32 = t_ref.astype(np.float32)
     same_t = t_gpu.view(np.uint32) == t_ref32.view(np.uint32)
     diff_t = np.flatnonzero(~same_t)
     ties = np.flatnonzero(same_t & (p_gpu != p_ref))
-    print(f"config {config}: rays {N_RAYS}, t differs {diff_t.size}, id ties {ties.size}")
+    print(f"config {config} {mode}: rays {N_RAYS}, t differs {diff_t.size}, id ties {ties.size}")
     assert diff_t.size <= N_RAYS // 100000
     if diff_t.size:
         rel = np.abs(t_gpu[diff_t] - t_ref32[diff_t]) / t_ref32[diff_t]
@@ -85,8 +88,10 @@ def test_fast_traversal_closest_hit(ct, orc, gpu_ctx, config):
 
 
 @pytest.mark.parametrize("config", [1, 2])
-def test_fast_traversal_any_hit(ct, orc, gpu_ctx, config):
-    """Any-hit over the wide tree equals the exact any-hit (order-free)."""
+@pytest.mark.parametrize("mode", ["fast", "compact"])
+def test_fast_traversal_any_hit(ct, orc, gpu_ctx, config, mode):
+    """Any-hit over the wide tree and over the compact quad tree equals the
+    exact any-hit (order-free)."""
     from conftest import get_scene
     scene, arrays, osc = get_scene(ct, orc, config)
     _upload(gpu_ctx, scene)
@@ -96,12 +101,39 @@ def test_fast_traversal_any_hit(ct, orc, gpu_ctx, config):
     tmax = rng.uniform(0.0, 60.0, n).astype(np.float32)
     opts, _, _ = scene.frame_opts()
     opts.accel = ct.ACCEL_BVH4
-    opts.traversal = ct.TRAV_FAST
+    opts.traversal = ct.TRAV_FAST if mode == "fast" else ct.TRAV_COMPACT
     occ_gpu = gpu_ctx.trace_shadow_rays(o, d, tmax, opts)
     occ_ref = osc.trace_shadow(o, d, tmax, 1, 32, opts.triangle_epsilon, threads=16)
     bad = int(np.sum(occ_gpu != occ_ref))
-    print(f"config {config}: occluded {int(occ_ref.sum())} of {n}, differ {bad}")
+    print(f"config {config} {mode}: occluded {int(occ_ref.sum())} of {n}, differ {bad}")
     assert bad <= n // 100000
+
+
+def test_compact_frame_c2(ct, gpu_ctx, c2):
+    """A C2 frame (depth 8, NEE, typed BSDFs) over the compact quad tree
+    equals the exact traversal's frame except at the few pixels whose path
+    met a t-tie or a ulp-level t difference; the compact traversal needs the
+    quad tree (refused for the binary one)."""
+    scene, _, _ = c2
+    _upload(gpu_ctx, scene)
+    opts, w, h = scene.frame_opts()
+    cam = scene.camera(w, h)
+    gpu_ctx.resize(w, h)
+    gpu_ctx.trace_frame(cam, opts)
+    ref = gpu_ctx.read_accum()[0].copy()
+    o = opts.copy()
+    o.traversal = ct.TRAV_COMPACT
+    gpu_ctx.reset()
+    gpu_ctx.trace_frame(cam, o)
+    got = gpu_ctx.read_accum()[0]
+    same = np.all(got.view(np.uint32) == ref.view(np.uint32), axis=-1)
+    print(f"C2 compact frame: {int(same.sum())} of {same.size} pixels bit-identical to exact")
+    assert same.mean() >= 0.999
+    rel = float(np.mean((got[..., :3] - ref[..., :3]) ** 2) / np.mean(ref[..., :3] ** 2))
+    assert rel < 1e-3, rel
+    o.accel = ct.ACCEL_BVH2
+    with pytest.raises(ct.CTError):
+        gpu_ctx.trace_frame(cam, o)
 
 
 @pytest.mark.parametrize("config,accel", [(1, 0), (1, 1), (2, 1)])
