@@ -298,9 +298,9 @@ CT_DEV void node4_fast(const float4* __restrict__ nodes4f, uint32_t e, f3 o, f3 
 //   6, 7    low and high y offsets;  8, 9  low and high z offsets
 //   10, 11  zero
 //   12..15  the four stack entries (kNoEntry for an empty slot)
-// A child's box is [p + lo 2^k, p + hi 2^k] per axis, quantised outwards with
-// one more step of margin (as the wide nodes, host/cnode.h), so it contains
-// the quad node's box.  The ray moves into the node's grid once per node,
+// A child's box is [p + lo 2^k, p + hi 2^k] per axis, quantised outwards, so
+// it contains the quad node's box (why no further margin is needed:
+// compact_axis in device/kernels.hip).  The ray moves into the node's grid once per node,
 // t = q (2^k / d) + (p - o) / d: one conversion and one fma per plane, with
 // the hardware min/max (no slab value is NaN for a compact_ray).  Entry
 // distances carry the slot tag and go through node4_fast's network, so the

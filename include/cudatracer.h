@@ -156,9 +156,11 @@ enum { CT_BSDF_REFERENCE = 0, CT_BSDF_TYPED = 1 };
  * accel setting; hit distances equal, ids equal except exact t-ties.
  * CT_TRAV_COMPACT: the quad tree itself (MBVHNode.cuh:28-63: same nodes,
  * children and leaves) re-laid as 64-byte nodes with 8-bit child boxes on a
- * per-node grid (conservative); CT_ACCEL_BVH4 only.  Children are visited
- * nearest first as in exact mode; hit distances equal, ids equal except
- * exact t-ties. */
+ * per-node grid, rounded outwards; CT_ACCEL_BVH4 only.  Children are visited
+ * nearest first as in exact mode.  With primitive boxes padded as the
+ * builders pad them (1e-3) hit distances equal, ids equal except exact
+ * t-ties; a hand-made tree with unpadded boxes can differ on rays that
+ * graze a box within rounding. */
 enum { CT_TRAV_EXACT = 0, CT_TRAV_FAST = 1, CT_TRAV_COMPACT = 2 };
 /* Light-selection pdf in the MIS weights.  CT_LIGHTPDF_REFERENCE multiplies
  * the per-triangle solid-angle pdf by the light count as the reference does
