@@ -370,11 +370,11 @@ def _host_info() -> dict:
 
 
 def measured_traffic(config: int, traversal: str):
-    """HBM bytes per extend launch of this workload from the committed
+    """HBM bytes per k_trace launch of this workload from the committed
     rocprofv3 summary (FETCH_SIZE and WRITE_SIZE per dispatch, each scaled by
     what the counter reads for a plain copy of known size), or None."""
     import re
-    f = ROOT / "profiles" / f"k_extend_c{config}.md"
+    f = ROOT / "profiles" / f"k_trace_c{config}.md"
     if not f.exists():
         return None
     m = re.search(r"^traffic_bytes_per_launch %s: (\d+)" % traversal, f.read_text(), re.M)
