@@ -56,20 +56,27 @@ struct DevScene {
     const float4* nodes2;   // 2 per binary node: {min, leftFirst}, {max, count}
     const uint4* fnodes;    // 8 per wide compressed node (128 B), layout in host/cnode.h
     const uint4* cnodes;    // 4 per quad node in the compact layout (64 B, cnode4_step in device/traverse.h)
-    const float4* sh_pos;   // 3 per primitive: v0 (w = material index bits), v1, v2
-    const float4* sh_nrm;   // 4 per primitive: n0, n1, n2, centre normal
+    const float4* sh_prim;  // kPrimRows per primitive, one 128-byte line (layout below)
     const float4* mats;     // 2 per material: {albedo, roughness}, {type bits, ior, alpha, 0}
     const float4* lights;   // 5 per light: v0, v1, v2, centre normal (w = area), emission
     const float4* texels;     // float textures' RGBA texels, concatenated
     const uint32_t* texels8;  // 8-bit textures' RGB texels (byte 3 unused), concatenated
     const uint4* tex_info;    // per texture: first texel (in its buffer), width, height, format (0 float, 1 8-bit)
-    const float4* sh_uv;    // 2 per primitive: {uv0, uv1}, {uv2, 0, 0} (only with textures)
     uint32_t n_tris, n_nodes4, n_nodes2, n_fnodes, n_mats, n_lights, n_tex;
     int sky_enabled;
     int sky_tex;            // -1: the constant sky colour
     float sky_r, sky_g, sky_b;
     uint32_t plane_flags;   // kPlaneNegZero | kPlaneNaN | kEmptyEntry | kCompactBad over the quad tree
 };
+
+// A primitive's shading record (sh_prim), indexed by primitive id: what
+// shade reads for a hit, in one 128-byte line (one miss per hit instead of
+// the three or four lines of separate position, normal and texture
+// coordinate arrays):
+//   0 {v0, material index bits}  1 {v1, u0}  2 {v2, v0 tex}
+//   3 {n0, u1}  4 {n1, v1 tex}  5 {n2, u2}  6 {centre normal, v2 tex}  7 zero
+// (texture coordinates are zero without textures).
+constexpr int kPrimRows = 8;
 
 // Properties of the quad tree's box planes, found at upload (k_plane_flags).
 constexpr uint32_t kPlaneNegZero = 1u;  // some plane is -0.0

@@ -216,7 +216,7 @@ CT_DEV uint32_t material_key(const DevScene& sc, const Queues& q, uint32_t i, ui
     if (i >= n) return kSortKeys - 1;
     const float2 h = q.hit[i];
     if (!(h.x < 1e34f)) return 0;
-    const uint32_t mi = __float_as_uint(sc.sh_pos[3 * (size_t)__float_as_int(h.y)].w);
+    const uint32_t mi = __float_as_uint(sc.sh_prim[kPrimRows * (size_t)__float_as_int(h.y)].w);
     const int mtype = __float_as_int(sc.mats[2 * (size_t)mi + 1].x);
     return 1u + (uint32_t)min(max(mtype, 0), (int)kSortKeys - 3);
 }
@@ -336,16 +336,15 @@ k_shade(DevScene sc, ShadeOpts so, const ViewDesc* __restrict__ views, const Seg
                     contrib = true;
                 }
             } else {
-                const float4 P0 = sc.sh_pos[3 * (size_t)prim], P1 = sc.sh_pos[3 * (size_t)prim + 1],
-                             P2 = sc.sh_pos[3 * (size_t)prim + 2];
+                const float4* rec = sc.sh_prim + kPrimRows * (size_t)prim;
+                const float4 P0 = rec[0], P1 = rec[1], P2 = rec[2];
                 const uint32_t mi = __float_as_uint(P0.w);
                 const float4 M0 = sc.mats[2 * (size_t)mi], M1 = sc.mats[2 * (size_t)mi + 1];
                 const int mtype = __float_as_int(M1.x);
                 const f3 p0 = xyz(P0), p1 = xyz(P1), p2 = xyz(P2);
                 const f3 org0 = xyz(o4);
                 const f3 hitp = org0 + t * dir;
-                const float4 N0 = sc.sh_nrm[4 * (size_t)prim], N1 = sc.sh_nrm[4 * (size_t)prim + 1],
-                             N2 = sc.sh_nrm[4 * (size_t)prim + 2], NC = sc.sh_nrm[4 * (size_t)prim + 3];
+                const float4 N0 = rec[3], N1 = rec[4], N2 = rec[5], NC = rec[6];
                 f3 N = xyz(NC);
                 const f3 bary = bary_coords(hitp, N, p0, p1, p2);
                 f3 iN = bary.x * xyz(N0) + bary.y * xyz(N1) + bary.z * xyz(N2);
@@ -354,9 +353,8 @@ k_shade(DevScene sc, ShadeOpts so, const ViewDesc* __restrict__ views, const Seg
                 const int dtex = (int)(tpack & 0xFFFFu) - 1, ntex = (int)(tpack >> 16) - 1;
                 float tu = 0.0f, tv = 0.0f;
                 if (dtex >= 0 || ntex >= 0) {
-                    const float4 U0 = sc.sh_uv[2 * (size_t)prim], U1 = sc.sh_uv[2 * (size_t)prim + 1];
-                    tu = (bary.x * U0.x + bary.y * U0.z) + bary.z * U1.x;
-                    tv = (bary.x * U0.y + bary.y * U0.w) + bary.z * U1.y;
+                    tu = (bary.x * P1.w + bary.y * N0.w) + bary.z * N2.w;
+                    tv = (bary.x * P2.w + bary.y * N1.w) + bary.z * NC.w;
                 }
                 if (ntex >= 0) {
                     const float4 c = tex_fetch(sc.texels, sc.texels8, sc.tex_info, ntex, tu, tv);

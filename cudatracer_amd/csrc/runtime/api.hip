@@ -193,8 +193,8 @@ int check_geometry(ct_ctx* c, const ct_scene_view* v) {
 
 // Host staging of the tree-independent device arrays.
 struct Shading {
-    std::vector<float4> pos, nrm, mats, lights;
-    std::vector<float4> texels, uv;  // uv only when the scene has textures
+    std::vector<float4> prim, mats, lights;  // prim: ctd::kPrimRows rows per primitive
+    std::vector<float4> texels;
     std::vector<uint32_t> texels8;
     std::vector<uint4> tex_info;
 };
@@ -217,15 +217,24 @@ bool is_8bit(const ct_texture& x) {
 
 void stage_shading(const ct_scene_view* v, Shading& sh) {
     const uint32_t N = v->n_triangles;
-    sh.pos.resize((size_t)N * 3);
-    sh.nrm.resize((size_t)N * 4);
+    constexpr size_t R = ctd::kPrimRows;
+    sh.prim.assign((size_t)N * R, make_float4(0.0f, 0.0f, 0.0f, 0.0f));
     for (uint32_t prim = 0; prim < N; ++prim) {
+        float4* row = &sh.prim[R * prim];
+        // texture coordinates u0, v0, u1, v1, u2, v2 (TriangleList.cpp:148-178)
+        float t[6] = {0, 0, 0, 0, 0, 0};
+        if (v->n_textures && v->texcoords)
+            for (int j = 0; j < 3; ++j) {
+                const uint32_t vi = v->indices[3 * (size_t)prim + j];
+                t[2 * j] = v->texcoords[2 * (size_t)vi];
+                t[2 * j + 1] = v->texcoords[2 * (size_t)vi + 1];
+            }
         for (int j = 0; j < 3; ++j) {
             const uint32_t vi = v->indices[3 * (size_t)prim + j];
-            sh.pos[3 * (size_t)prim + j] = f4(v->vertices + 3 * (size_t)vi, j == 0 ? ubits(v->material_idx[prim]) : 0.0f);
-            sh.nrm[4 * (size_t)prim + j] = f4(v->normals + 3 * (size_t)vi, 0.0f);
+            row[j] = f4(v->vertices + 3 * (size_t)vi, j == 0 ? ubits(v->material_idx[prim]) : t[j - 1]);
+            row[3 + j] = f4(v->normals + 3 * (size_t)vi, t[2 + j]);
         }
-        sh.nrm[4 * (size_t)prim + 3] = f4(v->center_normals + 3 * (size_t)prim, 0.0f);
+        row[6] = f4(v->center_normals + 3 * (size_t)prim, t[5]);
     }
     sh.mats.resize((size_t)v->n_materials * 2);
     for (uint32_t m = 0; m < v->n_materials; ++m) mat_rows(v->materials[m], &sh.mats[2 * (size_t)m]);
@@ -248,25 +257,10 @@ void stage_shading(const ct_scene_view* v, Shading& sh) {
                 sh.texels.push_back(make_float4(x.rgba[4 * i], x.rgba[4 * i + 1], x.rgba[4 * i + 2], x.rgba[4 * i + 3]));
         }
     }
-    if (v->n_textures) {
-        sh.uv.resize((size_t)N * 2);
-        for (uint32_t prim = 0; prim < N; ++prim) {
-            float t[6] = {0, 0, 0, 0, 0, 0};
-            if (v->texcoords)
-                for (int j = 0; j < 3; ++j) {
-                    const uint32_t vi = v->indices[3 * (size_t)prim + j];
-                    t[2 * j] = v->texcoords[2 * (size_t)vi];
-                    t[2 * j + 1] = v->texcoords[2 * (size_t)vi + 1];
-                }
-            sh.uv[2 * (size_t)prim] = make_float4(t[0], t[1], t[2], t[3]);
-            sh.uv[2 * (size_t)prim + 1] = make_float4(t[4], t[5], 0.0f, 0.0f);
-        }
-    }
 }
 
 int upload_shading(ct_ctx* c, const Shading& sh, ctd::DevScene& sc) {
-    int r = upload_vec(c, sh.pos, &sc.sh_pos, "shading positions");
-    if (r == CT_OK) r = upload_vec(c, sh.nrm, &sc.sh_nrm, "shading normals");
+    int r = upload_vec(c, sh.prim, &sc.sh_prim, "shading records");
     if (r == CT_OK) r = upload_vec(c, sh.mats, &sc.mats, "materials");
     if (r == CT_OK) r = upload_vec(c, sh.lights, &sc.lights, "lights");
     if (r == CT_OK && !sh.tex_info.empty()) {
@@ -277,7 +271,6 @@ int upload_shading(ct_ctx* c, const Shading& sh, ctd::DevScene& sc) {
         const float4* ti = nullptr;
         if (r == CT_OK) r = upload_vec(c, sh.tex_info, &ti, "texture table");
         sc.tex_info = reinterpret_cast<const uint4*>(ti);
-        if (r == CT_OK) r = upload_vec(c, sh.uv, &sc.sh_uv, "texture coordinates");
     }
     return r;
 }
