@@ -193,8 +193,8 @@ def timed_frames(ctx, cam, opts, steps: int, warmup: int) -> dict:
     for _ in range(warmup):
         ctx.trace_frame(cam, opts)
     ctx.reset()
-    acc = {"primary": 0, "extension": 0, "shadow": 0, "launches": 0, "gpu_ms": 0.0,
-           "stage_ms": {"trace": 0.0, "shade": 0.0, "trace_last_shadow": 0.0, "other": 0.0}}
+    acc = {"primary": 0, "extension": 0, "shadow": 0, "launches": 0, "chunks": 0, "gpu_ms": 0.0,
+           "stage_ms": {"trace": 0.0, "shade": 0.0, "trace_last_shadow": 0.0, "tail": 0.0, "other": 0.0}}
     t0 = time.perf_counter()
     for _ in range(steps):
         st = ctx.trace_frame(cam, opts)
@@ -202,10 +202,12 @@ def timed_frames(ctx, cam, opts, steps: int, warmup: int) -> dict:
         acc["extension"] += st.extension_rays
         acc["shadow"] += st.shadow_rays
         acc["launches"] += st.extend_launches
+        acc["chunks"] += st.chunks
         acc["gpu_ms"] += st.gpu_ms
         acc["stage_ms"]["trace"] += st.extend_ms
         acc["stage_ms"]["shade"] += st.shade_ms
         acc["stage_ms"]["trace_last_shadow"] += st.connect_ms
+        acc["stage_ms"]["tail"] += st.tail_ms
         acc["stage_ms"]["other"] += st.other_ms
     acc["wall"] = time.perf_counter() - t0
     acc["steps"] = steps
@@ -215,10 +217,13 @@ def timed_frames(ctx, cam, opts, steps: int, warmup: int) -> dict:
 
 
 def trace_roofline(ctx, cam, opts, tm: dict, config: int, traversal: str) -> dict:
-    """The traversal kernel's algorithmic bytes (from an instrumented frame's
-    node and triangle test counts) against three times: the mean launch of
-    the timed frames, the launches of a serial frame (each alone on the
-    device) and the frame wall time."""
+    """The traversal's algorithmic bytes (from an instrumented frame's node
+    and triangle test counts) against three times: the summed durations of
+    the launches that trace the timed frames' rays, the same launches in a
+    serial frame (each alone on the device), and the frame wall time.  With
+    roulette, the bounces from rr_start on are traced inside k_tail, which
+    also shades them: its whole duration counts as traversal time, so the
+    first two figures are lower bounds there."""
     cst, work = ctx.trace_frame_counted(cam, opts)  # the same frame: identical counts (deterministic)
     closest = cst.primary_rays + cst.extension_rays
     bytes_frame = (36 * closest + 64 * work[0] + 36 * work[1]) + (48 * cst.shadow_rays + 64 * work[2] + 36 * work[3])
@@ -227,37 +232,46 @@ def trace_roofline(ctx, cam, opts, tm: dict, config: int, traversal: str) -> dic
     steps = tm["steps"]
     launches = tm["launches"] / steps
     trace_ms = (tm["stage_ms"]["trace"] + tm["stage_ms"]["trace_last_shadow"]) / steps
-    bytes_per_launch = bytes_frame / max(launches, 1)
-    mean_launch_ms = trace_ms / max(launches, 1)
+    tail_ms = tm["stage_ms"]["tail"] / steps
+    tail_launches = tm["chunks"] / steps if tail_ms > 0 else 0  # one k_tail launch per chunk
+    bytes_per_launch = bytes_frame / max(launches + tail_launches, 1)
+    mean_launch_ms = (trace_ms + tail_ms) / max(launches + tail_launches, 1)
     achieved = bytes_per_launch / (mean_launch_ms * 1e-3) / 1e9
-    iso_ms = ser.extend_ms + ser.connect_ms
+    iso_ms = ser.extend_ms + ser.connect_ms + ser.tail_ms
     iso = bytes_frame / (iso_ms * 1e-3) / 1e9
     dev = bytes_frame / (tm["ms_per_step"] * 1e-3) / 1e9
-    return {
-        "kernel": "k_trace (closest hit of bounce b + any hit of bounce b-1, quad BVH)",
+    out = {
+        "kernel": "k_trace (closest hit of bounce b + any hit of bounce b-1, quad BVH)" +
+                  ("; k_tail from bounce rr_start" if tail_ms > 0 else ""),
         "bound": "hbm",
         "achieved": round(achieved, 1),
         "peak": HBM_PEAK_GBS,
         "unit": "GB/s",
         "frac": round(achieved / HBM_PEAK_GBS, 4),
         "traffic": measured_traffic(config, traversal),
-        "label": "per launch: algorithmic bytes per k_trace launch / mean launch duration over the timed frames "
-                 "(two chunks in flight, so launches share the device)",
+        "label": "per launch: algorithmic traversal bytes per traversal launch / the launches' mean duration "
+                 "over the timed frames (two chunks in flight, so launches share the device)" +
+                 ("; the launches are k_trace's and one k_tail per chunk, whose whole duration counts (it also "
+                  "shades): a lower bound" if tail_ms > 0 else ""),
         "bytes_per_launch": int(bytes_per_launch),
         "mean_launch_ms": round(mean_launch_ms, 4),
-        "launches_per_frame": round(launches, 2),
+        "launches_per_frame": round(launches + tail_launches, 2),
         "node_tests_per_ray": round(work[0] / max(closest, 1), 2),
         "tri_tests_per_ray": round(work[1] / max(closest, 1), 2),
         "shadow_node_tests_per_ray": round(work[2] / max(cst.shadow_rays, 1), 2),
         "shadow_tri_tests_per_ray": round(work[3] / max(cst.shadow_rays, 1), 2),
-        "isolated": {"label": "the same bytes over the k_trace launches of a frame whose chunks run one at a time "
-                              "(each launch alone on the device)",
+        "isolated": {"label": "the same bytes over the traversal launches of a frame whose chunks run one at a "
+                              "time (each launch alone on the device)",
                      "achieved": round(iso, 1), "frac": round(iso / HBM_PEAK_GBS, 4),
                      "trace_ms_per_frame": round(iso_ms, 4), "launches": int(ser.extend_launches)},
         "device_level": {"label": "the frame's traversal bytes over the frame wall time (the frame also shades)",
                          "achieved": round(dev, 1), "frac": round(dev / HBM_PEAK_GBS, 4),
                          "bytes_per_frame": int(bytes_frame)},
     }
+    if tail_ms > 0:
+        out["k_trace_mean_launch_ms"] = round(trace_ms / max(launches, 1), 4)
+        out["k_tail_ms_per_frame"] = round(tail_ms, 4)
+    return out
 
 
 def c3_leg(ct, ctx, args, steps: int = 10, warmup: int = 3) -> dict:

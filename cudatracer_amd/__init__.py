@@ -99,7 +99,7 @@ class FrameStats(C.Structure):
     _fields_ = [("primary_rays", C.c_uint64), ("extension_rays", C.c_uint64), ("shadow_rays", C.c_uint64),
                 ("samples", C.c_uint64), ("gpu_ms", C.c_double), ("extend_ms", C.c_double),
                 ("shade_ms", C.c_double), ("connect_ms", C.c_double), ("other_ms", C.c_double),
-                ("chunks", C.c_uint32), ("extend_launches", C.c_uint32)]
+                ("chunks", C.c_uint32), ("extend_launches", C.c_uint32), ("tail_ms", C.c_double)]
 
     def as_dict(self) -> dict:
         return {k: getattr(self, k) for k, _ in self._fields_}

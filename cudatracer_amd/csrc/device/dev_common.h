@@ -197,6 +197,10 @@ CT_DEV void add_term(float4& L, float4 t) {
 }
 CT_DEV uint32_t* cnt_active(uint32_t* c, int b) { return c + b; }
 CT_DEV uint32_t* cnt_shadow(uint32_t* c, int b) { return c + (kMaxBounce + 1) + b; }
+// Extension and shadow rays traced by k_tail (a chunk's roulette tail), over
+// all the bounces it runs.
+CT_DEV uint32_t* cnt_tail_ext(uint32_t* c) { return c + 2 * (kMaxBounce + 1); }
+CT_DEV uint32_t* cnt_tail_shadow(uint32_t* c) { return c + 2 * (kMaxBounce + 1) + 1; }
 // Work-fetch counters, [queue][range][bounce]: each queue range owns a
 // 384-byte slot so ranges never share a cache line (same-line atomics from
 // different XCDs serialise).  Four ranges per XCD.
@@ -204,6 +208,7 @@ constexpr int kXcds = 8;
 constexpr int kFetchRanges = 4 * kXcds;
 constexpr int kFetchSlot = 96;
 constexpr int kFetchBase = 192;
+static_assert(2 * (kMaxBounce + 1) + 2 <= kFetchBase, "counters overlap");
 CT_DEV uint32_t* cnt_fetch_ext(uint32_t* c, int b) { return c + kFetchBase + b; }
 CT_DEV uint32_t* cnt_fetch_sh(uint32_t* c, int b) { return c + kFetchBase + kFetchRanges * kFetchSlot + b; }
 constexpr int kCounterWords = kFetchBase + 2 * kFetchRanges * kFetchSlot + 64;

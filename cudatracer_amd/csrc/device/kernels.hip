@@ -283,6 +283,230 @@ CT_DEV void block_append2(bool pe, bool ps, uint32_t* ce, uint32_t* cs, uint32_t
     js = lds[9] + lds[4 + w] + (uint32_t)__popcll(ms & lt);
 }
 
+// One queue entry of shade (Kernel.cu:88-206; the oracle's path_sample
+// restates the same steps): the hit (t, prim) of the ray (org0, dir) of path
+// slot `slot` at `bounce`, with throughput thr, the last bounce's pdf and
+// type.  Out: the hit's radiance term (add, when contrib), the light
+// sample's shadow ray (push_sh) and the continuation ray (push_ext).  k_shade
+// and k_tail both shade through it.
+struct ShadeOut {
+    f3 add, s_o, s_d, s_c, e_o, e_d, e_thr;
+    float s_tmax, e_pdf;
+    uint32_t e_type;
+    bool contrib, push_sh, push_ext;
+};
+
+CT_DEV void shade_one(const DevScene& sc, const ShadeOpts& so, const ViewDesc* __restrict__ views,
+                      const Segment* __restrict__ segs, int nseg, int bounce, uint32_t slot, uint32_t lastType,
+                      const f3 org0, f3 dir, f3 thr, const float lastPdf, const float t, const int prim,
+                      ShadeOut& out) {
+    out.add = out.s_o = out.s_d = out.s_c = out.e_o = out.e_d = out.e_thr = mk3(0, 0, 0);
+    out.s_tmax = out.e_pdf = 0.0f;
+    out.e_type = 0;
+    out.contrib = out.push_sh = out.push_ext = false;
+    f3 &add = out.add, &s_o = out.s_o, &s_d = out.s_d, &s_c = out.s_c, &e_o = out.e_o, &e_d = out.e_d,
+       &e_thr = out.e_thr;
+    float &s_tmax = out.s_tmax, &e_pdf = out.e_pdf;
+    uint32_t& e_type = out.e_type;
+    bool &contrib = out.contrib, &push_sh = out.push_sh, &push_ext = out.push_ext;
+    if (!(t < 1e34f)) {
+        if (sc.sky_enabled) {
+            f3 sky = mk3(sc.sky_r, sc.sky_g, sc.sky_b);
+            if (sc.sky_tex >= 0) {  // Kernel.cu:96-100
+                float u, v;
+                ct_sky_uv(dir.x, dir.y, dir.z, &u, &v);
+                const float4 c = tex_fetch(sc.texels, sc.texels8, sc.tex_info, sc.sky_tex, u, v);
+                sky = mk3(c.x, c.y, c.z);
+            }
+            add = (thr / lastPdf) * sky;
+            contrib = true;
+        }
+    } else {
+        const float4* rec = sc.sh_prim + kPrimRows * (size_t)prim;
+        const float4 P0 = rec[0], P1 = rec[1], P2 = rec[2];
+        const uint32_t mi = __float_as_uint(P0.w);
+        const float4 M0 = sc.mats[2 * (size_t)mi], M1 = sc.mats[2 * (size_t)mi + 1];
+        const int mtype = __float_as_int(M1.x);
+        const f3 p0 = xyz(P0), p1 = xyz(P1), p2 = xyz(P2);
+        const f3 hitp = org0 + t * dir;
+        const float4 N0 = rec[3], N1 = rec[4], N2 = rec[5], NC = rec[6];
+        f3 N = xyz(NC);
+        const f3 bary = bary_coords(hitp, N, p0, p1, p2);
+        f3 iN = bary.x * xyz(N0) + bary.y * xyz(N1) + bary.z * xyz(N2);
+        // textures: diffuse colour and normal map (Kernel.cu:106-112,146)
+        const uint32_t tpack = __float_as_uint(M1.w);
+        const int dtex = (int)(tpack & 0xFFFFu) - 1, ntex = (int)(tpack >> 16) - 1;
+        float tu = 0.0f, tv = 0.0f;
+        if (dtex >= 0 || ntex >= 0) {
+            tu = (bary.x * P1.w + bary.y * N0.w) + bary.z * N2.w;
+            tv = (bary.x * P2.w + bary.y * N1.w) + bary.z * NC.w;
+        }
+        if (ntex >= 0) {
+            const float4 c = tex_fetch(sc.texels, sc.texels8, sc.tex_info, ntex, tu, tv);
+            iN = sample_to_world(normalize(mk3(c.x, c.y, c.z)), iN);
+        }
+        if (mtype == MAT_LIGHT) {
+            if (so.indirect) {
+                const float DdotNL = -dot(dir, iN);
+                const f3 Le = xyz(M0);
+                f3 c;
+                if (lastType == LT_SPECULAR) {
+                    c = thr * Le;
+                } else {
+                    const float lightPdf = light_pdf_scale(t * t / (DdotNL * tri_area(p0, p1, p2)), sc.n_lights, so.light_pdf);
+                    c = thr * Le * (1.0f / (lastPdf + lightPdf));
+                }
+                add = vmax0(c);
+                contrib = true;
+            }
+        } else {
+            const SlotInfo si = decode_slot(views, segs, nseg, slot);
+            f3 org = hitp;
+            thr = vmax0(thr / lastPdf);
+            const bool flipped = dot(dir, N) > 0.0f;
+            if (flipped) { N = -N; iN = -iN; }
+            uint32_t seed = ct_shade_seed(si.pixel, si.frame, ct_path_length_key(bounce));
+            const float r3 = ct_rand_float(&seed);
+            const float r4 = ct_rand_float(&seed);
+            f3 color = xyz(M0);
+            if (dtex >= 0) {
+                const float4 c = tex_fetch(sc.texels, sc.texels8, sc.tex_info, dtex, tu, tv);
+                color = mk3(c.x, c.y, c.z);
+            }
+            const f3 wiw = -dir;
+            f3 wo = mk3(0, 0, 0), f = mk3(0, 0, 0);
+            float pdf = 0.0f;
+            uint32_t type = LT_DIFFUSE;
+            bool valid = false, transmit = false;
+            const bool typed = so.bsdf_mode == CT_BSDF_TYPED;
+            const float roughness = M0.w;
+            const float alpha = gmax(roughness, 1e-3f);
+            // `type` is the material's lobe, set before sampling:
+            // next-event estimation runs for a diffuse or glossy
+            // lobe also when the sample is rejected (the path then
+            // ends after it), which keeps the MIS sum unbiased
+            if ((!typed && roughness < 0.1f) || (typed && mtype == MAT_SPECULAR)) {
+                type = LT_SPECULAR;
+                wo = reflect(dir, iN);
+                if (dot(N, wo) > 0.0f) {
+                    pdf = 1.0f;
+                    f = color * (1.0f / ct_fabsf(dot(iN, wo)));
+                    valid = true;
+                }
+            } else if (typed && is_dielectric(mtype)) {
+                const float ior = M1.y;
+                const float etaI = flipped ? ior : 1.0f;
+                const float etaT = flipped ? 1.0f : ior;
+                const float cosI = dot(wiw, iN);
+                const float F = fresnel_dielectric(cosI, etaT / etaI);
+                type = LT_SPECULAR;
+                pdf = 1.0f;
+                if (r3 < F || F >= 1.0f) {
+                    wo = reflect(dir, iN);
+                    valid = dot(N, wo) > 0.0f;
+                } else {
+                    const float eta = etaI / etaT;
+                    const float dv = dot(iN, dir);
+                    const float k = 1.0f - eta * eta * (1.0f - dv * dv);
+                    wo = eta * dir - (eta * dv + ct_sqrtf(k)) * iN;
+                    valid = k >= 0.0f && dot(N, wo) < 0.0f;
+                    transmit = true;
+                }
+                if (valid) f = color * (1.0f / ct_fabsf(dot(iN, wo)));
+            } else if (typed && is_microfacet(mtype)) {
+                type = LT_GLOSSY;
+                f3 Tv, Bv;
+                tangent_frame(iN, Tv, Bv);
+                const f3 wol = mk3(dot(wiw, Tv), dot(wiw, Bv), dot(wiw, iN));
+                if (wol.z > 0.0f) {
+                    const f3 wm = sample_vndf(mtype, wol, alpha, r3, r4);
+                    const f3 wil = reflect(-wol, wm);
+                    if (wil.z > 0.0f) {
+                        wo = normalize(wil.x * Tv + wil.y * Bv + wil.z * iN);
+                        if (dot(N, wo) > 0.0f) {
+                            mf_eval(mtype, alpha, color, wol, wil, &f, &pdf);
+                            valid = pdf > 0.0f;
+                        }
+                    }
+                }
+            } else {
+                type = LT_DIFFUSE;
+                wo = normalize(tangent_to_world(cos_weighted(r3, r4), iN));
+                if (dot(N, wo) > 0.0f) {
+                    pdf = gmax(0.0f, dot(wo, iN)) * CT_INV_PI_F;
+                    f = color * CT_INV_PI_F;
+                    valid = true;
+                }
+            }
+            {
+                // next-event estimation (Kernel.cu:161-195)
+                if (type != LT_SPECULAR && so.shadow && sc.n_lights > 0) {
+                    int light = ct_rand_int_max(&seed, (int)sc.n_lights - 1);
+                    if (light > (int)sc.n_lights - 1) light = (int)sc.n_lights - 1;
+                    const float r1 = ct_rand_float(&seed);
+                    const float r2 = ct_rand_float(&seed);
+                    const float4* lr = sc.lights + 5 * (size_t)light;
+                    const float4 L0 = lr[0], L1 = lr[1], L2 = lr[2], LN = lr[3], LE = lr[4];
+                    const f3 lightPos = random_point(xyz(L0), xyz(L1), xyz(L2), r1, r2);
+                    f3 Lv = lightPos - org;
+                    const float sqDist = dot(Lv, Lv);
+                    const float distance = ct_sqrtf(sqDist);
+                    Lv = Lv / distance;
+                    const f3 lN = xyz(LN);
+                    const float NdotL = dot(iN, Lv);
+                    const float LNdotL = -dot(lN, Lv);
+                    if (NdotL > 0.0f && LNdotL > 0.0f) {
+                        const float ar = LN.w;
+                        const f3 emission = xyz(LE);
+                        f3 fe;
+                        float shadowPdf;
+                        if (type == LT_DIFFUSE) {
+                            shadowPdf = ct_fabsf(dot(Lv, iN)) * CT_INV_PI_F;
+                            fe = color * CT_INV_PI_F;
+                        } else {
+                            f3 Tv, Bv;
+                            tangent_frame(iN, Tv, Bv);
+                            const f3 wol = mk3(dot(wiw, Tv), dot(wiw, Bv), dot(wiw, iN));
+                            const f3 wil = mk3(dot(Lv, Tv), dot(Lv, Bv), dot(Lv, iN));
+                            mf_eval(mtype, alpha, color, wol, wil, &fe, &shadowPdf);
+                        }
+                        if (shadowPdf > 0.0f) {
+                            const float lightPdf = light_pdf_scale(sqDist / (LNdotL * ar), sc.n_lights, so.light_pdf);
+                            s_c = thr * emission * fe * (NdotL / (shadowPdf + lightPdf));
+                            s_o = org + so.neps * Lv;
+                            s_d = Lv;
+                            s_tmax = distance - 2.0f * so.deps;
+                            push_sh = true;
+                        }
+                    }
+                }
+                // continuation (Kernel.cu:197-205); a rejected sample ends the path
+                if (valid && !(pdf < 1e-6f || isnanf_(pdf) || isnanf_(thr.x) || isnanf_(thr.y) || isnanf_(thr.z) ||
+                      (thr.x <= 0.0f && thr.y <= 0.0f && thr.z <= 0.0f))) {
+                    thr = vmax0(thr * f * ct_fabsf(dot(iN, wo)));
+                    org = transmit ? org - N * so.neps : org + N * so.neps;
+                    bool alive = bounce + 1 < so.max_depth;
+                    if (alive && so.rr_start >= 0 && bounce >= so.rr_start) {
+                        const f3 tt = thr / pdf;
+                        const float qs = gmin(1.0f, gmax(tt.x, gmax(tt.y, tt.z)));
+                        const float rr = ct_rand_float(&seed);
+                        if (!(rr < qs)) alive = false;
+                        else thr = thr / qs;
+                    }
+                    if (alive) {
+                        push_ext = true;
+                        e_o = org;
+                        e_d = wo;
+                        e_thr = thr;
+                        e_pdf = pdf;
+                        e_type = type;
+                    }
+                }
+            }
+        }
+    }
+}
+
 // shade (Kernel.cu:88-206) for queue entries [0, active[bounce]); see the
 // oracle's path_sample for the same steps on the CPU.  SORT: shade in
 // block-local material order (ct_frame_opts.sort_by_material).  Five waves
@@ -310,216 +534,25 @@ k_shade(DevScene sc, ShadeOpts so, const ViewDesc* __restrict__ views, const Seg
         uint32_t e_type = 0, slot = 0;
         f3 s_o = mk3(0, 0, 0), s_d = mk3(0, 0, 0), s_c = mk3(0, 0, 0);
         float s_tmax = 0.0f;
+        ShadeOut sh;
         if (active) {
             const float4 o4 = q.ray_o[sel][i], d4 = q.ray_d[sel][i], th4 = q.thr[sel][i];
             const uint32_t pw = q.path[sel][i];
             const float2 h = q.hit[i];
             slot = pw & kSlotMask;
-            const uint32_t lastType = pw >> 30;
-            const float t = h.x;
-            const int prim = __float_as_int(h.y);
-            f3 dir = xyz(d4);
-            f3 thr = xyz(th4);
-            const float lastPdf = th4.w;
-            f3 add = mk3(0, 0, 0);
-            bool contrib = false;
-            if (!(t < 1e34f)) {
-                if (sc.sky_enabled) {
-                    f3 sky = mk3(sc.sky_r, sc.sky_g, sc.sky_b);
-                    if (sc.sky_tex >= 0) {  // Kernel.cu:96-100
-                        float u, v;
-                        ct_sky_uv(dir.x, dir.y, dir.z, &u, &v);
-                        const float4 c = tex_fetch(sc.texels, sc.texels8, sc.tex_info, sc.sky_tex, u, v);
-                        sky = mk3(c.x, c.y, c.z);
-                    }
-                    add = (thr / lastPdf) * sky;
-                    contrib = true;
-                }
-            } else {
-                const float4* rec = sc.sh_prim + kPrimRows * (size_t)prim;
-                const float4 P0 = rec[0], P1 = rec[1], P2 = rec[2];
-                const uint32_t mi = __float_as_uint(P0.w);
-                const float4 M0 = sc.mats[2 * (size_t)mi], M1 = sc.mats[2 * (size_t)mi + 1];
-                const int mtype = __float_as_int(M1.x);
-                const f3 p0 = xyz(P0), p1 = xyz(P1), p2 = xyz(P2);
-                const f3 org0 = xyz(o4);
-                const f3 hitp = org0 + t * dir;
-                const float4 N0 = rec[3], N1 = rec[4], N2 = rec[5], NC = rec[6];
-                f3 N = xyz(NC);
-                const f3 bary = bary_coords(hitp, N, p0, p1, p2);
-                f3 iN = bary.x * xyz(N0) + bary.y * xyz(N1) + bary.z * xyz(N2);
-                // textures: diffuse colour and normal map (Kernel.cu:106-112,146)
-                const uint32_t tpack = __float_as_uint(M1.w);
-                const int dtex = (int)(tpack & 0xFFFFu) - 1, ntex = (int)(tpack >> 16) - 1;
-                float tu = 0.0f, tv = 0.0f;
-                if (dtex >= 0 || ntex >= 0) {
-                    tu = (bary.x * P1.w + bary.y * N0.w) + bary.z * N2.w;
-                    tv = (bary.x * P2.w + bary.y * N1.w) + bary.z * NC.w;
-                }
-                if (ntex >= 0) {
-                    const float4 c = tex_fetch(sc.texels, sc.texels8, sc.tex_info, ntex, tu, tv);
-                    iN = sample_to_world(normalize(mk3(c.x, c.y, c.z)), iN);
-                }
-                if (mtype == MAT_LIGHT) {
-                    if (so.indirect) {
-                        const float DdotNL = -dot(dir, iN);
-                        const f3 Le = xyz(M0);
-                        f3 c;
-                        if (lastType == LT_SPECULAR) {
-                            c = thr * Le;
-                        } else {
-                            const float lightPdf = light_pdf_scale(t * t / (DdotNL * tri_area(p0, p1, p2)), sc.n_lights, so.light_pdf);
-                            c = thr * Le * (1.0f / (lastPdf + lightPdf));
-                        }
-                        add = vmax0(c);
-                        contrib = true;
-                    }
-                } else {
-                    const SlotInfo si = decode_slot(views, segs, nseg, slot);
-                    f3 org = hitp;
-                    thr = vmax0(thr / lastPdf);
-                    const bool flipped = dot(dir, N) > 0.0f;
-                    if (flipped) { N = -N; iN = -iN; }
-                    uint32_t seed = ct_shade_seed(si.pixel, si.frame, ct_path_length_key(bounce));
-                    const float r3 = ct_rand_float(&seed);
-                    const float r4 = ct_rand_float(&seed);
-                    f3 color = xyz(M0);
-                    if (dtex >= 0) {
-                        const float4 c = tex_fetch(sc.texels, sc.texels8, sc.tex_info, dtex, tu, tv);
-                        color = mk3(c.x, c.y, c.z);
-                    }
-                    const f3 wiw = -dir;
-                    f3 wo = mk3(0, 0, 0), f = mk3(0, 0, 0);
-                    float pdf = 0.0f;
-                    uint32_t type = LT_DIFFUSE;
-                    bool valid = false, transmit = false;
-                    const bool typed = so.bsdf_mode == CT_BSDF_TYPED;
-                    const float roughness = M0.w;
-                    const float alpha = gmax(roughness, 1e-3f);
-                    // `type` is the material's lobe, set before sampling:
-                    // next-event estimation runs for a diffuse or glossy
-                    // lobe also when the sample is rejected (the path then
-                    // ends after it), which keeps the MIS sum unbiased
-                    if ((!typed && roughness < 0.1f) || (typed && mtype == MAT_SPECULAR)) {
-                        type = LT_SPECULAR;
-                        wo = reflect(dir, iN);
-                        if (dot(N, wo) > 0.0f) {
-                            pdf = 1.0f;
-                            f = color * (1.0f / ct_fabsf(dot(iN, wo)));
-                            valid = true;
-                        }
-                    } else if (typed && is_dielectric(mtype)) {
-                        const float ior = M1.y;
-                        const float etaI = flipped ? ior : 1.0f;
-                        const float etaT = flipped ? 1.0f : ior;
-                        const float cosI = dot(wiw, iN);
-                        const float F = fresnel_dielectric(cosI, etaT / etaI);
-                        type = LT_SPECULAR;
-                        pdf = 1.0f;
-                        if (r3 < F || F >= 1.0f) {
-                            wo = reflect(dir, iN);
-                            valid = dot(N, wo) > 0.0f;
-                        } else {
-                            const float eta = etaI / etaT;
-                            const float dv = dot(iN, dir);
-                            const float k = 1.0f - eta * eta * (1.0f - dv * dv);
-                            wo = eta * dir - (eta * dv + ct_sqrtf(k)) * iN;
-                            valid = k >= 0.0f && dot(N, wo) < 0.0f;
-                            transmit = true;
-                        }
-                        if (valid) f = color * (1.0f / ct_fabsf(dot(iN, wo)));
-                    } else if (typed && is_microfacet(mtype)) {
-                        type = LT_GLOSSY;
-                        f3 Tv, Bv;
-                        tangent_frame(iN, Tv, Bv);
-                        const f3 wol = mk3(dot(wiw, Tv), dot(wiw, Bv), dot(wiw, iN));
-                        if (wol.z > 0.0f) {
-                            const f3 wm = sample_vndf(mtype, wol, alpha, r3, r4);
-                            const f3 wil = reflect(-wol, wm);
-                            if (wil.z > 0.0f) {
-                                wo = normalize(wil.x * Tv + wil.y * Bv + wil.z * iN);
-                                if (dot(N, wo) > 0.0f) {
-                                    mf_eval(mtype, alpha, color, wol, wil, &f, &pdf);
-                                    valid = pdf > 0.0f;
-                                }
-                            }
-                        }
-                    } else {
-                        type = LT_DIFFUSE;
-                        wo = normalize(tangent_to_world(cos_weighted(r3, r4), iN));
-                        if (dot(N, wo) > 0.0f) {
-                            pdf = gmax(0.0f, dot(wo, iN)) * CT_INV_PI_F;
-                            f = color * CT_INV_PI_F;
-                            valid = true;
-                        }
-                    }
-                    {
-                        // next-event estimation (Kernel.cu:161-195)
-                        if (type != LT_SPECULAR && so.shadow && sc.n_lights > 0) {
-                            int light = ct_rand_int_max(&seed, (int)sc.n_lights - 1);
-                            if (light > (int)sc.n_lights - 1) light = (int)sc.n_lights - 1;
-                            const float r1 = ct_rand_float(&seed);
-                            const float r2 = ct_rand_float(&seed);
-                            const float4* lr = sc.lights + 5 * (size_t)light;
-                            const float4 L0 = lr[0], L1 = lr[1], L2 = lr[2], LN = lr[3], LE = lr[4];
-                            const f3 lightPos = random_point(xyz(L0), xyz(L1), xyz(L2), r1, r2);
-                            f3 Lv = lightPos - org;
-                            const float sqDist = dot(Lv, Lv);
-                            const float distance = ct_sqrtf(sqDist);
-                            Lv = Lv / distance;
-                            const f3 lN = xyz(LN);
-                            const float NdotL = dot(iN, Lv);
-                            const float LNdotL = -dot(lN, Lv);
-                            if (NdotL > 0.0f && LNdotL > 0.0f) {
-                                const float ar = LN.w;
-                                const f3 emission = xyz(LE);
-                                f3 fe;
-                                float shadowPdf;
-                                if (type == LT_DIFFUSE) {
-                                    shadowPdf = ct_fabsf(dot(Lv, iN)) * CT_INV_PI_F;
-                                    fe = color * CT_INV_PI_F;
-                                } else {
-                                    f3 Tv, Bv;
-                                    tangent_frame(iN, Tv, Bv);
-                                    const f3 wol = mk3(dot(wiw, Tv), dot(wiw, Bv), dot(wiw, iN));
-                                    const f3 wil = mk3(dot(Lv, Tv), dot(Lv, Bv), dot(Lv, iN));
-                                    mf_eval(mtype, alpha, color, wol, wil, &fe, &shadowPdf);
-                                }
-                                if (shadowPdf > 0.0f) {
-                                    const float lightPdf = light_pdf_scale(sqDist / (LNdotL * ar), sc.n_lights, so.light_pdf);
-                                    s_c = thr * emission * fe * (NdotL / (shadowPdf + lightPdf));
-                                    s_o = org + so.neps * Lv;
-                                    s_d = Lv;
-                                    s_tmax = distance - 2.0f * so.deps;
-                                    push_sh = true;
-                                }
-                            }
-                        }
-                        // continuation (Kernel.cu:197-205); a rejected sample ends the path
-                        if (valid && !(pdf < 1e-6f || isnanf_(pdf) || isnanf_(thr.x) || isnanf_(thr.y) || isnanf_(thr.z) ||
-                              (thr.x <= 0.0f && thr.y <= 0.0f && thr.z <= 0.0f))) {
-                            thr = vmax0(thr * f * ct_fabsf(dot(iN, wo)));
-                            org = transmit ? org - N * so.neps : org + N * so.neps;
-                            bool alive = bounce + 1 < so.max_depth;
-                            if (alive && so.rr_start >= 0 && bounce >= so.rr_start) {
-                                const f3 tt = thr / pdf;
-                                const float qs = gmin(1.0f, gmax(tt.x, gmax(tt.y, tt.z)));
-                                const float rr = ct_rand_float(&seed);
-                                if (!(rr < qs)) alive = false;
-                                else thr = thr / qs;
-                            }
-                            if (alive) {
-                                push_ext = true;
-                                e_o = org;
-                                e_d = wo;
-                                e_thr = thr;
-                                e_pdf = pdf;
-                                e_type = type;
-                            }
-                        }
-                    }
-                }
-            }
+            shade_one(sc, so, views, segs, nseg, bounce, slot, pw >> 30, xyz(o4), xyz(d4), xyz(th4), th4.w, h.x,
+                      __float_as_int(h.y), sh);
+            push_ext = sh.push_ext;
+            e_o = sh.e_o;
+            e_d = sh.e_d;
+            e_thr = sh.e_thr;
+            e_pdf = sh.e_pdf;
+            e_type = sh.e_type;
+            push_sh = sh.push_sh;
+            s_o = sh.s_o;
+            s_d = sh.s_d;
+            s_c = sh.s_c;
+            s_tmax = sh.s_tmax;
             // bounce - 2's pair of terms into the radiance (shade's, then
             // connect's), then this bounce's term for shade(bounce + 2)
             if (bounce >= 2) {
@@ -534,7 +567,7 @@ k_shade(DevScene sc, ShadeOpts so, const ViewDesc* __restrict__ views, const Seg
                     q.lpath[slot] = L;
                 }
             }
-            if (contrib) q.ebuf[bounce & 1][slot] = term(add, term_stamp(gen, bounce));
+            if (sh.contrib) q.ebuf[bounce & 1][slot] = term(sh.add, term_stamp(gen, bounce));
         }
         // block-aggregated queue appends (every thread of the block reaches here)
         uint32_t je = 0, js = 0;
@@ -551,6 +584,134 @@ k_shade(DevScene sc, ShadeOpts so, const ViewDesc* __restrict__ views, const Seg
             q.sh_d[bounce & 1][js] = make_float4(s_d.x, s_d.y, s_d.z, __uint_as_float(slot));
             q.sh_c[bounce & 1][js] = make_float4(s_c.x, s_c.y, s_c.z, 0.0f);
         }
+    }
+}
+
+// ----------------------------------------------------------------- tail --
+// A chunk's roulette tail in one launch.  Once roulette has thinned the
+// paths, a bounce's trace and shade launches hold few rays each and last as
+// long as their slowest ray, plus a dependent-launch gap each.  From bounce
+// b0 on, k_tail instead gives each pair of lanes one path of bounce b0's
+// extension queue (its hit already traced by trace(b0)) and runs the path's
+// remaining bounces itself: the even lane shades (shade_one), then traces
+// the continuation ray while the odd lane traces the light sample's shadow
+// ray, both in one trace_single loop, so a bounce costs the longer of the
+// two traversals rather than their sum.  Per path the steps and their
+// arithmetic are those of the queue pipeline, so the results are the same;
+// the radiance terms are added in the same order (bounce b0 - 2's and
+// b0 - 1's pairs first, which shade(b0) and shade(b0 + 1) would fold, then
+// each bounce's shade term and light sample in turn), and the path's sum is
+// marked fully folded for k_resolve.  The rays it traces are counted in
+// cnt_tail_ext / cnt_tail_shadow.  Both lanes of a pair make the same loop
+// decisions (exchanged with lane shuffles), so each shuffle reads an active
+// partner.
+template <int ACCEL>
+__global__ void __launch_bounds__(1024)
+k_tail(DevScene sc, TraceOpts to, ShadeOpts so, const ViewDesc* __restrict__ views, const Segment* __restrict__ segs,
+       const ChunkHdr* hdr, Queues q, int sel, int bounce0, int lds_depth) {
+    LaneStack st = make_stack(q, lds_depth);
+    const uint32_t n = *cnt_active(q.counters, bounce0);
+    const int nseg = (int)hdr->nseg;
+    const uint32_t gen = hdr->gen;
+    const bool shader = (__lane_id() & 1u) == 0u;  // the even lane of a pair
+    const int partner = (int)(__lane_id() ^ 1u);
+    uint32_t n_ext = 0, n_sh = 0;
+    const uint32_t pairs = (gridDim.x * blockDim.x) / 2u;
+    for (uint32_t i = (blockIdx.x * blockDim.x + threadIdx.x) / 2u; i < n; i += pairs) {
+        uint32_t slot = 0, type = 0;
+        float4 L = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        f3 org = mk3(0, 0, 0), dir = mk3(0, 0, 0), thr = mk3(0, 0, 0);
+        float pdf = 0.0f, t = 0.0f;
+        int prim = -1;
+        if (shader) {
+            const float4 o4 = q.ray_o[sel][i], d4 = q.ray_d[sel][i], th4 = q.thr[sel][i];
+            const uint32_t pw = q.path[sel][i];
+            const float2 h = q.hit[i];
+            slot = pw & kSlotMask;
+            type = pw >> 30;
+            L = q.lpath[slot];
+            for (int b = bounce0 - 2; b < bounce0; ++b) {
+                if (b < 0) continue;
+                const uint32_t stamp = term_stamp(gen, b);
+                const float4 e = q.ebuf[b & 1][slot], nn = q.nbuf[b & 1][slot];
+                if (__float_as_uint(e.w) == stamp) add_term(L, e);
+                if (__float_as_uint(nn.w) == stamp) add_term(L, nn);
+            }
+            org = xyz(o4);
+            dir = xyz(d4);
+            thr = xyz(th4);
+            pdf = th4.w;
+            t = h.x;
+            prim = __float_as_int(h.y);
+        }
+        for (int b = bounce0;; ++b) {
+            ShadeOut sh;
+            sh.push_sh = sh.push_ext = false;
+            if (shader) {
+                shade_one(sc, so, views, segs, nseg, b, slot, type, org, dir, thr, pdf, t, prim, sh);
+                if (sh.contrib) add_term(L, term(sh.add, 0u));
+            }
+            // the shadow ray to the odd lane; the even lane traces the
+            // continuation, the odd lane the shadow ray, in one loop
+            const float sx = __shfl(sh.s_o.x, partner), sy = __shfl(sh.s_o.y, partner),
+                        sz = __shfl(sh.s_o.z, partner);
+            const float dx = __shfl(sh.s_d.x, partner), dy = __shfl(sh.s_d.y, partner),
+                        dz = __shfl(sh.s_d.z, partner);
+            const float tmax = __shfl(sh.s_tmax, partner);
+            const bool has_sh = __shfl((int)sh.push_sh, partner) != 0;
+            f3 ro, rd;
+            float rt;
+            bool go;
+            if (shader) {
+                ro = sh.e_o;
+                rd = sh.e_d;
+                rt = 1e34f;
+                go = sh.push_ext;
+            } else {
+                ro = mk3(sx, sy, sz);
+                rd = mk3(dx, dy, dz);
+                rt = tmax;
+                go = has_sh;
+            }
+            int hit = -1;
+            if (go) hit = trace_single<ACCEL>(sc, to.teps, ro, rd, rt, !shader, st, q.err);
+            const int occ = __shfl(hit, partner);
+            if (shader) {
+                if (sh.push_sh) {
+                    ++n_sh;
+                    if (occ == -1) add_term(L, term(sh.s_c, 0u));
+                }
+                if (sh.push_ext) {
+                    ++n_ext;
+                    org = sh.e_o;
+                    dir = sh.e_d;
+                    thr = sh.e_thr;
+                    pdf = sh.e_pdf;
+                    type = sh.e_type;
+                    t = rt;
+                    prim = hit;
+                    if (prim == -2) {
+                        t = 1e34f;
+                        prim = -1;
+                    }
+                }
+            }
+            const bool more = __shfl((int)sh.push_ext, (int)(__lane_id() & ~1u)) != 0;
+            if (!more) break;
+        }
+        if (shader) {
+            L.w = __uint_as_float((uint32_t)kMaxBounce);  // every term folded
+            q.lpath[slot] = L;
+        }
+    }
+    // one atomic per wave and counter
+    for (int off = 32; off > 0; off >>= 1) {
+        n_ext += __shfl_xor(n_ext, off);
+        n_sh += __shfl_xor(n_sh, off);
+    }
+    if (__lane_id() == 0) {
+        if (n_ext) atomicAdd(cnt_tail_ext(q.counters), n_ext);
+        if (n_sh) atomicAdd(cnt_tail_shadow(q.counters), n_sh);
     }
 }
 
@@ -711,6 +872,31 @@ hipError_t launch_trace(const LaunchShape& ls, const DevScene& sc, const TraceOp
         case 4: trace_t<4>(ls, sc, to, q, hdr, sel, bounce, wstats, s); break;
         case kTreeCompact: trace_t<kTreeCompact>(ls, sc, to, q, hdr, sel, bounce, wstats, s); break;
         default: trace_t<8>(ls, sc, to, q, hdr, sel, bounce, wstats, s); break;
+    }
+    return hipGetLastError();
+}
+hipError_t launch_tail(const LaunchShape& ls, const DevScene& sc, const TraceOpts& to, const ShadeOpts& so,
+                       const ViewDesc* views, const Segment* segs, const ChunkHdr* hdr, const Queues& q, int sel,
+                       int bounce0, hipStream_t s) {
+    const size_t lds = (size_t)ls.lds_depth * ls.block * 4;
+    const dim3 grid(ls.tail_blocks), block(ls.block);
+    switch (tree_of(to)) {
+        case 2:
+            hipLaunchKernelGGL(k_tail<2>, grid, block, lds, s, sc, to, so, views, segs, hdr, q, sel, bounce0,
+                               ls.lds_depth);
+            break;
+        case 4:
+            hipLaunchKernelGGL(k_tail<4>, grid, block, lds, s, sc, to, so, views, segs, hdr, q, sel, bounce0,
+                               ls.lds_depth);
+            break;
+        case kTreeCompact:
+            hipLaunchKernelGGL(k_tail<kTreeCompact>, grid, block, lds, s, sc, to, so, views, segs, hdr, q, sel,
+                               bounce0, ls.lds_depth);
+            break;
+        default:
+            hipLaunchKernelGGL(k_tail<8>, grid, block, lds, s, sc, to, so, views, segs, hdr, q, sel, bounce0,
+                               ls.lds_depth);
+            break;
     }
     return hipGetLastError();
 }
@@ -918,6 +1104,16 @@ bool set_trace_grids(LaunchShape& ls, int num_cus) {
     ls.grid8 = std::max(lo, num_cus * p8);
     ls.gridc = std::max(lo, num_cus * pc);
     ls.trace_blocks = std::max(std::max(ls.grid2, ls.grid4), std::max(ls.grid8, ls.gridc));
+    // k_tail: the blocks of its most register-hungry tree that fit at once
+    int pt = 1 << 30;
+    for (const void* f : {reinterpret_cast<const void*>(&ctd::k_tail<2>), reinterpret_cast<const void*>(&ctd::k_tail<4>),
+                          reinterpret_cast<const void*>(&ctd::k_tail<ctd::kTreeCompact>),
+                          reinterpret_cast<const void*>(&ctd::k_tail<8>)}) {
+        int nb = 0;
+        if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, f, ls.block, lds) != hipSuccess || nb <= 0) return false;
+        pt = std::min(pt, nb);
+    }
+    ls.tail_blocks = std::min(ls.trace_blocks, std::max(1, num_cus * pt));
     return true;
 }
 }  // namespace ctk

@@ -13,6 +13,7 @@ struct LaunchShape {
     int block = 256;          // threads per block for extend / connect
     int lds_depth = 16;       // per-lane LDS stack entries
     int trace_blocks = 1024;  // largest persistent grid (sizes the stack spill area)
+    int tail_blocks = 1024;   // k_tail's resident grid (<= trace_blocks)
     int grid2 = 1024, grid4 = 1024, grid8 = 1024, gridc = 1024;  // persistent grids per tree (c: compact quad)
     int shade_blocks = 1024;  // grid-stride grid of shade
     int max_blocks = 65536;   // cap for the simple grid-stride kernels
@@ -40,6 +41,14 @@ hipError_t launch_extend(const LaunchShape& ls, const ctd::DevScene& sc, const c
 hipError_t launch_trace(const LaunchShape& ls, const ctd::DevScene& sc, const ctd::TraceOpts& to,
                         const ctd::Queues& q, const ctd::ChunkHdr* hdr, int sel, int bounce,
                         unsigned long long* wstats, hipStream_t s);
+// The roulette tail of a chunk from bounce bounce0 on, after trace(bounce0):
+// each lane runs the remaining bounces of one path of bounce0's extension
+// queue (k_tail).  Grid: ls.tail_blocks blocks of ls.block threads, the
+// blocks that fit the GPU at once (within the stack spill area's
+// trace_blocks).
+hipError_t launch_tail(const LaunchShape& ls, const ctd::DevScene& sc, const ctd::TraceOpts& to,
+                       const ctd::ShadeOpts& so, const ctd::ViewDesc* views, const ctd::Segment* segs,
+                       const ctd::ChunkHdr* hdr, const ctd::Queues& q, int sel, int bounce0, hipStream_t s);
 hipError_t launch_shade(const LaunchShape& ls, const ctd::DevScene& sc, const ctd::ShadeOpts& so,
                         const ctd::ViewDesc* views, const ctd::Segment* segs, const ctd::ChunkHdr* hdr,
                         const ctd::Queues& q, int sel, int bounce, hipStream_t s);

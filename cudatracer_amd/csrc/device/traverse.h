@@ -647,4 +647,67 @@ CT_DEV void trace_persistent(const DevScene& sc, float eps, uint32_t n, uint32_t
     }
 }
 
+// One ray per lane, traced to its end: the step loop of trace_persistent
+// (the same node and triangle tests, pushes and visit order, so the same
+// result for the ray) for kernels whose lanes each follow a ray of their own
+// (k_tail).  t: in, the ray's extent; out, the closest hit's distance.
+// Returns the hit primitive, -1 for none, -2 when the stack bound was
+// exceeded (the error flag is raised).  any: stop at the first hit (a
+// per-lane flag, so closest-hit and any-hit lanes step together).
+template <int ACCEL>
+CT_DEV int trace_single(const DevScene& sc, float eps, f3 o, f3 d, float& t, bool any, LaneStack& st,
+                        uint32_t* err) {
+    const f3 inv = mk3(1.0f / d.x, 1.0f / d.y, 1.0f / d.z);
+    const bool fast = (ACCEL == 4 && fast_ray(o, inv, sc.plane_flags)) ||
+                      (ACCEL == kTreeCompact && compact_ray(o, inv));
+    int prim = -1;
+    uint32_t k = kNoLeaf;
+    st.reset();
+    st.push4(0u, true, 0u, false, 0u, false, 0u, false);
+    while (true) {
+        if (k == kNoLeaf) {
+            const uint32_t e = st.top_pop();
+            if (ACCEL == 4 || ACCEL == kTreeCompact || ACCEL == 8) {
+                if (e & kLeafBit) {
+                    k = e & ~kLeafBit;
+                } else if (ACCEL == 4) {
+                    if (fast) node4_fast(sc.nodes4f, e, o, inv, t, st);
+                    else node4_step(sc.nodes4, e, o, inv, t, st);
+                } else if (ACCEL == kTreeCompact) {
+                    if (fast) cnode4_step(sc.cnodes, e, o, inv, t, st);
+                    else node4_step(sc.nodes4, e, o, inv, t, st);
+                } else {
+                    fnode8_step(true, sc.fnodes, e, o, inv, t, st);
+                }
+            } else {
+                const float4 lo = sc.nodes2[2 * (size_t)e];
+                const float4 hi = sc.nodes2[2 * (size_t)e + 1];
+                const int count = (int)fbits(hi.w);
+                if (count > -1) {
+                    if (count > 0) k = fbits(lo.w);
+                } else {
+                    node2_step(sc.nodes2, fbits(lo.w), o, inv, st);
+                }
+            }
+        }
+        if (k != kNoLeaf) {
+            const float4 a = sc.tris[3 * (size_t)k];
+            const float4 b = sc.tris[3 * (size_t)k + 1];
+            const float4 c = sc.tris[3 * (size_t)k + 2];
+            const uint32_t w = fbits(a.w);
+            if (tri_test(o, d, t, xyz(a), xyz(b), xyz(c), eps)) {
+                prim = (int)(w & ~kLastInLeaf);
+                if (any) break;
+            }
+            k = (w & kLastInLeaf) ? kNoLeaf : k + 1;
+        }
+        if (st.overflow) {
+            atomicOr(err, 1u);
+            return -2;
+        }
+        if (k == kNoLeaf && st.empty()) break;
+    }
+    return prim;
+}
+
 }  // namespace ctd

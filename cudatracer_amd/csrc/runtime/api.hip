@@ -557,6 +557,7 @@ static void fill_stats(const ChunkStats& s, ct_frame_stats* out) {
     out->shade_ms = s.shade_ms;
     out->connect_ms = s.connect_ms;
     out->other_ms = s.other_ms;
+    out->tail_ms = s.tail_ms;
 }
 
 int ct_trace_frame(ct_ctx* c, const ct_camera* cam, const ct_frame_opts* opts, ct_frame_stats* stats) {

@@ -14,7 +14,7 @@
 
 struct ChunkStats {
     uint64_t primary = 0, extension = 0, shadow = 0, samples = 0;
-    double extend_ms = 0, shade_ms = 0, connect_ms = 0, other_ms = 0, total_ms = 0;
+    double extend_ms = 0, shade_ms = 0, connect_ms = 0, other_ms = 0, total_ms = 0, tail_ms = 0;
     uint64_t work[4] = {0, 0, 0, 0};  // closest-hit node/tri tests, shadow node/tri tests
     int extend_launches = 0;
     int chunks = 0;
@@ -25,6 +25,17 @@ struct ChunkStats {
 #endif
 // Pipelines the chunk driver rotates through.
 constexpr int kLanes = CT_LANES;
+// The bounce from which a chunk's remaining bounces run in one k_tail
+// launch (tail_bounce): with roulette, rr_start + CT_TAIL_AFTER_ROULETTE;
+// without, max_depth - CT_TAIL_BEFORE_END (0: no tail).
+#ifndef CT_TAIL_AFTER_ROULETTE
+#define CT_TAIL_AFTER_ROULETTE 0
+#endif
+#ifndef CT_TAIL_BEFORE_END
+#define CT_TAIL_BEFORE_END 0
+#endif
+constexpr int kTailAfterRoulette = CT_TAIL_AFTER_ROULETTE;
+constexpr int kTailBeforeEnd = CT_TAIL_BEFORE_END;
 static_assert(kLanes >= 2, "at least two lanes");
 
 // One of the pipelines the chunk driver rotates through.  Each lane
@@ -56,9 +67,10 @@ struct Lane {
     int max_depth = 0;
     bool timed = false, count = false;
     bool nee = false;  // the chunk ends with a launch for the last bounce's shadow rays
+    int tail = -1;     // the chunk's k_tail bounce, or -1
 };
 // h_read layout (uint32 words)
-constexpr int kReadCounters = 2 * (ctd::kMaxBounce + 1);
+constexpr int kReadCounters = 2 * (ctd::kMaxBounce + 1) + 2;  // per-bounce counts, then k_tail's two
 constexpr int kReadErr = kReadCounters;
 constexpr int kReadWork = kReadCounters + 2;  // 4 x uint64, 8-byte aligned
 constexpr int kReadWords = kReadWork + 8;

@@ -341,6 +341,17 @@ static ctd::ShadeOpts shade_opts(const ct_frame_opts& o) {
 // so every lane starts at once instead of after the host has enqueued the
 // whole of the previous lanes' chunks.  timed records an event pair around
 // every kernel.
+// The bounce from which a chunk's remaining bounces run in one k_tail
+// launch, or -1 (a counting frame: k_tail does not count node and triangle
+// tests).
+int tail_bounce(const ct_frame_opts& o, bool count) {
+    if (count) return -1;
+    int b0 = -1;
+    if (o.rr_start >= 0) b0 = std::max(0, o.rr_start + kTailAfterRoulette);
+    else if (kTailBeforeEnd > 0) b0 = std::max(0, o.max_depth - kTailBeforeEnd);
+    return b0 >= 0 && b0 + 1 < o.max_depth ? b0 : -1;
+}
+
 struct BounceSeq {
     ct_ctx* c = nullptr;
     Lane* L = nullptr;
@@ -349,13 +360,14 @@ struct BounceSeq {
     ctd::ShadeOpts so{};
     unsigned long long* ws = nullptr;
     bool timed = false, nee = false;
+    int tail = -1;  // k_tail from this bounce on, or -1
     int e = 0, sel = 0;
 
     BounceSeq() = default;
     BounceSeq(ct_ctx* c_, Lane* L_, const ct_frame_opts& o_, bool count, bool timed_)
         : c(c_), L(L_), o(o_), to(trace_opts(o_)), so(shade_opts(o_)), ws(count ? L_->wstats : nullptr),
-          timed(timed_), nee(o_.shadow && c_->sc.n_lights > 0) {}
-    int steps() const { return 1 + o.max_depth; }
+          timed(timed_), nee(o_.shadow && c_->sc.n_lights > 0), tail(tail_bounce(o_, count)) {}
+    int steps() const { return 1 + (tail >= 0 ? tail + 1 : o.max_depth); }
     void rec() {
         if (timed) (void)hipEventRecord(L->ev[e++], L->stream);
     }
@@ -374,25 +386,30 @@ struct BounceSeq {
         // trace(b): the extension rays of bounce b and the shadow rays of
         // bounce b - 1 (their radiance terms are added by shade(b + 1) or
         // resolve, in order); then shade(b) appends bounce b + 1's extension
-        // rays and bounce b's shadow rays
+        // rays and bounce b's shadow rays, or, at the tail bounce, k_tail
+        // runs every remaining bounce
         const int b = k - 1;
         rec();
         if ((r = ctk::launch_trace(c->shape, c->sc, to, L->q, L->d_hdr, sel, b, ws, s)) != hipSuccess) return r;
         rec();
         rec();
-        if ((r = ctk::launch_shade(c->shape, c->sc, so, L->d_views, L->d_segs, L->d_hdr, L->q, sel, b, s)) !=
-            hipSuccess)
-            return r;
+        if (b == tail)
+            r = ctk::launch_tail(c->shape, c->sc, to, so, L->d_views, L->d_segs, L->d_hdr, L->q, sel, b, s);
+        else
+            r = ctk::launch_shade(c->shape, c->sc, so, L->d_views, L->d_segs, L->d_hdr, L->q, sel, b, s);
+        if (r != hipSuccess) return r;
         rec();
         sel ^= 1;
         return hipSuccess;
     }
-    // the last bounce's shadow rays (the launch has no extension rays)
+    // the last bounce's shadow rays (the launch has no extension rays; k_tail
+    // traces its own)
     hipError_t finish() {
         hipError_t r;
         rec();
-        if (nee && (r = ctk::launch_trace(c->shape, c->sc, to, L->q, L->d_hdr, sel, o.max_depth, ws, L->stream)) !=
-                       hipSuccess)
+        if (nee && tail < 0 &&
+            (r = ctk::launch_trace(c->shape, c->sc, to, L->q, L->d_hdr, sel, o.max_depth, ws, L->stream)) !=
+                hipSuccess)
             return r;
         rec();
         L->n_events = e;
@@ -510,6 +527,7 @@ static int close_chunk(ct_ctx* c, Lane& L, uint32_t n, float4* accum, const ct_f
     L.n_paths = n;
     L.max_depth = o.max_depth;
     L.nee = o.shadow && c->sc.n_lights > 0;
+    L.tail = tail_bounce(o, count);
     L.timed = timed;
     L.count = count;
     return CT_OK;
@@ -530,12 +548,15 @@ static int finish_chunk(ct_ctx* c, Lane& L, ChunkStats* st) {
     st->primary += cnt[0];
     for (int b = 1; b < L.max_depth; ++b) st->extension += cnt[b];
     for (int b = 0; b < L.max_depth; ++b) st->shadow += cnt[(ctd::kMaxBounce + 1) + b];
+    st->extension += cnt[2 * (ctd::kMaxBounce + 1)];  // k_tail's (cnt_tail_ext, cnt_tail_shadow)
+    st->shadow += cnt[2 * (ctd::kMaxBounce + 1) + 1];
     st->samples += L.n_paths;
     float ms = 0;
     if (hipEventElapsedTime(&ms, L.ev[kEvTotal0], L.ev[kEvTotal0 + 1]) == hipSuccess) st->total_ms += ms;
     if (L.timed) {
-        // event pairs: [generate] then per bounce [trace][shade]; [trace of
-        // the last shadow rays]; [resolve]
+        // event pairs: [generate] then per bounce [trace][shade] (at the tail
+        // bounce [trace][k_tail], the last bounce the chunk launches); [trace
+        // of the last shadow rays]; [resolve]
         int e = 0;
         auto pair = [&](int i) {
             float m = 0;
@@ -544,13 +565,15 @@ static int finish_chunk(ct_ctx* c, Lane& L, ChunkStats* st) {
         };
         st->other_ms += pair(e);
         e += 2;
-        for (int b = 0; b < L.max_depth; ++b) {
+        const int nb = L.tail >= 0 ? L.tail + 1 : L.max_depth;
+        for (int b = 0; b < nb; ++b) {
             st->extend_ms += pair(e);
             st->extend_launches += 1;
-            st->shade_ms += pair(e + 2);
+            if (b == L.tail) st->tail_ms += pair(e + 2);
+            else st->shade_ms += pair(e + 2);
             e += 4;
         }
-        if (L.nee) {
+        if (L.nee && L.tail < 0) {
             st->connect_ms += pair(e);
             st->extend_launches += 1;
         }
@@ -716,7 +739,8 @@ void ctr_apply_shape(ct_ctx* c, int block, int lds_depth) {
     c->shape.block = block;
     c->shape.lds_depth = lds_depth;
     if (!ctk::set_trace_grids(c->shape, c->num_cus)) {
-        c->shape.grid2 = c->shape.grid4 = c->shape.grid8 = c->shape.gridc = c->shape.trace_blocks = std::max(8, c->num_cus * 4);
+        c->shape.grid2 = c->shape.grid4 = c->shape.grid8 = c->shape.gridc = c->shape.trace_blocks = c->shape.tail_blocks =
+            std::max(8, c->num_cus * 4);
     }
     c->shape.shade_blocks = std::max(1, c->num_cus * 8);
 }

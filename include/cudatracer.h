@@ -27,7 +27,7 @@
 extern "C" {
 #endif
 
-#define CT_ABI_VERSION 4
+#define CT_ABI_VERSION 5
 
 /* ---------------------------------------------------------------- status */
 enum {
@@ -201,7 +201,13 @@ typedef struct ct_frame_stats {
     uint32_t chunks;          /* wavefront chunks the frame was split into    */
     uint32_t extend_launches; /* traversal launches of the bounces (chunks x depth;
                                  plus one per chunk for the last shadow rays
-                                 when NEE is on)                              */
+                                 when NEE is on; with roulette, the bounces
+                                 up to the tail's first)                      */
+    double tail_ms;           /* with roulette (rr_start >= 0): the device time of
+                                 the launches that run each chunk's remaining
+                                 bounces once roulette has thinned its paths
+                                 (a lane per path: shading, the light sample's
+                                 shadow ray and the next ray traced in turn) */
 } ct_frame_stats;
 
 /* Fill with the reference defaults (depth 5, 1 spp, no RR, BVH4, reference BSDF,

@@ -348,6 +348,7 @@ def test_radiance_beckmann_roulette(ct, orc, gpu_ctx):
     ref, ost = osc.render(cam, opts, w, h, 0, 2, threads=16)
     assert (st.primary_rays, st.extension_rays, st.shadow_rays) == (
         ost["primary_rays"], ost["extension_rays"], ost["shadow_rays"])
+    assert st.tail_ms > 0.0  # bounces 5-15 ran in the roulette-tail launch
     diff = np.abs(acc - ref)
     bad = int(np.sum((diff / np.maximum(np.abs(ref), 1e-30) > 1e-4) & (diff > 1e-7)))
     same = float(np.mean(acc.view(np.uint32) == ref.view(np.uint32)))
