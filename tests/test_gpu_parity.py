@@ -381,6 +381,38 @@ def test_batch_depth8_vs_oracle(ct, orc, gpu_ctx, c1):
         assert np.array_equal(out.view(np.uint32), ref.view(np.uint32)), (w, h, spp)
 
 
+@pytest.mark.parametrize("rr_start", [0, 2])
+def test_roulette_tail_batch_vs_oracle(ct, orc, gpu_ctx, c1, rr_start):
+    """The roulette tail (k_tail: a lane pair per path from bounce rr_start
+    on) under ct_render_batch, whose chunks mix views of uneven size and spp:
+    bit-identical to the FP32 oracle, with the ray counts equal to the
+    oracle's (the batch runs untimed, from its captured graph; the tail's
+    launch time is checked in test_radiance_beckmann_roulette).  rr_start 0
+    puts the tail right after the primary rays' trace, so k_tail shades every
+    bounce."""
+    scene, arrays, osc = c1
+    gpu_ctx.upload(scene)
+    opts, _, _ = scene.frame_opts()
+    opts.max_depth = 10
+    opts.rr_start = rr_start
+    rng = np.random.default_rng(10 + rr_start)
+    views = []
+    for i, (w, h, spp) in enumerate([(120, 90, 3), (64, 64, 8), (150, 100, 1)]):
+        d = rng.normal(size=3)
+        d /= np.linalg.norm(d)
+        cam = ct.look_at(d * 42.0, rng.uniform(-5, 5, 3), 70.0)
+        views.append((cam, w, h, spp, 5 * i))
+    outs, st = gpu_ctx.render_batch(views, opts)
+    rays = [0, 0, 0]
+    for (cam, w, h, spp, fb), out in zip(views, outs):
+        ref, ost = osc.render(cam, opts, w, h, fb, spp, threads=16)
+        assert np.array_equal(out.view(np.uint32), ref.view(np.uint32)), (w, h, spp)
+        rays[0] += ost["primary_rays"]
+        rays[1] += ost["extension_rays"]
+        rays[2] += ost["shadow_rays"]
+    assert (st.primary_rays, st.extension_rays, st.shadow_rays) == tuple(rays)
+
+
 @pytest.mark.parametrize("config,size", [(1, (256, 256)), (2, (480, 270))])
 def test_ray_and_work_counts_equal_oracle(ct, orc, gpu_ctx, config, size):
     """The counts behind the benchmark's numbers: primary, extension and
