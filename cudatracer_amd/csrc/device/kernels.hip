@@ -834,12 +834,15 @@ int tree_of(const TraceOpts& to) {
 int grid_of(const LaunchShape& ls, int tree) {
     return tree == 2 ? ls.grid2 : (tree == 4 ? ls.grid4 : (tree == kTreeCompact ? ls.gridc : ls.grid8));
 }
+int full_of(const LaunchShape& ls, int tree) {
+    return tree == 2 ? ls.full2 : (tree == 4 ? ls.full4 : (tree == kTreeCompact ? ls.fullc : ls.full8));
+}
 
 template <int TREE>
 void extend_t(const LaunchShape& ls, const DevScene& sc, const TraceOpts& to, const Queues& q, int sel, int bounce,
               hipStream_t s) {
     const size_t lds = (size_t)ls.lds_depth * ls.block * 4;
-    const dim3 grid(grid_of(ls, TREE)), block(ls.block);
+    const dim3 grid(full_of(ls, TREE)), block(ls.block);
     hipLaunchKernelGGL((k_extend<TREE>), grid, block, lds, s, sc, to, q, sel, bounce, ls.lds_depth);
 }
 template <int TREE>
@@ -1065,7 +1068,7 @@ hipError_t launch_shadow_flags(const LaunchShape& ls, const DevScene& sc, const 
                                uint32_t n, uint8_t* out, hipStream_t s) {
     const size_t lds = (size_t)ls.lds_depth * ls.block * 4;
     const int tree = tree_of(to);
-    const dim3 grid(grid_of(ls, tree)), block(ls.block);
+    const dim3 grid(full_of(ls, tree)), block(ls.block);
     if (tree == 2) hipLaunchKernelGGL(k_shadow_flags<2>, grid, block, lds, s, sc, to, q, n, ls.lds_depth, out);
     else if (tree == 4) hipLaunchKernelGGL(k_shadow_flags<4>, grid, block, lds, s, sc, to, q, n, ls.lds_depth, out);
     else if (tree == kTreeCompact)
@@ -1099,11 +1102,16 @@ bool set_trace_grids(LaunchShape& ls, int num_cus) {
     // at least one block per fetch range (the persistent kernels drain ranges
     // by block id)
     const int lo = ctd::kFetchRanges;
-    ls.grid2 = std::max(lo, num_cus * p2);
-    ls.grid4 = std::max(lo, num_cus * p4);
-    ls.grid8 = std::max(lo, num_cus * p8);
-    ls.gridc = std::max(lo, num_cus * pc);
-    ls.trace_blocks = std::max(std::max(ls.grid2, ls.grid4), std::max(ls.grid8, ls.gridc));
+    const int fill = std::min(8, std::max(1, ls.trace_fill));
+    ls.full2 = std::max(lo, num_cus * p2);
+    ls.full4 = std::max(lo, num_cus * p4);
+    ls.full8 = std::max(lo, num_cus * p8);
+    ls.fullc = std::max(lo, num_cus * pc);
+    ls.grid2 = std::max(lo, ls.full2 * fill / 8);
+    ls.grid4 = std::max(lo, ls.full4 * fill / 8);
+    ls.grid8 = std::max(lo, ls.full8 * fill / 8);
+    ls.gridc = std::max(lo, ls.fullc * fill / 8);
+    ls.trace_blocks = std::max(std::max(ls.full2, ls.full4), std::max(ls.full8, ls.fullc));
     // k_tail: the blocks of its most register-hungry tree that fit at once
     int pt = 1 << 30;
     for (const void* f : {reinterpret_cast<const void*>(&ctd::k_tail<2>), reinterpret_cast<const void*>(&ctd::k_tail<4>),

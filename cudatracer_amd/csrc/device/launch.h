@@ -12,9 +12,15 @@ namespace ctk {
 struct LaunchShape {
     int block = 256;          // threads per block for extend / connect
     int lds_depth = 16;       // per-lane LDS stack entries
-    int trace_blocks = 1024;  // largest persistent grid (sizes the stack spill area)
+    int trace_blocks = 1024;  // largest persistent grid at full fill (sizes the stack spill area)
     int tail_blocks = 1024;   // k_tail's resident grid (<= trace_blocks)
     int grid2 = 1024, grid4 = 1024, grid8 = 1024, gridc = 1024;  // persistent grids per tree (c: compact quad)
+    // Eighths of the resident blocks that the frame loop's persistent
+    // traversal grids (grid2 ... gridc) fill.  Below 8 the other lane's shade
+    // kernel runs beside a traversal launch instead of waiting for its waves
+    // to retire.  The ray-level launches always fill every slot (full2 ...).
+    int trace_fill = 8;
+    int full2 = 1024, full4 = 1024, full8 = 1024, fullc = 1024;
     int shade_blocks = 1024;  // grid-stride grid of shade
     int max_blocks = 65536;   // cap for the simple grid-stride kernels
 };

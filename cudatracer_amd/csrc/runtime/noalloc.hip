@@ -81,10 +81,10 @@ extern "C" int ct_trace_rays_device_noalloc(ct_ctx* c, const void* d_rays_o, con
         return fail(c, CT_ERR_INVALID_ARG, "launch shape (block size x stack depth) needs more than the 32 MiB "
                                            "scratch array; use a smaller block or ct_trace_rays_device");
     const int cap = (int)std::min<size_t>(fit, (size_t)1 << 30);
-    ls.grid2 = std::min(ls.grid2, cap);
-    ls.grid4 = std::min(ls.grid4, cap);
-    ls.grid8 = std::min(ls.grid8, cap);
-    ls.gridc = std::min(ls.gridc, cap);
+    ls.full2 = std::min(ls.full2, cap);
+    ls.full4 = std::min(ls.full4, cap);
+    ls.full8 = std::min(ls.full8, cap);
+    ls.fullc = std::min(ls.fullc, cap);
     uint8_t* base = static_cast<uint8_t*>(c->scratch);  // the device address, never the symbol itself
     ctd::Queues q{};
     q.ray_o[0] = (float4*)d_rays_o;

@@ -739,8 +739,8 @@ void ctr_apply_shape(ct_ctx* c, int block, int lds_depth) {
     c->shape.block = block;
     c->shape.lds_depth = lds_depth;
     if (!ctk::set_trace_grids(c->shape, c->num_cus)) {
-        c->shape.grid2 = c->shape.grid4 = c->shape.grid8 = c->shape.gridc = c->shape.trace_blocks = c->shape.tail_blocks =
-            std::max(8, c->num_cus * 4);
+        c->shape.grid2 = c->shape.grid4 = c->shape.grid8 = c->shape.gridc = c->shape.full2 = c->shape.full4 =
+            c->shape.full8 = c->shape.fullc = c->shape.trace_blocks = c->shape.tail_blocks = std::max(8, c->num_cus * 4);
     }
     c->shape.shade_blocks = std::max(1, c->num_cus * 8);
 }
