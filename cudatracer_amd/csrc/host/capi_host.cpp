@@ -21,6 +21,7 @@ static_assert(sizeof(ct_bvh_node) == 32, "ct_bvh_node must be the 32-byte BVHNod
 static_assert(sizeof(ct_mbvh_node) == 128, "ct_mbvh_node must be the 128-byte MBVHNode");
 static_assert(sizeof(ct_camera) == 40, "ct_camera: position, forward, up, plane distance");
 static_assert(sizeof(ct_frame_opts) == 52, "ct_frame_opts: 13 four-byte fields");
+static_assert(sizeof(ct_frame_stats) == 88, "ct_frame_stats: 4 counts, 6 times, 2 launch counts (ABI 5)");
 
 struct ct_host_scene {
     ct::TriangleList list;

@@ -539,6 +539,18 @@ def test_cabi_exports_every_declared_symbol(ct):
         assert hasattr(ct.lib, n), f"{n} declared in include/cudatracer.h but not exported"
 
 
+def test_cabi_struct_layouts_match_the_binding(ct):
+    """The ctypes mirrors have the sizes the library is built with (the C side
+    checks them with static_asserts in host/capi_host.cpp) and the library
+    reports the header's ABI version."""
+    import ctypes as C
+    assert C.sizeof(ct.Material) == 52 and C.sizeof(ct.Camera) == 40
+    assert C.sizeof(ct.FrameOpts) == 52 and C.sizeof(ct.FrameStats) == 88
+    ver = int(re.search(r"#define CT_ABI_VERSION (\d+)", (ROOT / "include" / "cudatracer.h").read_text()).group(1))
+    assert ct.lib.ct_abi_version() == ver
+    assert (ct.TRAV_EXACT, ct.TRAV_FAST, ct.TRAV_COMPACT) == (0, 1, 2)
+
+
 def test_cabi_host_errors(ct):
     s = ct.HostScene.empty()
     with pytest.raises(ct.CTError):
