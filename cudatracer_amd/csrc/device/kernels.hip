@@ -645,8 +645,8 @@ k_tail(DevScene sc, TraceOpts to, ShadeOpts so, const ViewDesc* __restrict__ vie
             prim = __float_as_int(h.y);
         }
         for (int b = bounce0;; ++b) {
-            ShadeOut sh;
-            sh.push_sh = sh.push_ext = false;
+            // zeroed: the odd lane's copy is only read through the shuffles
+            ShadeOut sh{};
             if (shader) {
                 shade_one(sc, so, views, segs, nseg, b, slot, type, org, dir, thr, pdf, t, prim, sh);
                 if (sh.contrib) add_term(L, term(sh.add, 0u));
