@@ -115,6 +115,36 @@ def test_server_honours_every_option(ct, gpu_ctx, c2):
             assert not np.array_equal(_bits(out), _bits(base))  # the option took effect
 
 
+def test_served_roulette_tail_vs_oracle(ct, orc, gpu_ctx, c2):
+    """Requests with Russian roulette from bounce 0 and from bounce 2 (so
+    each server chunk ends in the launch that runs every path's remaining
+    bounces), submitted together with a plain one: each result equals the
+    FP32 oracle's render bit for bit."""
+    scene, _, osc = c2
+    gpu_ctx.upload(scene)
+    opts, _, _ = scene.frame_opts()
+    cam = ct.look_at((-82.0, 10.0, -70.0), (0.0, 10.0, 12.0), 70.0)
+    w, h, spp = 128, 80, 2
+    variants = []
+    for rr in (0, 2, -1):
+        o = opts.copy()
+        o.spp = spp
+        o.rr_start = rr
+        variants.append(o)
+    srv = ct.Server(gpu_ctx, chunk_paths=1 << 13)
+    try:
+        ids = [srv.submit(cam, w, h, spp, o, frame_base=3) for o in variants]
+        outs = [srv.wait(rid)[1] for rid in ids]
+    finally:
+        srv.close()
+    for o, out in zip(variants, outs):
+        print(f"request {w}x{h} spp {spp} rr_start {o.rr_start}", flush=True)
+        ref = np.zeros((h, w, 4), np.float32)
+        osc.render(cam, o, w, h, 3, spp, threads=16, accum=ref)
+        assert np.array_equal(_bits(out), _bits(ref)), o.rr_start
+        assert np.all(out[..., 3] == spp)
+
+
 def test_server_refuses_bad_requests(ct, gpu_ctx, c1):
     scene, _, _ = c1
     gpu_ctx.upload(scene)
