@@ -502,6 +502,7 @@ def serve_leg(ct, ctx, scene, opts, seed: int, n: int = 120, rate: float = 40.0,
             "plain_p50_ms": _pct(plain_lat, 50), "plain_p99_ms": _pct(plain_lat, 99),
             "images_per_s": round(served_ips, 2), "plain_images_per_s": round(plain_ips, 2),
             "speedup": round(served_ips / plain_ips, 3), "plain_multi_spp_images_per_s": round(multi_ips, 2),
+            "speedup_vs_multi_spp": round(served_ips / multi_ips, 3),
             "bit_exact_vs_plain": f"{same}/{len(plain_out)}"}
 
 
@@ -546,8 +547,8 @@ def batched_leg(ct, ctx, seed: int, n_views: int = 64) -> dict:
     return {"views": n_views, "path_samples": int(sum(w * h * spp for (_, w, h, spp, _) in views)),
             "images_per_s": round(n_views / t_batch, 2), "plain_images_per_s": round(n_views / t_plain, 2),
             "speedup": round(t_plain / t_batch, 3), "plain_multi_spp_images_per_s": round(n_views / t_multi, 2),
-            "chunks": int(st.chunks), "autotuned": {"block": tune.block_size, "stack_lds": tune.stack_lds,
-                                                     "chunk_paths": tune.chunk_paths},
+            "speedup_vs_multi_spp": round(t_multi / t_batch, 3), "chunks": int(st.chunks),
+            "autotuned": {"block": tune.block_size, "stack_lds": tune.stack_lds, "chunk_paths": tune.chunk_paths},
             "bit_exact_vs_plain": f"{same}/8"}
 
 
