@@ -381,6 +381,46 @@ def test_batch_depth8_vs_oracle(ct, orc, gpu_ctx, c1):
         assert np.array_equal(out.view(np.uint32), ref.view(np.uint32)), (w, h, spp)
 
 
+@pytest.mark.parametrize("mode", ["exact", "compact", "fast"])
+def test_roulette_tail_each_quad_traversal(ct, orc, gpu_ctx, c2, mode):
+    """The roulette tail (k_tail) over each traversal of the quad tree's
+    family on the atrium (depth 12, roulette from bounce 2, 320x180, 2 spp):
+    with the exact traversal the frame is bit-identical to the FP32 oracle
+    and its ray counts are the oracle's; the compact quad tree and the wide
+    tree give the exact traversal's frame except at the few pixels whose path
+    met a t-tie or a ulp-level t difference."""
+    scene, _, osc = c2
+    _upload(gpu_ctx, scene)
+    opts, _, _ = scene.frame_opts()
+    opts.accel = ct.ACCEL_BVH4
+    opts.max_depth = 12
+    opts.rr_start = 2
+    opts.spp = 2
+    w, h = 320, 180
+    cam = scene.camera(w, h)
+    gpu_ctx.resize(w, h)
+    st = gpu_ctx.trace_frame(cam, opts)
+    exact = gpu_ctx.read_accum()[0].copy()
+    assert st.tail_ms > 0.0  # bounces 2-11 ran in the roulette-tail launch
+    if mode == "exact":
+        ref, ost = osc.render(cam, opts, w, h, 0, 2, threads=16)
+        assert np.array_equal(exact.view(np.uint32), ref.view(np.uint32))
+        assert (st.primary_rays, st.extension_rays, st.shadow_rays) == (
+            ost["primary_rays"], ost["extension_rays"], ost["shadow_rays"])
+        return
+    o = opts.copy()
+    o.traversal = ct.TRAV_FAST if mode == "fast" else ct.TRAV_COMPACT
+    gpu_ctx.reset()
+    st2 = gpu_ctx.trace_frame(cam, o)
+    got = gpu_ctx.read_accum()[0]
+    assert st2.tail_ms > 0.0
+    same = np.all(got.view(np.uint32) == exact.view(np.uint32), axis=-1)
+    print(f"roulette tail, {mode} traversal: {int(same.sum())} of {same.size} pixels bit-identical to exact")
+    assert same.mean() >= 0.999
+    rel = float(np.mean((got[..., :3] - exact[..., :3]) ** 2) / np.mean(exact[..., :3] ** 2))
+    assert rel < 1e-3, rel
+
+
 @pytest.mark.parametrize("rr_start", [0, 2])
 def test_roulette_tail_batch_vs_oracle(ct, orc, gpu_ctx, c1, rr_start):
     """The roulette tail (k_tail: a lane pair per path from bounce rr_start
