@@ -67,6 +67,34 @@ def test_textured_atrium_vs_oracle(ct, orc, gpu_ctx):
     assert not np.array_equal(_frame(gpu_ctx, cam, opts, w, h), acc)
 
 
+def test_textured_atrium_roulette_tail(ct, orc, gpu_ctx):
+    """Config 4 with Russian roulette from bounce 1 at depth 10 (160x90,
+    2 spp), so the roulette-tail launch shades textured, normal-mapped
+    surfaces and the textured sky for every bounce from 1 on: every value
+    within 1e-4 of the FP32 oracle and the ray counts equal the oracle's."""
+    scene = ct.HostScene.generate(4, 7)
+    osc = orc.Scene(scene.arrays())
+    gpu_ctx.upload(scene)
+    opts, _, _ = scene.frame_opts()
+    w, h = 160, 90
+    opts.spp = 2
+    opts.max_depth = 10
+    opts.rr_start = 1
+    cam = scene.camera(w, h)
+    gpu_ctx.resize(w, h)
+    st = gpu_ctx.trace_frame(cam, opts)
+    acc, _ = gpu_ctx.read_accum()
+    assert st.tail_ms > 0.0
+    ref, ost = osc.render(cam, opts, w, h, 0, 2, threads=16)
+    assert (st.primary_rays, st.extension_rays, st.shadow_rays) == (
+        ost["primary_rays"], ost["extension_rays"], ost["shadow_rays"])
+    diff = np.abs(acc - ref)
+    bad = int(np.sum((diff / np.maximum(np.abs(ref), 1e-30) > 1e-4) & (diff > 1e-7)))
+    exact = int(np.sum(_bits(acc) == _bits(ref)))
+    print(f"textured atrium, roulette tail: over 1e-4 rel {bad}, bit-identical {exact}/{acc.size}")
+    assert bad == 0
+
+
 def test_gpu_texture_invariants(ct, orc, gpu_ctx, c1, c2):
     scene, arrays, _ = c1
     opts, _, _ = scene.frame_opts()
