@@ -94,6 +94,35 @@ def test_deep_chain_within_bound_matches_oracle(ct, orc, gpu_ctx):
     assert np.array_equal(occ, occ_ref)
 
 
+def test_deep_chain_frame_with_roulette_tail(ct, orc, gpu_ctx):
+    """A frame over the 42-level chain (127 stack entries, 111 of them
+    spilled) with roulette from bounce 0, so every bounce after the primary
+    rays is traced by the roulette-tail launch's per-lane traversal through
+    the spill region: bit-identical to the FP32 oracle, with the oracle's
+    ray counts."""
+    arrays = chain_scene(ct, 42, seed=1)
+    av = ArrayView(ct, arrays)
+    gpu_ctx.upload_view(av.view)
+    osc = orc.Scene(arrays, build=False)
+    osc.set_trees(arrays["bvh"], arrays["mbvh"], arrays["prim_indices"])
+    opts = ct.default_opts()
+    opts.max_depth = 6
+    opts.rr_start = 0
+    opts.spp = 2
+    cam = ct.look_at((-30.0, 0.5, 0.3), (0.0, 0.0, 0.0), 20.0)
+    w = h = 48
+    gpu_ctx.resize(w, h)
+    st = gpu_ctx.trace_frame(cam, opts)
+    acc, _ = gpu_ctx.read_accum()
+    ref, ost = osc.render(cam, opts, w, h, 0, 2, threads=16)
+    print(f"chain frame: primary {st.primary_rays}, extension {st.extension_rays}, shadow {st.shadow_rays}")
+    assert st.tail_ms > 0.0
+    assert st.extension_rays > 0
+    assert (st.primary_rays, st.extension_rays, st.shadow_rays) == (
+        ost["primary_rays"], ost["extension_rays"], ost["shadow_rays"])
+    assert np.array_equal(acc.view(np.uint32), ref.view(np.uint32))
+
+
 def test_stack_overflow_reported_and_cleared(ct, gpu_ctx, c1):
     """50 levels need 151 entries: ct_trace_rays, ct_trace_rays_device and
     ct_trace_frame return CT_ERR_STACK_OVERFLOW (the kernels end normally,
