@@ -46,13 +46,15 @@ cameras).  p50 / p99 of request latency (completion - arrival) for the render
 server (continuous batching) and for the plain per-view loop of the reference
 (one request at a time, one ct_trace_frame call per 1-spp frame, like
 launchKernels); images/s of both with all requests queued at once, and of the
-plain loop with all spp of a request in one call.  Every request renders the
-same path samples either way (checked bit-exact on a subset), so the
+plain loop with all spp of a request in one call (speedup: over the per-frame
+loop; speedup_vs_multi_spp: over the one-call loop).  Every request renders
+the same path samples either way (checked bit-exact on a subset), so the
 comparison is at equal rel-MSE.
 
 batched: 64 views of the box-and-sphere scene (sides log-uniform in
 [128, 1024], aspect 1:1 or 4:3, spp in {1, 2, 4, 8, 16}, depth 8) in one
-ct_render_batch call after ct_autotune vs the plain per-view loop; images/s.
+ct_render_batch call after ct_autotune vs the same two plain per-view loops;
+images/s and the speedup over each.
 """
 from __future__ import annotations
 
