@@ -393,7 +393,10 @@ void gen_c3(uint64_t seed, TriangleList& L, SceneConfig& cfg) {
     cfg.opts.rr_start = 3;
     cfg.opts.accel = CT_ACCEL_BVH4;
     cfg.opts.bsdf_mode = CT_BSDF_TYPED;
-    cfg.opts.sort_by_material = 1;  // mixed BSDFs: shade in material order
+    // two microfacet types that shade along nearly the same code: grouping
+    // them saves less than the sort's key lookups cost (shading 8% faster
+    // without the sort on this scene)
+    cfg.opts.sort_by_material = 0;
     cfg.opts.normal_epsilon = 1e-3f;
     cfg.opts.dist_epsilon = 1e-2f;
     cfg.width = 3840;
