@@ -332,18 +332,6 @@ void install_scene(ct_ctx* c, const ct_scene_view* v, ctd::DevScene sc, const Sh
     c->h_light_idx.assign(v->light_indices, v->light_indices + v->n_lights);
     c->h_light_rows.assign(reinterpret_cast<const float*>(sh.lights.data()),
                            reinterpret_cast<const float*>(sh.lights.data()) + sh.lights.size() * 4);
-    // How full the persistent traversal grids run: a scene whose nodes and
-    // triangle records fit the Infinity Cache with room to spare keeps the
-    // traversal issue-bound, and leaving three eighths of the wave slots to
-    // the other lane's shade kernel shortens the frame (C2, 21 MB: +4%); a
-    // larger scene is bound by memory latency and needs every wave slot for
-    // traversal (C3, 0.65 GB: -2% at half the grid).
-    const uint64_t bytes = 48ull * N + std::max(128ull * sc.n_nodes4, 32ull * sc.n_nodes2);
-    const int fill = bytes <= kResidentSceneBytes ? 5 : 8;
-    if (fill != c->shape.trace_fill) {
-        c->shape.trace_fill = fill;
-        ctr_apply_shape(c, c->shape.block, c->shape.lds_depth);
-    }
     drop_graphs(c);
 }
 

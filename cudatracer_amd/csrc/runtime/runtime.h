@@ -36,10 +36,6 @@ constexpr int kLanes = CT_LANES;
 #endif
 constexpr int kTailAfterRoulette = CT_TAIL_AFTER_ROULETTE;
 constexpr int kTailBeforeEnd = CT_TAIL_BEFORE_END;
-// Traversal data (triangle records and the larger tree) up to which a scene
-// counts as cache-resident (half the 256 MB Infinity Cache): its traversal
-// grids fill 5/8 of the wave slots (LaunchShape::trace_fill).
-constexpr uint64_t kResidentSceneBytes = 128ull << 20;
 static_assert(kLanes >= 2, "at least two lanes");
 
 // One of the pipelines the chunk driver rotates through.  Each lane
