@@ -107,8 +107,9 @@ struct Segment {
 
 // Pixel of a view's pixel ordinal (path index / spp).  Ordinals run down
 // strips of kTileRows rows column by column, so the 64 primary rays of a
-// wave cover a 16 x 4 tile and share more of the tree than a run of one
-// row (4 rows measured ahead of 1, 8 and 16 on the 7.24M-triangle 4K scene).
+// wave cover a 32 x 2 tile and share more of the tree than a run of one
+// row.  On the 7.24M-triangle 4K scene 2 rows ran 1.5% ahead of 4 (which had
+// been ahead of 1, 8 and 16); the 259K-triangle atrium is indifferent.
 // The n full strips are taken in the order strip = k * mul mod n (mul
 // coprime to n, about 0.618 n), so any run of ordinals, such as one
 // wavefront chunk, covers strips spread over the whole image: a frame's
@@ -118,7 +119,7 @@ struct Segment {
 // bijection of [0, w * h); seeds and the accumulator use the pixel itself,
 // so the order changes no result.
 #ifndef CT_TILE_ROWS
-#define CT_TILE_ROWS 4
+#define CT_TILE_ROWS 2
 #endif
 constexpr uint32_t kTileRows = CT_TILE_ROWS;
 CT_DEV uint32_t pixel_of(uint32_t ord, uint32_t w, uint32_t h, uint32_t mul) {

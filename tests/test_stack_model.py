@@ -163,7 +163,7 @@ def _strip_stride(n: int) -> int:
     return 1
 
 
-def _pixel_of(o, w: int, h: int, mul: int, rows_per_strip: int = 4):
+def _pixel_of(o, w: int, h: int, mul: int, rows_per_strip: int = 2):
     """pixel_of of device/dev_common.h over an array of ordinals."""
     o = np.asarray(o, dtype=np.int64)
     k, r = np.divmod(o, rows_per_strip * w)
@@ -175,15 +175,17 @@ def _pixel_of(o, w: int, h: int, mul: int, rows_per_strip: int = 4):
     return (y0 + dy) * w + x
 
 
+@pytest.mark.parametrize("rows", [2, 4])
 @pytest.mark.parametrize("w,h", [(1, 1), (1, 7), (5, 3), (33, 17), (64, 6), (37, 23), (256, 256), (480, 270),
                                  (1920, 1080), (3840, 135), (3840, 2160), (7, 4 * 97 + 3)])
-def test_pixel_order_is_a_bijection_and_spreads_chunks(w, h):
-    """pixel_of (device/dev_common.h) maps the ordinals [0, w h) onto the
-    pixels one to one for every size; on frames of 256 rows and more each
-    half of the ordinals (a frame's two wavefront chunks) covers rows spread
-    over the whole image, so the chunks carry similar work."""
-    mul = _strip_stride(h // 4)
-    pix = _pixel_of(np.arange(w * h), w, h, mul)
+def test_pixel_order_is_a_bijection_and_spreads_chunks(w, h, rows):
+    """pixel_of (device/dev_common.h, strips of 2 rows; 4 checked too) maps
+    the ordinals [0, w h) onto the pixels one to one for every size; on
+    frames of 256 rows and more each half of the ordinals (a frame's two
+    wavefront chunks) covers rows spread over the whole image, so the chunks
+    carry similar work."""
+    mul = _strip_stride(h // rows)
+    pix = _pixel_of(np.arange(w * h), w, h, mul, rows)
     assert np.array_equal(np.sort(pix), np.arange(w * h))
     if h >= 256:
         for half in (pix[: (w * h) // 2], pix[(w * h) // 2:]):
