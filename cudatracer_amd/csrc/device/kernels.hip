@@ -509,11 +509,13 @@ CT_DEV void shade_one(const DevScene& sc, const ShadeOpts& so, const ViewDesc* _
 
 // shade (Kernel.cu:88-206) for queue entries [0, active[bounce]); see the
 // oracle's path_sample for the same steps on the CPU.  SORT: shade in
-// block-local material order (ct_frame_opts.sort_by_material).  Five waves
-// per SIMD (at most 96 VGPRs, no spills) measured 1.3% faster per frame than
-// the four the unconstrained allocation leaves.
+// block-local material order (ct_frame_opts.sort_by_material).  Register
+// budgets, each the largest occupancy without spills: sorted, five waves per
+// SIMD (at most 96 VGPRs; 1.3% faster per frame than the four the
+// unconstrained allocation leaves, and six spill); unsorted, six (at most 80
+// VGPRs; on the 7.24M-triangle 4K scene 0.8% faster per frame than five).
 template <bool SORT>
-__global__ void __launch_bounds__(kShadeBlock) __attribute__((amdgpu_waves_per_eu(5)))
+__global__ void __launch_bounds__(kShadeBlock) __attribute__((amdgpu_waves_per_eu(SORT ? 5 : 6)))
 k_shade(DevScene sc, ShadeOpts so, const ViewDesc* __restrict__ views, const Segment* __restrict__ segs,
         const ChunkHdr* hdr, Queues q, int sel, int bounce) {
     __shared__ uint32_t s_hist[kSortKeys];
