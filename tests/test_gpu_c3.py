@@ -8,7 +8,10 @@ triangles) on a noisy ground (2M), 50/50 Beckmann/GGX, one small light,
 - a 3840x135 band through the middle of the frame: frame 0 bit-identical to
   the FP32 oracle (Kernel.cu:88-256 restated, with the documented
   microfacet, roulette and MIS decisions), 16 frames per pixel within 1e-4
-  relative, and the 16-frame mean within rel-MSE 1e-3 of the FP64 oracle.
+  relative, and the 16-frame mean within rel-MSE 1e-3 of the FP64 oracle;
+- the whole frame bit-identical under every schedule of its chunks (one
+  lane at a time, four chunks, the counting frame without the roulette-tail
+  launch), with equal ray counts.
 """
 from __future__ import annotations
 
@@ -143,3 +146,62 @@ def test_c3_radiance_band_depth16_roulette(ct, orc, gpu_ctx, c3):
     print(f"C3 band rel-MSE of the 16-frame mean vs FP64: {rm:.3e}; nonzero pixels {int(np.sum(np.any(x > 0, 2)))}")
     assert rm <= 1e-3
     assert np.sum(np.any(x > 0, 2)) > rows * W // 10
+
+
+def test_c3_full_frame_same_under_every_schedule(ct, gpu_ctx, c3):
+    """The whole 3840x2160 frame (8.29M paths, where the oracle would take
+    minutes) is a function of the scene, camera, options and frame index
+    alone.  It comes out bit-identical, with equal primary, extension and
+    shadow ray counts, under every way the library schedules it:
+    - two chunks on two lanes, with per-kernel timing events;
+    - the same without events;
+    - chunks one at a time (ct_trace_frame_serial);
+    - the counting frame (ct_trace_frame_counted), which traces every bounce
+      in its own instrumented launch instead of the roulette-tail launch;
+    - a context whose chunks hold 2^21 paths (four chunks, two per lane).
+    The middle band of the same frame is pinned bit for bit to the oracle
+    by test_c3_radiance_band_depth16_roulette."""
+    scene, _, _ = c3
+    gpu_ctx.upload_build(scene)
+    opts, W, H = scene.frame_opts()
+    cam = scene.camera(W, H)
+    gpu_ctx.resize(W, H)
+
+    def counts(st):
+        return (st.primary_rays, st.extension_rays, st.shadow_rays)
+
+    gpu_ctx.reset()
+    st0 = gpu_ctx.trace_frame(cam, opts)
+    ref, s = gpu_ctx.read_accum()
+    assert s == 1 and st0.primary_rays == W * H and st0.chunks == 2
+    print(f"C3 full frame: rays {counts(st0)}", flush=True)
+    assert np.all(np.isfinite(ref)) and float(np.max(ref[..., 3])) > 0.0
+
+    def same(label, st):
+        acc, s = gpu_ctx.read_accum()
+        diff = int(np.sum(acc.view(np.uint32) != ref.view(np.uint32)))
+        print(f"C3 full frame, {label}: rays {counts(st)}, words differing {diff}", flush=True)
+        assert s == 1 and diff == 0 and counts(st) == counts(st0)
+
+    gpu_ctx.reset()
+    gpu_ctx.trace_frame(cam, opts, stats=False)
+    acc, _ = gpu_ctx.read_accum()
+    assert np.array_equal(acc.view(np.uint32), ref.view(np.uint32))
+    gpu_ctx.reset()
+    same("chunks one at a time", gpu_ctx.trace_frame_serial(cam, opts))
+    gpu_ctx.reset()
+    st, work = gpu_ctx.trace_frame_counted(cam, opts)
+    assert min(work) > 0
+    same("counting frame, per-bounce launches", st)
+
+    four = ct.Context(0, chunk_paths=1 << 21)
+    try:
+        four.upload_build(scene)
+        four.resize(W, H)
+        st4 = four.trace_frame(cam, opts)
+        acc, s = four.read_accum()
+        diff = int(np.sum(acc.view(np.uint32) != ref.view(np.uint32)))
+        print(f"C3 full frame, {st4.chunks} chunks: rays {counts(st4)}, words differing {diff}", flush=True)
+        assert st4.chunks == 4 and s == 1 and diff == 0 and counts(st4) == counts(st0)
+    finally:
+        four.close()
