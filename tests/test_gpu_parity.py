@@ -136,6 +136,64 @@ def test_compact_frame_c2(ct, gpu_ctx, c2):
         gpu_ctx.trace_frame(cam, o)
 
 
+def test_c2_full_frame_same_under_every_schedule(ct, gpu_ctx, c2):
+    """The headline frame (config 2, 1920x1080, depth 8) in full: a band of
+    eight rows through its middle bit-identical to the FP32 oracle, and the
+    whole frame bit-identical, with equal primary, extension and shadow ray
+    counts, under every way the library schedules it: two chunks on two
+    lanes with and without timing events, chunks one at a time
+    (ct_trace_frame_serial), the instrumented counting frame
+    (ct_trace_frame_counted), shading without the block-local material sort,
+    and a context whose chunks hold 2^18 paths (eight chunks, four per
+    lane)."""
+    scene, _, osc = c2
+    _upload(gpu_ctx, scene)
+    opts, w, h = scene.frame_opts()
+    assert (w, h, opts.max_depth, opts.sort_by_material) == (1920, 1080, 8, 1)
+    cam = scene.camera(w, h)
+    gpu_ctx.resize(w, h)
+
+    def counts(st):
+        return (st.primary_rays, st.extension_rays, st.shadow_rays)
+
+    gpu_ctx.reset()
+    st0 = gpu_ctx.trace_frame(cam, opts)
+    ref = gpu_ctx.read_accum()[0].copy()
+    assert st0.primary_rays == w * h and st0.chunks == 2
+    rows, r0 = 8, (h - 8) // 2
+    band = np.zeros((h, w, 4), np.float32)
+    osc.render(cam, opts, w, h, 0, 1, rows=(r0, r0 + rows), threads=16, accum=band)
+    assert np.array_equal(ref[r0:r0 + rows].view(np.uint32), band[r0:r0 + rows].view(np.uint32))
+    print(f"C2 full frame: rays {counts(st0)}; rows {r0}..{r0 + rows - 1} bit-identical to the oracle", flush=True)
+
+    def same(label, ctx, st):
+        acc, s = ctx.read_accum()
+        diff = int(np.sum(acc.view(np.uint32) != ref.view(np.uint32)))
+        print(f"C2 full frame, {label}: rays {counts(st)}, words differing {diff}", flush=True)
+        assert s == 1 and diff == 0 and counts(st) == counts(st0)
+
+    gpu_ctx.reset()
+    gpu_ctx.trace_frame(cam, opts, stats=False)
+    assert np.array_equal(gpu_ctx.read_accum()[0].view(np.uint32), ref.view(np.uint32))
+    gpu_ctx.reset()
+    same("chunks one at a time", gpu_ctx, gpu_ctx.trace_frame_serial(cam, opts))
+    gpu_ctx.reset()
+    same("counting frame", gpu_ctx, gpu_ctx.trace_frame_counted(cam, opts)[0])
+    o = opts.copy()
+    o.sort_by_material = 0
+    gpu_ctx.reset()
+    same("unsorted shading", gpu_ctx, gpu_ctx.trace_frame(cam, o))
+    eight = ct.Context(0, chunk_paths=1 << 18)
+    try:
+        eight.upload(scene)
+        eight.resize(w, h)
+        st = eight.trace_frame(cam, opts)
+        assert st.chunks == 8
+        same("8 chunks", eight, st)
+    finally:
+        eight.close()
+
+
 @pytest.mark.parametrize("config,accel", [(1, 0), (1, 1), (2, 1)])
 def test_shadow_any_hit(ct, orc, gpu_ctx, config, accel):
     from conftest import get_scene
