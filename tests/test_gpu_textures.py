@@ -188,3 +188,49 @@ def test_texture_index_checks(ct, gpu_ctx, c1):
         _upload(gpu_ctx, _View(ct, arrays, [t], sky=1))
     # a zeroed sky_texture without textures keeps the constant sky
     _upload(gpu_ctx, _View(ct, arrays, (), None, sky=0))
+
+
+def test_textured_atrium_full_frame_schedules(ct, orc, gpu_ctx):
+    """Config 4 at its full size (1920x1080, depth 8): eight middle rows
+    bit-identical to the FP32 oracle, and the whole frame bit-identical, with
+    equal ray counts, under chunks one at a time, the counting frame and a
+    context of eight chunks; texture fetches depend on nothing but the hit."""
+    scene = ct.HostScene.generate(4, 7)
+    osc = orc.Scene(scene.arrays())
+    gpu_ctx.upload(scene)
+    opts, w, h = scene.frame_opts()
+    assert (w, h) == (1920, 1080)
+    cam = scene.camera(w, h)
+    gpu_ctx.resize(w, h)
+
+    def counts(st):
+        return (st.primary_rays, st.extension_rays, st.shadow_rays)
+
+    gpu_ctx.reset()
+    st0 = gpu_ctx.trace_frame(cam, opts)
+    ref = gpu_ctx.read_accum()[0].copy()
+    rows, r0 = 8, (h - 8) // 2
+    band = np.zeros((h, w, 4), np.float32)
+    osc.render(cam, opts, w, h, 0, 1, rows=(r0, r0 + rows), threads=16, accum=band)
+    assert np.array_equal(_bits(ref[r0:r0 + rows]), _bits(band[r0:r0 + rows]))
+    print(f"textured atrium full frame: rays {counts(st0)}; rows {r0}..{r0 + rows - 1} bit-identical", flush=True)
+
+    def same(label, ctx, st):
+        acc, s = ctx.read_accum()
+        diff = int(np.sum(_bits(acc) != _bits(ref)))
+        print(f"textured atrium full frame, {label}: words differing {diff}", flush=True)
+        assert s == 1 and diff == 0 and counts(st) == counts(st0)
+
+    gpu_ctx.reset()
+    same("chunks one at a time", gpu_ctx, gpu_ctx.trace_frame_serial(cam, opts))
+    gpu_ctx.reset()
+    same("counting frame", gpu_ctx, gpu_ctx.trace_frame_counted(cam, opts)[0])
+    eight = ct.Context(0, chunk_paths=1 << 18)
+    try:
+        eight.upload(scene)
+        eight.resize(w, h)
+        st = eight.trace_frame(cam, opts)
+        assert st.chunks == 8
+        same("8 chunks", eight, st)
+    finally:
+        eight.close()
